@@ -1,0 +1,160 @@
+"""ctypes binding of libhydrant_hip.so (include/hydrant_hip.h, include/hydrant_dataset.h).
+
+The library is built in-tree by ``hydrant_amd.build.build_native`` (or
+``__graft_entry__.build()``).  Loading fails loudly when it is missing: there is
+no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libhydrant_hip.so"
+
+HYD_OK, HYD_ERR_ARG, HYD_ERR_HIP, HYD_ERR_STATE, HYD_ERR_RANGE = 0, 1, 2, 3, 4
+FMT_U8, FMT_F16, FMT_F32 = 0, 1, 2
+MODE_DEFAULT, MODE_PARTITION, MODE_PAGING = 0, 1, 2
+SCHED_REFERENCE, SCHED_LATTICE = 0, 1
+BLOCKS_VOLUME, BLOCKS_SOLID = 0, 1
+ISO_COLOR, ISO_NORMAL, ISO_POSITION = 0, 1, 2
+
+f3 = C.c_float * 3
+f16 = C.c_float * 16
+i3 = C.c_int32 * 3
+
+
+class View(C.Structure):
+    _fields_ = [("trans", f16), ("ctg_fovy_2", C.c_float), ("ray_o", f3)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", f3), ("target", f3), ("up", f3), ("fovy", C.c_float)]
+
+
+class VolumeParams(C.Structure):
+    _fields_ = [
+        ("bbox_min", f3), ("bbox_max", f3), ("step", C.c_float), ("du", C.c_float),
+        ("max_steps", C.c_int32), ("mode", C.c_int32), ("rank", C.c_float), ("density", C.c_float),
+        ("schedule", C.c_int32), ("ert", C.c_float),
+    ]
+
+
+class VolumePixel(C.Structure):
+    _fields_ = [("o", f3), ("d", f3), ("nsteps", C.c_int32), ("step_size", C.c_int32),
+                ("v", C.c_float * 4), ("theta", f3), ("phi", C.c_float)]
+
+
+class FetchPixel(C.Structure):
+    _fields_ = [("theta", f3), ("phi", C.c_float), ("val", C.c_float * 4)]
+
+
+class TraceRec(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("ix", "iy", "iz", "pgid", "tix", "tiy", "tiz", "tjx", "tjy", "tjz")]
+
+
+class BlocksParams(C.Structure):
+    _fields_ = [("bbox_min", f3), ("bbox_max", f3), ("step", C.c_float), ("max_steps", C.c_int32),
+                ("mode", C.c_int32), ("density", C.c_float)]
+
+
+class IsoParams(C.Structure):
+    _fields_ = [
+        ("bbox_min", f3), ("bbox_max", f3), ("step", C.c_float), ("du", C.c_float),
+        ("max_steps", C.c_int32), ("mode", C.c_int32), ("to_world", f16), ("eye_pos", f3),
+        ("light_pos", f3), ("surface_color", f3), ("isovalue", C.c_float), ("corrected", C.c_int32),
+        ("bisect_steps", C.c_int32),
+    ]
+
+
+class IsoPixel(C.Structure):
+    _fields_ = [("v", C.c_float * 4), ("depth", C.c_float), ("hit", C.c_int32)]
+
+
+P = C.c_void_p
+I32 = C.c_int32
+PI32 = C.POINTER(C.c_int32)
+PF = C.POINTER(C.c_float)
+
+# name -> (restype, argtypes); every symbol the public headers declare.
+SIGNATURES = {
+    # hydrant_hip.h
+    "hyd_last_error": (C.c_char_p, []),
+    "hyd_version": (I32, []),
+    "hyd_create": (I32, [I32, C.POINTER(P)]),
+    "hyd_destroy": (I32, [P]),
+    "hyd_set_stream": (I32, [P, P]),
+    "hyd_synchronize": (I32, [P]),
+    "hyd_orbit_camera": (I32, [PF, PF, C.c_float, C.POINTER(Camera)]),
+    "hyd_make_view": (I32, [C.POINTER(Camera), PF, PF, C.POINTER(View)]),
+    "hyd_exhibit_iet": (I32, [PF, PF, PF]),
+    "hyd_mat4_inverse": (I32, [PF, PF]),
+    "hyd_upload_pool": (I32, [P, I32, I32, I32, P]),
+    "hyd_bind_pool_device": (I32, [P, I32, I32, I32, P]),
+    "hyd_set_page_table": (I32, [P, PI32, PI32, I32, PF, PF, I32]),
+    "hyd_set_empty_grid": (I32, [P, PI32, PI32]),
+    "hyd_set_mean_grid": (I32, [P, PI32, PF]),
+    "hyd_set_tf": (I32, [P, I32, PF]),
+    "hyd_render_volume": (I32, [P, C.POINTER(VolumeParams), C.POINTER(View), I32, I32, P, P, P]),
+    "hyd_resume_volume": (I32, [P, C.POINTER(VolumeParams), I32, I32, P, P, P]),
+    "hyd_pixel_pass": (I32, [P, P, I32, I32, C.POINTER(C.c_uint8), P]),
+    "hyd_fetch_pass": (I32, [P, P, I32, I32, P]),
+    "hyd_composite": (I32, [P, I32, P, PI32, I32, I32, P, P]),
+    "hyd_trace_volume": (I32, [P, C.POINTER(VolumeParams), C.POINTER(View), I32, I32, I32, I32,
+                               C.POINTER(TraceRec), I32, PI32]),
+    "hyd_render_blocks": (I32, [P, C.POINTER(BlocksParams), C.POINTER(View), I32, I32, P]),
+    "hyd_render_iso": (I32, [P, C.POINTER(IsoParams), C.POINTER(View), I32, I32, P, P]),
+    # hydrant_dataset.h
+    "hyd_gen_blobs_u8": (I32, [I32, I32, I32, I32, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                               C.c_double, C.c_double, P]),
+    "hyd_gen_perlin_u8": (I32, [I32, I32, C.c_float, C.c_float, C.c_float, C.c_uint64, P]),
+    "hyd_gen_sdf_spheres_f16": (I32, [I32, I32, C.c_float, C.c_float, C.c_float, C.c_uint64, P]),
+    "hyd_f32_to_f16": (C.c_uint16, [C.c_float]),
+    "hyd_f16_to_f32": (C.c_float, [C.c_uint16]),
+    "hyd_pack_bricks": (I32, [P, I32, PI32, I32, I32, I32, PI32, I32, P]),
+    "hyd_brick_stats": (I32, [P, I32, PI32, I32, I32, PF, PF, PF]),
+    "hyd_tf_alpha_zero_on": (I32, [PF, I32, C.c_float, C.c_float]),
+    "hyd_chebyshev": (I32, [PI32, C.POINTER(C.c_uint8), PI32]),
+    "hyd_seeded_page_table": (I32, [PI32, C.POINTER(C.c_uint8), I32, C.c_uint64, PI32, PI32, PI32]),
+}
+
+_lib = None
+
+
+class HydError(RuntimeError):
+    """A libhydrant_hip call returned a non-zero status."""
+
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed with status {code}: {msg}")
+        self.code = code
+
+
+def lib() -> C.CDLL:
+    """Load libhydrant_hip.so (built in-tree); raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                "(the HIP path has no fallback)")
+        handle = C.CDLL(str(LIB_PATH), mode=C.RTLD_GLOBAL)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(fn: str, code: int) -> None:
+    if code != HYD_OK:
+        msg = lib().hyd_last_error().decode(errors="replace")
+        raise HydError(fn, code, msg)
+
+
+def call(fn: str, *args) -> None:
+    check(fn, getattr(lib(), fn)(*args))
+
+
+def ptr(arr, ctype):
+    """Pointer to a contiguous numpy array's data as POINTER(ctype)."""
+    return arr.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,432 @@
+// C ABI implementation: context, resident state and pass dispatch.
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "hyd_internal.hpp"
+
+namespace hyd {
+
+namespace {
+thread_local char g_err[1024] = "";
+}
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+namespace {
+
+constexpr int kMaxTf = 256;
+
+int fmt_bytes(int fmt) { return fmt == HYD_FMT_U8 ? 1 : fmt == HYD_FMT_F16 ? 2 : fmt == HYD_FMT_F32 ? 4 : 0; }
+
+template <class T> void free_dev(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+int32_t activate(Ctx *c) {
+  HYD_REQUIRE(c, HYD_ERR_ARG, "null context");
+  HYD_CHECK_HIP(hipSetDevice(c->device));
+  return HYD_OK;
+}
+
+template <class T> int32_t upload(T *&dst, const void *src, size_t bytes, hipStream_t s) {
+  free_dev(dst);
+  HYD_CHECK_HIP(hipMalloc((void **)&dst, bytes));
+  HYD_CHECK_HIP(hipMemcpyAsync((void *)dst, src, bytes, hipMemcpyHostToDevice, s));
+  HYD_CHECK_HIP(hipStreamSynchronize(s));
+  return HYD_OK;
+}
+
+int32_t check_volume_state(const Ctx *c) {
+  HYD_REQUIRE(c->pool.data, HYD_ERR_STATE, "no brick pool: call hyd_upload_pool first");
+  HYD_REQUIRE(c->paging.vaddr, HYD_ERR_STATE, "no page table: call hyd_set_page_table first");
+  HYD_REQUIRE(c->cheb.data, HYD_ERR_STATE, "no empty-space grid: call hyd_set_empty_grid first");
+  HYD_REQUIRE(c->tf, HYD_ERR_STATE, "no transfer function: call hyd_set_tf first");
+  HYD_REQUIRE(c->paging.n_pages <= c->pool.n_pages, HYD_ERR_STATE,
+              "page table names %d pages but the pool holds %d", c->paging.n_pages, c->pool.n_pages);
+  return HYD_OK;
+}
+
+int32_t fill_volume_args(const Ctx *c, const hyd_volume_params *p, VolumeArgs &a) {
+  HYD_REQUIRE(p, HYD_ERR_ARG, "null params");
+  HYD_REQUIRE(p->step > 0.f, HYD_ERR_ARG, "step must be > 0");
+  HYD_REQUIRE(p->schedule == HYD_SCHED_REFERENCE || p->schedule == HYD_SCHED_LATTICE, HYD_ERR_ARG,
+              "unknown schedule %d", p->schedule);
+  HYD_REQUIRE(p->mode >= HYD_MODE_DEFAULT && p->mode <= HYD_MODE_PAGING, HYD_ERR_ARG, "unknown mode %d", p->mode);
+  std::memset(&a, 0, sizeof(a));
+  a.bmin = f3{p->bbox_min[0], p->bbox_min[1], p->bbox_min[2]};
+  a.bmax = f3{p->bbox_max[0], p->bbox_max[1], p->bbox_max[2]};
+  a.step = p->step;
+  a.max_steps = p->max_steps;
+  a.mode = p->mode;
+  a.rank = p->rank;
+  a.density = p->density;
+  a.schedule = p->schedule;
+  a.ert = p->ert;
+  a.tf = c->tf;
+  a.tf_n = c->tf_n;
+  a.cheb = c->cheb.data;
+  std::memcpy(a.cdim, c->cheb.dim, sizeof(a.cdim));
+  a.vaddr = c->paging.vaddr;
+  std::memcpy(a.vdim, c->paging.vdim, sizeof(a.vdim));
+  a.lowest_blkcnt = c->paging.lowest_blkcnt;
+  a.page = c->paging.page;
+  a.pool = c->pool.data;
+  a.pad = c->pool.pad;
+  a.u8_lut = c->u8_lut;
+  return HYD_OK;
+}
+
+void fill_view(const hyd_view *v, float trans[16], float &ctg, f3 &ray_o) {
+  std::memcpy(trans, v->trans, 16 * sizeof(float));
+  ctg = v->ctg_fovy_2;
+  ray_o = f3{v->ray_o[0], v->ray_o[1], v->ray_o[2]};
+}
+
+}  // namespace
+}  // namespace hyd
+
+using namespace hyd;
+
+struct hyd_ctx : Ctx {};
+
+extern "C" {
+
+const char *hyd_last_error(void) { return g_err; }
+
+int32_t hyd_version(void) { return 1; }
+
+int32_t hyd_create(int32_t device, hyd_ctx **out) {
+  HYD_REQUIRE(out, HYD_ERR_ARG, "hyd_create: null out");
+  *out = nullptr;
+  int n = 0;
+  HYD_CHECK_HIP(hipGetDeviceCount(&n));
+  HYD_REQUIRE(device >= 0 && device < n, HYD_ERR_ARG, "device %d out of range (%d devices)", device, n);
+  HYD_CHECK_HIP(hipSetDevice(device));
+  hyd_ctx *c = new hyd_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    set_error("hipStreamCreateWithFlags failed");
+    return HYD_ERR_HIP;
+  }
+  c->stream = c->own_stream;
+  // float(v) / 255.f for every u8 value: the normalisation Texture3D<u8>
+  // applies on upload (texture_3d.hpp:78-86, glm_math.hpp:83-89).
+  float lut[256];
+  for (int i = 0; i < 256; ++i) lut[i] = float(i) / 255.f;
+  int32_t st = upload(c->own_lut, lut, sizeof(lut), c->stream);
+  if (st != HYD_OK) { (void)hipStreamDestroy(c->own_stream); delete c; return st; }
+  c->u8_lut = c->own_lut;
+  *out = c;
+  return HYD_OK;
+}
+
+int32_t hyd_destroy(hyd_ctx *c) {
+  if (!c) return HYD_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_dev(c->own_pool);
+  free_dev(c->own_vaddr);
+  free_dev(c->own_page);
+  free_dev(c->own_cheb);
+  free_dev(c->own_mean);
+  free_dev(c->own_tf);
+  free_dev(c->own_lut);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+  return HYD_OK;
+}
+
+int32_t hyd_set_stream(hyd_ctx *c, void *stream) {
+  HYD_REQUIRE(c, HYD_ERR_ARG, "null context");
+  c->stream = static_cast<hipStream_t>(stream);  // NULL = the device's null stream
+  return HYD_OK;
+}
+
+int32_t hyd_synchronize(hyd_ctx *c) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_upload_pool(hyd_ctx *c, int32_t fmt, int32_t pad_edge, int32_t n_pages, const void *host) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(fmt_bytes(fmt) > 0, HYD_ERR_ARG, "unknown brick format %d", fmt);
+  HYD_REQUIRE(pad_edge > 0 && pad_edge <= 1024, HYD_ERR_ARG, "pad_edge %d out of range", pad_edge);
+  HYD_REQUIRE(n_pages > 0 && host, HYD_ERR_ARG, "empty pool");
+  const size_t bytes = (size_t)n_pages * pad_edge * pad_edge * pad_edge * fmt_bytes(fmt);
+  st = upload(c->own_pool, host, bytes, c->stream);
+  if (st) return st;
+  c->pool = Pool{c->own_pool, fmt, pad_edge, n_pages, true};
+  return HYD_OK;
+}
+
+int32_t hyd_bind_pool_device(hyd_ctx *c, int32_t fmt, int32_t pad_edge, int32_t n_pages, const void *pool_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(fmt_bytes(fmt) > 0, HYD_ERR_ARG, "unknown brick format %d", fmt);
+  HYD_REQUIRE(pad_edge > 0 && pad_edge <= 1024 && n_pages > 0 && pool_dev, HYD_ERR_ARG, "invalid pool");
+  free_dev(c->own_pool);
+  c->pool = Pool{pool_dev, fmt, pad_edge, n_pages, false};
+  return HYD_OK;
+}
+
+int32_t hyd_set_page_table(hyd_ctx *c, const int32_t dims[3], const int32_t *vaddr, int32_t n_pages,
+                           const float *page_k, const float *page_b, int32_t lowest_blkcnt) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(dims && vaddr && page_k && page_b && n_pages > 0, HYD_ERR_ARG, "invalid page table");
+  HYD_REQUIRE(dims[0] > 0 && dims[1] > 0 && dims[2] > 0, HYD_ERR_ARG, "page table dims must be > 0");
+  const size_t cells = (size_t)dims[0] * dims[1] * dims[2];
+  for (size_t i = 0; i < cells; ++i)
+    HYD_REQUIRE(vaddr[i] >= -1 && vaddr[i] < n_pages, HYD_ERR_RANGE, "vaddr[%zu] = %d outside [-1, %d)", i,
+                vaddr[i], n_pages);
+  std::vector<float4> pages(n_pages);
+  for (int i = 0; i < n_pages; ++i)
+    pages[i] = make_float4(page_k[i], page_b[3 * i], page_b[3 * i + 1], page_b[3 * i + 2]);
+  st = upload(c->own_vaddr, vaddr, cells * sizeof(int32_t), c->stream);
+  if (st) return st;
+  st = upload(c->own_page, pages.data(), pages.size() * sizeof(float4), c->stream);
+  if (st) return st;
+  c->paging.vaddr = c->own_vaddr;
+  std::memcpy(c->paging.vdim, dims, 3 * sizeof(int32_t));
+  c->paging.page = c->own_page;
+  c->paging.n_pages = n_pages;
+  c->paging.lowest_blkcnt = lowest_blkcnt;
+  return HYD_OK;
+}
+
+int32_t hyd_set_empty_grid(hyd_ctx *c, const int32_t dims[3], const int32_t *chebyshev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(dims && chebyshev && dims[0] > 0 && dims[1] > 0 && dims[2] > 0, HYD_ERR_ARG, "invalid empty grid");
+  const size_t cells = (size_t)dims[0] * dims[1] * dims[2];
+  for (size_t i = 0; i < cells; ++i)
+    HYD_REQUIRE(chebyshev[i] >= 0, HYD_ERR_RANGE, "chebyshev[%zu] = %d is negative", i, chebyshev[i]);
+  st = upload(c->own_cheb, chebyshev, cells * sizeof(int32_t), c->stream);
+  if (st) return st;
+  c->cheb.data = c->own_cheb;
+  std::memcpy(c->cheb.dim, dims, 3 * sizeof(int32_t));
+  return HYD_OK;
+}
+
+int32_t hyd_set_mean_grid(hyd_ctx *c, const int32_t dims[3], const float *mean) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(dims && mean && dims[0] > 0 && dims[1] > 0 && dims[2] > 0, HYD_ERR_ARG, "invalid mean grid");
+  const size_t cells = (size_t)dims[0] * dims[1] * dims[2];
+  st = upload(c->own_mean, mean, cells * sizeof(float), c->stream);
+  if (st) return st;
+  c->mean.data = c->own_mean;
+  std::memcpy(c->mean.dim, dims, 3 * sizeof(int32_t));
+  return HYD_OK;
+}
+
+int32_t hyd_set_tf(hyd_ctx *c, int32_t n, const float *rgba) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(rgba && n >= 1, HYD_ERR_ARG, "invalid transfer function");
+  HYD_REQUIRE(n <= kMaxTf, HYD_ERR_RANGE, "transfer function has %d entries; the limit is %d", n, kMaxTf);
+  st = upload(c->own_tf, rgba, (size_t)n * 4 * sizeof(float), c->stream);
+  if (st) return st;
+  c->tf = c->own_tf;
+  c->tf_n = n;
+  return HYD_OK;
+}
+
+int32_t hyd_render_volume(hyd_ctx *c, const hyd_volume_params *p, const hyd_view *view, int32_t width,
+                          int32_t height, hyd_volume_pixel *film_dev, float *rgba_dev,
+                          unsigned long long *samples_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  if ((st = check_volume_state(c))) return st;
+  HYD_REQUIRE(view && width > 0 && height > 0, HYD_ERR_ARG, "invalid view or image size");
+  VolumeArgs a;
+  if ((st = fill_volume_args(c, p, a))) return st;
+  fill_view(view, a.trans, a.ctg, a.ray_o);
+  a.W = width;
+  a.H = height;
+  a.film = film_dev;
+  a.rgba = reinterpret_cast<float4 *>(rgba_dev);
+  a.samples = samples_dev;
+  HYD_CHECK_HIP(launch_volume(a, c->pool.fmt, false, c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_resume_volume(hyd_ctx *c, const hyd_volume_params *p, int32_t width, int32_t height,
+                          hyd_volume_pixel *film_dev, float *rgba_dev, unsigned long long *samples_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  if ((st = check_volume_state(c))) return st;
+  HYD_REQUIRE(film_dev && width > 0 && height > 0, HYD_ERR_ARG, "resume needs a film");
+  VolumeArgs a;
+  if ((st = fill_volume_args(c, p, a))) return st;
+  a.W = width;
+  a.H = height;
+  a.film = film_dev;
+  a.rgba = reinterpret_cast<float4 *>(rgba_dev);
+  a.samples = samples_dev;
+  HYD_CHECK_HIP(launch_volume(a, c->pool.fmt, true, c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_pixel_pass(hyd_ctx *c, const float *rgba_dev, int32_t width, int32_t height,
+                       const uint8_t clear_color[3], uint8_t *rgb_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(rgba_dev && rgb_dev && width > 0 && height > 0, HYD_ERR_ARG, "invalid pixel pass arguments");
+  uchar3 clear = make_uchar3(0, 0, 0);
+  if (clear_color) clear = make_uchar3(clear_color[0], clear_color[1], clear_color[2]);
+  HYD_CHECK_HIP(launch_pixel_pass(reinterpret_cast<const float4 *>(rgba_dev), width, height, clear, rgb_dev,
+                                  c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_fetch_pass(hyd_ctx *c, const hyd_volume_pixel *film_dev, int32_t width, int32_t height,
+                       hyd_fetch_pixel *out_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(film_dev && out_dev && width > 0 && height > 0, HYD_ERR_ARG, "invalid fetch pass arguments");
+  HYD_CHECK_HIP(launch_fetch_pass(film_dev, width, height, out_dev, c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_composite(hyd_ctx *c, int32_t n, const hyd_fetch_pixel *partials_dev, const int32_t *z_order,
+                      int32_t width, int32_t height, float *rgba_dev, uint8_t *rgb_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(n >= 1 && partials_dev && z_order && width > 0 && height > 0, HYD_ERR_ARG,
+              "invalid composite arguments");
+  std::vector<char> seen(n, 0);
+  for (int i = 0; i < n; ++i) {
+    HYD_REQUIRE(z_order[i] >= 0 && z_order[i] < n && !seen[z_order[i]], HYD_ERR_ARG,
+                "z_order must be a permutation of 0..%d", n - 1);
+    seen[z_order[i]] = 1;
+  }
+  int32_t *zdev = nullptr;
+  HYD_CHECK_HIP(hipMallocAsync((void **)&zdev, n * sizeof(int32_t), c->stream));
+  HYD_CHECK_HIP(hipMemcpyAsync(zdev, z_order, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  hipError_t e = launch_composite(n, partials_dev, zdev, width, height, reinterpret_cast<float4 *>(rgba_dev),
+                                  rgb_dev, c->stream);
+  (void)hipFreeAsync(zdev, c->stream);
+  HYD_CHECK_HIP(e);
+  HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_trace_volume(hyd_ctx *c, const hyd_volume_params *p, const hyd_view *view, int32_t width,
+                         int32_t height, int32_t x, int32_t y, hyd_trace_rec *out, int32_t max_recs,
+                         int32_t *n_out) {
+  int32_t st = activate(c);
+  if (st) return st;
+  if ((st = check_volume_state(c))) return st;
+  HYD_REQUIRE(view && out && n_out && max_recs > 0, HYD_ERR_ARG, "invalid trace arguments");
+  HYD_REQUIRE(x >= 0 && x < width && y >= 0 && y < height, HYD_ERR_ARG, "pixel outside the image");
+  VolumeArgs a;
+  if ((st = fill_volume_args(c, p, a))) return st;
+  HYD_REQUIRE(a.schedule == HYD_SCHED_REFERENCE, HYD_ERR_ARG, "trace supports the reference schedule only");
+  fill_view(view, a.trans, a.ctg, a.ray_o);
+  a.W = width;
+  a.H = height;
+  hyd_trace_rec *dout = nullptr;
+  int *dn = nullptr;
+  HYD_CHECK_HIP(hipMalloc((void **)&dout, (size_t)max_recs * sizeof(hyd_trace_rec)));
+  if (hipMalloc((void **)&dn, sizeof(int)) != hipSuccess) {
+    (void)hipFree(dout);
+    set_error("hipMalloc failed");
+    return HYD_ERR_HIP;
+  }
+  hipError_t e = launch_volume_trace(a, c->pool.fmt, x, y, dout, max_recs, dn, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  int n = 0;
+  if (e == hipSuccess) e = hipMemcpy(&n, dn, sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && n > 0) e = hipMemcpy(out, dout, (size_t)n * sizeof(hyd_trace_rec), hipMemcpyDeviceToHost);
+  (void)hipFree(dout);
+  (void)hipFree(dn);
+  HYD_CHECK_HIP(e);
+  *n_out = n;
+  return HYD_OK;
+}
+
+int32_t hyd_render_blocks(hyd_ctx *c, const hyd_blocks_params *p, const hyd_view *view, int32_t width,
+                          int32_t height, float *rgba_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(p && view && rgba_dev && width > 0 && height > 0, HYD_ERR_ARG, "invalid blocks arguments");
+  HYD_REQUIRE(p->step > 0.f, HYD_ERR_ARG, "step must be > 0");
+  HYD_REQUIRE(c->cheb.data, HYD_ERR_STATE, "no empty-space grid: call hyd_set_empty_grid first");
+  HYD_REQUIRE(p->mode == HYD_BLOCKS_SOLID || c->mean.data, HYD_ERR_STATE,
+              "Blocks.Volume needs a mean grid: call hyd_set_mean_grid first");
+  HYD_REQUIRE(p->mode == HYD_BLOCKS_SOLID ||
+                  std::memcmp(c->mean.dim, c->cheb.dim, sizeof(c->cheb.dim)) == 0,
+              HYD_ERR_STATE, "mean grid and empty-space grid dims differ");
+  BlocksArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.bmin = f3{p->bbox_min[0], p->bbox_min[1], p->bbox_min[2]};
+  a.bmax = f3{p->bbox_max[0], p->bbox_max[1], p->bbox_max[2]};
+  a.step = p->step;
+  a.max_steps = p->max_steps;
+  a.mode = p->mode;
+  a.density = p->density;
+  fill_view(view, a.trans, a.ctg, a.ray_o);
+  a.W = width;
+  a.H = height;
+  a.cheb = c->cheb.data;
+  a.mean = c->mean.data;
+  std::memcpy(a.dim, c->cheb.dim, sizeof(a.dim));
+  a.rgba = reinterpret_cast<float4 *>(rgba_dev);
+  HYD_CHECK_HIP(launch_blocks(a, c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_render_iso(hyd_ctx *c, const hyd_iso_params *p, const hyd_view *view, int32_t width,
+                       int32_t height, hyd_iso_pixel *out_dev, unsigned long long *samples_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(c->pool.data && c->paging.vaddr && c->cheb.data, HYD_ERR_STATE,
+              "isosurface needs pool, page table and empty-space grid");
+  HYD_REQUIRE(p && view && out_dev && width > 0 && height > 0, HYD_ERR_ARG, "invalid isosurface arguments");
+  HYD_REQUIRE(p->step > 0.f, HYD_ERR_ARG, "step must be > 0");
+  HYD_REQUIRE(p->bisect_steps >= 0 && p->bisect_steps <= 64, HYD_ERR_ARG, "bisect_steps out of range");
+  IsoArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.bmin = f3{p->bbox_min[0], p->bbox_min[1], p->bbox_min[2]};
+  a.bmax = f3{p->bbox_max[0], p->bbox_max[1], p->bbox_max[2]};
+  a.step = p->step;
+  a.du = p->du;
+  a.max_steps = p->max_steps;
+  a.mode = p->mode;
+  std::memcpy(a.to_world, p->to_world, sizeof(a.to_world));
+  a.eye_pos = f3{p->eye_pos[0], p->eye_pos[1], p->eye_pos[2]};
+  a.light_pos = f3{p->light_pos[0], p->light_pos[1], p->light_pos[2]};
+  a.surface_color = f3{p->surface_color[0], p->surface_color[1], p->surface_color[2]};
+  a.isovalue = p->isovalue;
+  a.corrected = p->corrected;
+  a.bisect_steps = p->bisect_steps;
+  fill_view(view, a.trans, a.ctg, a.ray_o);
+  a.W = width;
+  a.H = height;
+  a.cheb = c->cheb.data;
+  std::memcpy(a.cdim, c->cheb.dim, sizeof(a.cdim));
+  a.vaddr = c->paging.vaddr;
+  std::memcpy(a.vdim, c->paging.vdim, sizeof(a.vdim));
+  a.page = c->paging.page;
+  a.pool = c->pool.data;
+  a.pad = c->pool.pad;
+  a.u8_lut = c->u8_lut;
+  a.out = out_dev;
+  a.samples = samples_dev;
+  HYD_CHECK_HIP(launch_iso(a, c->pool.fmt, c->stream));
+  return HYD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+// Internal declarations shared by the C ABI implementation and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/hydrant_hip.h"
+#include "hyd_math.hpp"
+
+namespace hyd {
+
+void set_error(const char *fmt, ...);
+
+#define HYD_CHECK_HIP(expr)                                                          \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess) {                                                          \
+      ::hyd::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, \
+                       __LINE__);                                                    \
+      return HYD_ERR_HIP;                                                            \
+    }                                                                                \
+  } while (0)
+
+#define HYD_REQUIRE(cond, code, ...)   \
+  do {                                 \
+    if (!(cond)) {                     \
+      ::hyd::set_error(__VA_ARGS__);   \
+      return (code);                   \
+    }                                  \
+  } while (0)
+
+// Device-resident paging state.  page[i] = {k, b.x, b.y, b.z}.
+struct Paging {
+  const int32_t *vaddr = nullptr;
+  int32_t vdim[3] = {0, 0, 0};
+  const float4 *page = nullptr;
+  int32_t n_pages = 0;
+  int32_t lowest_blkcnt = 0;
+};
+
+struct Pool {
+  const void *data = nullptr;
+  int32_t fmt = -1;
+  int32_t pad = 0;       // padded brick edge
+  int32_t n_pages = 0;
+  bool owned = false;
+};
+
+struct Grid {
+  const int32_t *data = nullptr;
+  int32_t dim[3] = {0, 0, 0};
+};
+
+struct GridF {
+  const float *data = nullptr;
+  int32_t dim[3] = {0, 0, 0};
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  Pool pool;
+  Paging paging;
+  Grid cheb;
+  GridF mean;
+  const float *tf = nullptr;
+  int32_t tf_n = 0;
+  // owned device allocations (freed at destroy)
+  void *own_pool = nullptr;
+  int32_t *own_vaddr = nullptr;
+  float4 *own_page = nullptr;
+  int32_t *own_cheb = nullptr;
+  float *own_mean = nullptr;
+  float *own_tf = nullptr;
+  const float *u8_lut = nullptr;  // 256-entry float(v)/255 table
+  float *own_lut = nullptr;
+};
+
+// Flattened kernel arguments for the volume march (passed by value).
+struct VolumeArgs {
+  // shader
+  f3 bmin, bmax;
+  float step;
+  int32_t max_steps;
+  int32_t mode;
+  float rank;
+  float density;
+  int32_t schedule;
+  float ert;
+  // view
+  float trans[16];
+  float ctg;
+  f3 ray_o;
+  int32_t W, H;
+  // tables
+  const float *tf;
+  int32_t tf_n;
+  const int32_t *cheb;
+  int32_t cdim[3];
+  const int32_t *vaddr;
+  int32_t vdim[3];
+  int32_t lowest_blkcnt;
+  const float4 *page;
+  const void *pool;
+  int32_t pad;
+  const float *u8_lut;
+  // outputs
+  hyd_volume_pixel *film;
+  float4 *rgba;
+  unsigned long long *samples;
+};
+
+hipError_t launch_volume(const VolumeArgs &a, int fmt, bool resume, hipStream_t s);
+hipError_t launch_volume_trace(const VolumeArgs &a, int fmt, int x, int y, hyd_trace_rec *out_dev, int max_recs,
+                               int *n_dev, hipStream_t s);
+hipError_t launch_pixel_pass(const float4 *rgba, int W, int H, uchar3 clear, uint8_t *rgb, hipStream_t s);
+hipError_t launch_fetch_pass(const hyd_volume_pixel *film, int W, int H, hyd_fetch_pixel *out, hipStream_t s);
+hipError_t launch_composite(int n, const hyd_fetch_pixel *partials, const int32_t *z_order_dev, int W, int H,
+                            float4 *rgba, uint8_t *rgb, hipStream_t s);
+
+struct BlocksArgs {
+  f3 bmin, bmax;
+  float step;
+  int32_t max_steps;
+  int32_t mode;
+  float density;
+  float trans[16];
+  float ctg;
+  f3 ray_o;
+  int32_t W, H;
+  const int32_t *cheb;
+  const float *mean;
+  int32_t dim[3];
+  float4 *rgba;
+};
+hipError_t launch_blocks(const BlocksArgs &a, hipStream_t s);
+
+struct IsoArgs {
+  f3 bmin, bmax;
+  float step, du;
+  int32_t max_steps;
+  int32_t mode;
+  float to_world[16];
+  f3 eye_pos, light_pos, surface_color;
+  float isovalue;
+  int32_t corrected, bisect_steps;
+  float trans[16];
+  float ctg;
+  f3 ray_o;
+  int32_t W, H;
+  const int32_t *cheb;
+  int32_t cdim[3];
+  const int32_t *vaddr;
+  int32_t vdim[3];
+  const float4 *page;
+  const void *pool;
+  int32_t pad;
+  const float *u8_lut;
+  hyd_iso_pixel *out;
+  unsigned long long *samples;
+};
+hipError_t launch_iso(const IsoArgs &a, int fmt, hipStream_t s);
+
+}  // namespace hyd

@@ -1,0 +1,272 @@
+// Blocks.Volume / Blocks.Solid and Isosurface kernels for gfx950.
+// One lane per pixel, 8x8 pixel packets per 64-lane workgroup.
+#include <hip/hip_fp16.h>
+
+#include "hyd_internal.hpp"
+
+namespace hyd {
+namespace {
+
+constexpr int kTile = 8;
+
+__device__ __forceinline__ int grid_index(f3 p, const int32_t dim[3]) {
+  float fx = gclamp(p.x, 0.f, float(dim[0]));
+  float fy = gclamp(p.y, 0.f, float(dim[1]));
+  float fz = gclamp(p.z, 0.f, float(dim[2]));
+  int ix = iclamp((int)floorf(fx), 0, dim[0] - 1);
+  int iy = iclamp((int)floorf(fy), 0, dim[1] - 1);
+  int iz = iclamp((int)floorf(fz), 0, dim[2] - 1);
+  return iz * dim[0] * dim[1] + iy * dim[0] + ix;
+}
+
+// skip_nblock_steps (blocks_shader.cu:5-14, isosurface_shader.cu:5-14).
+__device__ __forceinline__ int skip_nblock_steps(f3 &o, f3 d, f3 ip, int nblocks, float cdu, float step) {
+  float tn, tfar;
+  intersect_box(o, d, ip, f3{ip.x + 1.f, ip.y + 1.f, ip.z + 1.f}, tn, tfar);
+  const float di = ceilf((tfar + float(nblocks - 1) * cdu) / step);
+  o = add(o, muls(muls(d, di), step));
+  return (int)di;
+}
+
+// VisBlocksShaderKernel::main, blocks_shader.cu:16-50.
+__global__ __launch_bounds__(64) void blocks_kernel(BlocksArgs a) {
+  const int x = blockIdx.x * kTile + (threadIdx.x % kTile);
+  const int y = blockIdx.y * kTile + (threadIdx.x / kTile);
+  if (x >= a.W || y >= a.H) return;
+  const f3 d = pixel_ray_dir(a.trans, a.ctg, a.ray_o, x, y, a.W, a.H);
+  f3 o = a.ray_o;
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  float tnear, tfar;
+  if (intersect_box(o, d, a.bmin, a.bmax, tnear, tfar)) {
+    if (tnear < 0) tnear = 0;
+    if (tfar < 0) tfar = 0;
+    o = add(o, muls(d, tnear));
+    const int n = (int)((tfar - tnear) / a.step);
+    int nsteps = a.max_steps < n ? a.max_steps : n;
+    const float cdu = 1.f / comp_max(abs3(d));
+    const float opacity_threshold = 0.95f;
+    const f3 hi = f3{a.bmax.x - 1.f, a.bmax.y - 1.f, a.bmax.z - 1.f};
+    const f3 ext = sub(a.bmax, a.bmin);
+    while (nsteps > 0) {
+      const f3 ip = floor3(o);
+      const int gi = grid_index(ip, a.dim);
+      const int cd = a.cheb[gi];
+      if (cd) {
+        nsteps -= skip_nblock_steps(o, d, ip, cd, cdu, a.step);
+      } else {
+        const f3 c = {gclamp(ip.x, a.bmin.x, hi.x), gclamp(ip.y, a.bmin.y, hi.y), gclamp(ip.z, a.bmin.z, hi.z)};
+        const f3 rip = {c.x / ext.x, c.y / ext.y, c.z / ext.z};
+        if (a.mode == HYD_BLOCKS_VOLUME) {
+          const float mean = a.mean[gi];
+          const float4 col = make_float4(rip.x * a.density * mean, rip.y * a.density * mean,
+                                         rip.z * a.density * mean, 1.f * a.density * mean);
+          const float t = 1.f - v.w;
+          v = make_float4(v.x + col.x * t, v.y + col.y * t, v.z + col.z * t, v.w + col.w * t);
+          if (v.w > opacity_threshold) break;
+        } else {
+          v = make_float4(rip.x, rip.y, rip.z, 1.f);
+          break;
+        }
+      }
+      o = add(o, muls(d, a.step));
+      nsteps -= 1;
+    }
+  }
+  a.rgba[(size_t)y * a.W + x] = v;
+}
+
+template <int FMT> __device__ __forceinline__ float vox(const void *pool, size_t i, const float *lut);
+template <> __device__ __forceinline__ float vox<HYD_FMT_F32>(const void *pool, size_t i, const float *) {
+  return static_cast<const float *>(pool)[i];
+}
+template <> __device__ __forceinline__ float vox<HYD_FMT_F16>(const void *pool, size_t i, const float *) {
+  return __half2float(static_cast<const __half *>(pool)[i]);
+}
+template <> __device__ __forceinline__ float vox<HYD_FMT_U8>(const void *pool, size_t i, const float *lut) {
+  return lut[static_cast<const uint8_t *>(pool)[i]];
+}
+
+// BlockSampler::sample_3d (mapping + CpuSampler Wrap/Linear), as in the volume kernel.
+template <int FMT>
+__device__ float iso_sample(const IsoArgs &a, const float *lut, int pgid, f3 x) {
+  const float4 pg = a.page[pgid];
+  const int pad = a.pad;
+  const float fd = float(pad);
+  float u[3] = {x.x * pg.x + pg.y, x.y * pg.x + pg.z, x.z * pg.x + pg.w};
+  float w[3];
+  int i[3], j[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float fx = fd * u[c];
+    fx = fx - fd * floorf(fx / fd);
+    fx = fx - 0.5f;
+    float flr = floorf(fx);
+    w[c] = fx - flr;
+    i[c] = iclamp((int)flr, 0, pad - 1);
+    j[c] = iclamp((int)ceilf(fx), 0, pad - 1);
+  }
+  const size_t base = (size_t)pgid * pad * pad * pad;
+  auto at = [&](int xx, int yy, int zz) { return vox<FMT>(a.pool, base + ((size_t)zz * pad + yy) * pad + xx, lut); };
+  const float bx = 1.f - w[0], by = 1.f - w[1], bz = 1.f - w[2];
+  float x00 = at(i[0], i[1], i[2]) * bx + at(j[0], i[1], i[2]) * w[0];
+  float x10 = at(i[0], j[1], i[2]) * bx + at(j[0], j[1], i[2]) * w[0];
+  float x01 = at(i[0], i[1], j[2]) * bx + at(j[0], i[1], j[2]) * w[0];
+  float x11 = at(i[0], j[1], j[2]) * bx + at(j[0], j[1], j[2]) * w[0];
+  float y0 = x00 * by + x10 * w[1];
+  float y1 = x01 * by + x11 * w[1];
+  return y0 * bz + y1 * w[2];
+}
+
+__device__ __forceinline__ float gsign(float x) { return float((0.f < x) - (x < 0.f)); }
+
+// IsosurfaceShaderKernel::linear_to_srgb: double arithmetic on a float input
+// (isosurface_shader.cu:25-32).
+__device__ __forceinline__ float iso_srgb(float linear) {
+  const double l = linear;
+  if (l <= 0.0031308) return (float)(12.92 * l);
+  return (float)((1.0 + 0.055) * pow(l, 1.0 / 2.4) - 0.055);
+}
+
+// IsosurfaceShaderKernel::main (isosurface_shader.cu:50-125) when
+// corrected == 0; bracketing + bisection + oriented central differences when
+// corrected == 1.
+template <int FMT>
+__global__ __launch_bounds__(64) void iso_kernel(IsoArgs a) {
+  __shared__ float s_lut[FMT == HYD_FMT_U8 ? 256 : 1];
+  if (FMT == HYD_FMT_U8)
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_lut[i] = a.u8_lut[i];
+  __syncthreads();
+  const int x = blockIdx.x * kTile + (threadIdx.x % kTile);
+  const int y = blockIdx.y * kTile + (threadIdx.x / kTile);
+  unsigned count = 0;
+  if (x >= a.W || y >= a.H) return;
+  const f3 d = pixel_ray_dir(a.trans, a.ctg, a.ray_o, x, y, a.W, a.H);
+  const f3 origin = a.ray_o;
+  f3 o = a.ray_o;
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  float depth = INFINITY;
+  int hit = 0;
+  float tnear, tfar;
+  if (intersect_box(o, d, a.bmin, a.bmax, tnear, tfar)) {
+    if (tnear < 0) tnear = 0;
+    if (tfar < 0) tfar = 0;
+    o = add(o, muls(d, tnear));
+    const int n = (int)((tfar - tnear) / a.step);
+    int nsteps = a.max_steps < n ? a.max_steps : n;
+    const float cdu = 1.f / comp_max(abs3(d));
+    float prev_value = 0.f;
+    bool have_prev = false;
+    while (nsteps > 0) {
+      const f3 ip = floor3(o);
+      const int cd = a.cheb[grid_index(ip, a.cdim)];
+      if (cd) {
+        nsteps -= skip_nblock_steps(o, d, ip, cd, cdu, a.step);
+        have_prev = false;
+      } else {
+        const int pgid = a.vaddr[grid_index(ip, a.vdim)];
+        if (pgid == -1) break;
+        const float value = iso_sample<FMT>(a, s_lut, pgid, sub(o, ip));
+        ++count;
+        bool crossing = gsign(value - a.isovalue) != gsign(prev_value - a.isovalue);
+        if (a.corrected) crossing = crossing && have_prev;
+        if (crossing) {
+          const f3 p = o;
+          const float dt = a.step;
+          v = make_float4(a.surface_color.x, a.surface_color.y, a.surface_color.z, 1.f);
+          f3 inter_p, nn;
+          if (!a.corrected) {
+            const float al = (a.isovalue - prev_value) / (value - prev_value);
+            const float bl = 1.f - al;
+            inter_p = f3{bl * (p.x - dt * d.x) + al * p.x, bl * (p.y - dt * d.y) + al * p.y,
+                         bl * (p.z - dt * d.z) + al * p.z};
+            const f3 q = sub(inter_p, ip);
+            const float h = a.du * 2.f;
+            nn = f3{iso_sample<FMT>(a, s_lut, pgid, f3{q.x - h, q.y - 0.f, q.z - 0.f}) -
+                        iso_sample<FMT>(a, s_lut, pgid, f3{q.x + h, q.y + 0.f, q.z + 0.f}),
+                    iso_sample<FMT>(a, s_lut, pgid, f3{q.x - 0.f, q.y - h, q.z - 0.f}) -
+                        iso_sample<FMT>(a, s_lut, pgid, f3{q.x + 0.f, q.y + h, q.z + 0.f}),
+                    iso_sample<FMT>(a, s_lut, pgid, f3{q.x - 0.f, q.y - 0.f, q.z - h}) -
+                        iso_sample<FMT>(a, s_lut, pgid, f3{q.x + 0.f, q.y + 0.f, q.z + h})};
+          } else {
+            f3 plo = p;
+            f3 phi = f3{p.x - dt * d.x, p.y - dt * d.y, p.z - dt * d.z};
+            const float sprev = gsign(prev_value - a.isovalue);
+            for (int it = 0; it < a.bisect_steps; ++it) {
+              const f3 mid = {(plo.x + phi.x) * 0.5f, (plo.y + phi.y) * 0.5f, (plo.z + phi.z) * 0.5f};
+              const f3 mip = floor3(mid);
+              const int mpg = a.vaddr[grid_index(mip, a.vdim)];
+              const float mv = mpg == -1 ? value : iso_sample<FMT>(a, s_lut, mpg, sub(mid, mip));
+              if (gsign(mv - a.isovalue) == sprev) phi = mid; else plo = mid;
+            }
+            inter_p = f3{(plo.x + phi.x) * 0.5f, (plo.y + phi.y) * 0.5f, (plo.z + phi.z) * 0.5f};
+            f3 gip = floor3(inter_p);
+            int gpg = a.vaddr[grid_index(gip, a.vdim)];
+            if (gpg == -1) { gpg = pgid; gip = ip; }
+            const f3 q = sub(inter_p, gip);
+            const float h = a.du;
+            const f3 g = {iso_sample<FMT>(a, s_lut, gpg, f3{q.x + h, q.y + 0.f, q.z + 0.f}) -
+                              iso_sample<FMT>(a, s_lut, gpg, f3{q.x - h, q.y - 0.f, q.z - 0.f}),
+                          iso_sample<FMT>(a, s_lut, gpg, f3{q.x + 0.f, q.y + h, q.z + 0.f}) -
+                              iso_sample<FMT>(a, s_lut, gpg, f3{q.x - 0.f, q.y - h, q.z - 0.f}),
+                          iso_sample<FMT>(a, s_lut, gpg, f3{q.x + 0.f, q.y + 0.f, q.z + h}) -
+                              iso_sample<FMT>(a, s_lut, gpg, f3{q.x - 0.f, q.y - 0.f, q.z - h})};
+            nn = muls(g, sprev);
+          }
+          float wp4[4], n4[4];
+          mat4_vec4(a.to_world, inter_p.x, inter_p.y, inter_p.z, 1.f, wp4);
+          mat4_vec4(a.to_world, nn.x, nn.y, nn.z, 0.f, n4);
+          const f3 world_p = {wp4[0], wp4[1], wp4[2]};
+          const f3 nrm = normalize3(f3{n4[0], n4[1], n4[2]});
+          const f3 light_dir = normalize3(sub(a.light_pos, world_p));
+          const f3 hv = a.corrected ? normalize3(add(light_dir, normalize3(sub(a.eye_pos, world_p))))
+                                    : normalize3(sub(light_dir, world_p));
+          const float ambient = 0.2f;
+          const float diffuse = .6f * gclamp(dot3(light_dir, nrm), 0.f, 1.f);
+          const float specular = .2f * powf(gclamp(dot3(hv, nrm), 0.f, 1.f), 100.f);
+          const float dist = length3(sub(world_p, a.eye_pos)) / 2.f;
+          if (a.mode == HYD_ISO_COLOR) {
+            const float sc = ambient + (diffuse + specular) / dist;
+            v = make_float4(v.x * sc, v.y * sc, v.z * sc, v.w * sc);
+            v = make_float4(iso_srgb(v.x), iso_srgb(v.y), iso_srgb(v.z), v.w);
+          } else if (a.mode == HYD_ISO_POSITION) {
+            v = make_float4(inter_p.x / a.bmax.x, inter_p.y / a.bmax.y, inter_p.z / a.bmax.z, 1.f);
+          } else {
+            v = make_float4(nrm.x, nrm.y, nrm.z, 1.f);
+          }
+          depth = a.corrected ? length3(sub(inter_p, origin)) : length3(sub(origin, o));
+          hit = 1;
+          break;
+        }
+        if (a.corrected) { prev_value = value; have_prev = true; }
+      }
+      o = add(o, muls(d, a.step));
+      nsteps -= 1;
+    }
+  }
+  hyd_iso_pixel px;
+  px.v[0] = v.x; px.v[1] = v.y; px.v[2] = v.z; px.v[3] = v.w;
+  px.depth = depth;
+  px.hit = hit;
+  a.out[(size_t)y * a.W + x] = px;
+  if (a.samples && count) atomicAdd(a.samples, (unsigned long long)count);
+}
+
+}  // namespace
+
+hipError_t launch_blocks(const BlocksArgs &a, hipStream_t s) {
+  dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile);
+  hipLaunchKernelGGL(blocks_kernel, grid, dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_iso(const IsoArgs &a, int fmt, hipStream_t s) {
+  dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile);
+  switch (fmt) {
+    case HYD_FMT_U8: hipLaunchKernelGGL(iso_kernel<HYD_FMT_U8>, grid, dim3(64), 0, s, a); break;
+    case HYD_FMT_F16: hipLaunchKernelGGL(iso_kernel<HYD_FMT_F16>, grid, dim3(64), 0, s, a); break;
+    default: hipLaunchKernelGGL(iso_kernel<HYD_FMT_F32>, grid, dim3(64), 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace hyd

@@ -1,0 +1,93 @@
+// Film passes: pixel (RGBA float -> RGB8), fetch (film -> partial image) and
+// the in-GPU sort-last composite of partial images.
+#include "hyd_internal.hpp"
+
+namespace hyd {
+namespace {
+
+constexpr int kBlock = 256;
+
+// StdVec4Pixel::write_to (pixel_template.hpp:11-20): premultiplied colour
+// saturated to bytes (clamp(v * 255, 0, 255), truncation), clear colour when
+// alpha == 0.
+__global__ __launch_bounds__(kBlock) void pixel_kernel(const float4 *rgba, int n, uchar3 clear, uint8_t *rgb) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 v = rgba[i];
+  uint8_t r, g, b;
+  if (v.w != 0.f) {
+    r = (uint8_t)(int)gclamp(v.x * 255.f, 0.f, 255.f);
+    g = (uint8_t)(int)gclamp(v.y * 255.f, 0.f, 255.f);
+    b = (uint8_t)(int)gclamp(v.z * 255.f, 0.f, 255.f);
+  } else {
+    r = clear.x; g = clear.y; b = clear.z;
+  }
+  rgb[3 * i] = r; rgb[3 * i + 1] = g; rgb[3 * i + 2] = b;
+}
+
+// VolumeShaderKernel::fetch (volume_shader.cu:23-30).
+__global__ __launch_bounds__(kBlock) void fetch_kernel(const hyd_volume_pixel *film, int n, hyd_fetch_pixel *out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 *f = reinterpret_cast<const float4 *>(film + i);
+  float4 *o = reinterpret_cast<float4 *>(out + i);
+  o[0] = f[3];   // theta, phi
+  o[1] = f[2];   // val
+}
+
+// linear_to_srgb, src/slave/renderers/volume/volume.cc:175-181 (float).
+__device__ __forceinline__ float srgb(float x) {
+  if (x <= 0.0031308f) return 12.92f * x;
+  return 1.055f * powf(x, 1.f / 2.4f) - 0.055f;
+}
+
+// Front-to-back "over" of n partial images in z order, then sRGB + saturate
+// (volume.cc:267-292).  One lane per pixel; partials are image-major.
+__global__ __launch_bounds__(kBlock) void composite_kernel(int n, const hyd_fetch_pixel *partials,
+                                                           const int32_t *z_order, int P, float4 *rgba,
+                                                           uint8_t *rgb) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= P) return;
+  const float4 *base = reinterpret_cast<const float4 *>(partials);
+  float4 f = base[((size_t)z_order[0] * P + i) * 2 + 1];
+  for (int k = 1; k < n; ++k) {
+    const size_t j = ((size_t)z_order[k] * P + i) * 2;
+    const float4 bt = base[j];      // theta, phi
+    const float4 bv = base[j + 1];  // val
+    const float vx = f.x + bv.x - f.w * bt.x;
+    const float vy = f.y + bv.y - f.w * bt.y;
+    const float vz = f.z + bv.z - f.w * bt.z;
+    const float an = bt.w * f.w + bv.w;
+    f = make_float4(vx, vy, vz, an);
+  }
+  if (rgba) rgba[i] = f;
+  if (rgb) {
+    rgb[3 * i] = (uint8_t)(int)gclamp(srgb(f.x) * 255.f, 0.f, 255.f);
+    rgb[3 * i + 1] = (uint8_t)(int)gclamp(srgb(f.y) * 255.f, 0.f, 255.f);
+    rgb[3 * i + 2] = (uint8_t)(int)gclamp(srgb(f.z) * 255.f, 0.f, 255.f);
+  }
+}
+
+}  // namespace
+
+hipError_t launch_pixel_pass(const float4 *rgba, int W, int H, uchar3 clear, uint8_t *rgb, hipStream_t s) {
+  const int n = W * H;
+  hipLaunchKernelGGL(pixel_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rgba, n, clear, rgb);
+  return hipGetLastError();
+}
+
+hipError_t launch_fetch_pass(const hyd_volume_pixel *film, int W, int H, hyd_fetch_pixel *out, hipStream_t s) {
+  const int n = W * H;
+  hipLaunchKernelGGL(fetch_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, film, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_composite(int n, const hyd_fetch_pixel *partials, const int32_t *z_order_dev, int W, int H,
+                            float4 *rgba, uint8_t *rgb, hipStream_t s) {
+  const int P = W * H;
+  hipLaunchKernelGGL(composite_kernel, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, partials,
+                     z_order_dev, P, rgba, rgb);
+  return hipGetLastError();
+}
+
+}  // namespace hyd

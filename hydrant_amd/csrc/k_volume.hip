@@ -1,0 +1,373 @@
+// Volume ray-march kernels for gfx950.
+//
+// One lane per pixel; a 64-lane workgroup covers an 8x8 pixel tile so a
+// wavefront marches a coherent ray packet.  Ray emit, box clip and the march
+// loop are fused (the reference splits them over ray_emit_kernel_impl and a
+// device-function-pointer call to VolumeShaderKernel::main,
+// src/slave/core/cuda_shading.cu:14-37 and volume_shader.cu:32-81).
+//
+// Sampling is done in software with the reference CPU backend's exact
+// arithmetic (CpuSampler, include/hydrant/bridge/cpu_sampler.hpp:88-166):
+// hardware texture filtering would not reproduce it.  The transfer function
+// and the u8 normalisation table are staged in LDS per workgroup.
+#include <hip/hip_fp16.h>
+
+#include "hyd_internal.hpp"
+
+namespace hyd {
+namespace {
+
+constexpr int kTile = 8;           // 8x8 pixel packet per wavefront
+constexpr int kMaxTf = 256;        // LDS capacity of the TF table (entries)
+
+template <int FMT> struct Voxel;
+template <> struct Voxel<HYD_FMT_F32> {
+  static __device__ __forceinline__ float at(const void *base, size_t i, const float *) {
+    return static_cast<const float *>(base)[i];
+  }
+};
+template <> struct Voxel<HYD_FMT_F16> {
+  static __device__ __forceinline__ float at(const void *base, size_t i, const float *) {
+    return __half2float(static_cast<const __half *>(base)[i]);
+  }
+};
+template <> struct Voxel<HYD_FMT_U8> {
+  // float(v) / 255.f, correctly rounded, from the LDS table
+  static __device__ __forceinline__ float at(const void *base, size_t i, const float *lut) {
+    return lut[static_cast<const uint8_t *>(base)[i]];
+  }
+};
+
+// Point lookup in an int grid with Clamp addressing (Options::as_array,
+// basic_renderer.hpp:147-159; CpuSampler filter_none).
+__device__ __forceinline__ int grid_index(f3 p, const int32_t dim[3]) {
+  float fx = gclamp(p.x, 0.f, float(dim[0]));
+  float fy = gclamp(p.y, 0.f, float(dim[1]));
+  float fz = gclamp(p.z, 0.f, float(dim[2]));
+  int ix = iclamp((int)floorf(fx), 0, dim[0] - 1);
+  int iy = iclamp((int)floorf(fy), 0, dim[1] - 1);
+  int iz = iclamp((int)floorf(fz), 0, dim[2] - 1);
+  return iz * dim[0] * dim[1] + iy * dim[0] + ix;
+}
+
+struct TapIdx { int ix, iy, iz, jx, jy, jz; };
+
+// Brick sample: BlockSamplerMapping (k*x + b) then CpuSampler Wrap + Linear +
+// normalised coordinates over a pad^3 brick (block_paging.hpp:11-36,
+// cpu_sampler.hpp:88-104,142-166).
+template <int FMT>
+__device__ __forceinline__ float brick_sample(const void *pool, int pad, int pgid, float4 pg, f3 x,
+                                              const float *lut, TapIdx *taps) {
+  const float fd = float(pad);
+  float u[3] = {x.x * pg.x + pg.y, x.y * pg.x + pg.z, x.z * pg.x + pg.w};
+  float a[3];
+  int i[3], j[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float fx = fd * u[c];
+    fx = fx - fd * floorf(fx / fd);   // Wrap: glm mod(fx, fd)
+    fx = fx - 0.5f;
+    float flr = floorf(fx);
+    a[c] = fx - flr;
+    i[c] = iclamp((int)flr, 0, pad - 1);
+    j[c] = iclamp((int)ceilf(fx), 0, pad - 1);
+  }
+  if (taps) *taps = TapIdx{i[0], i[1], i[2], j[0], j[1], j[2]};
+  const size_t base = (size_t)pgid * pad * pad * pad;
+  const size_t row_ii = base + ((size_t)i[2] * pad + i[1]) * pad;
+  const size_t row_ji = base + ((size_t)i[2] * pad + j[1]) * pad;
+  const size_t row_ij = base + ((size_t)j[2] * pad + i[1]) * pad;
+  const size_t row_jj = base + ((size_t)j[2] * pad + j[1]) * pad;
+  float v000 = Voxel<FMT>::at(pool, row_ii + i[0], lut), v100 = Voxel<FMT>::at(pool, row_ii + j[0], lut);
+  float v010 = Voxel<FMT>::at(pool, row_ji + i[0], lut), v110 = Voxel<FMT>::at(pool, row_ji + j[0], lut);
+  float v001 = Voxel<FMT>::at(pool, row_ij + i[0], lut), v101 = Voxel<FMT>::at(pool, row_ij + j[0], lut);
+  float v011 = Voxel<FMT>::at(pool, row_jj + i[0], lut), v111 = Voxel<FMT>::at(pool, row_jj + j[0], lut);
+  const float ax = a[0], bx = 1.f - ax;
+  float x00 = v000 * bx + v100 * ax;
+  float x10 = v010 * bx + v110 * ax;
+  float x01 = v001 * bx + v101 * ax;
+  float x11 = v011 * bx + v111 * ax;
+  const float ay = a[1], by = 1.f - ay;
+  float y0 = x00 * by + x10 * ay;
+  float y1 = x01 * by + x11 * ay;
+  const float az = a[2], bz = 1.f - az;
+  return y0 * bz + y1 * az;
+}
+
+// Transfer function: Border + Linear + normalised 1-D lookup
+// (transfer_fn.hpp:14-28, cpu_sampler.hpp:117-124,142-166).
+__device__ __forceinline__ float4 tf_sample(const float *tf, int n, float s) {
+  const float fd = float(n);
+  float fx = fd * s;
+  if (!(0.f <= fx && fx <= fd)) return make_float4(0.f, 0.f, 0.f, 0.f);
+  fx = fx - 0.5f;
+  float flr = floorf(fx);
+  float a = fx - flr;
+  int ix = iclamp((int)flr, 0, n - 1);
+  int jx = iclamp((int)ceilf(fx), 0, n - 1);
+  const float w = 1.f - a;
+  const float *p = tf + 4 * ix, *q = tf + 4 * jx;
+  return make_float4(p[0] * w + q[0] * a, p[1] * w + q[1] * a, p[2] * w + q[2] * a, p[3] * w + q[3] * a);
+}
+
+struct MarchState {
+  f3 o, d;
+  int nsteps, step_size;
+  float4 v;
+  f3 theta;
+  float phi;
+};
+
+// Shade one resolved sample and composite it front to back
+// (volume_shader.cu:52-70, plus the density scale on the TF alpha).
+__device__ __forceinline__ void shade_composite(const VolumeArgs &a, const float *tf, float s, int pgid,
+                                                MarchState &st) {
+  float4 ub = tf_sample(tf, a.tf_n, s);
+  ub.w = ub.w * a.density;
+  if (a.mode == HYD_MODE_PARTITION) {
+    float l = length3(f3{ub.x, ub.y, ub.z});
+    float r = a.rank, q = 1.f - r;
+    ub = make_float4((1.f * q + 0.f * r) * l, (0.f * q + 0.f * r) * l, (0.f * q + 1.f * r) * l, ub.w);
+  } else if (a.mode == HYD_MODE_PAGING) {
+    ub = pgid >= a.lowest_blkcnt ? make_float4(0.f, 1.f, 0.f, ub.w) : make_float4(1.f, 0.f, 0.f, ub.w);
+  }
+  const float aw = ub.w;
+  ub.x = ub.x * aw; ub.y = ub.y * aw; ub.z = ub.z * aw;
+  st.theta.x = st.theta.x + ub.x * st.phi;
+  st.theta.y = st.theta.y + ub.y * st.phi;
+  st.theta.z = st.theta.z + ub.z * st.phi;
+  st.phi = st.phi * (1.f - ub.w);
+  const float t = 1.f - st.v.w;
+  st.v.x = st.v.x + ub.x * t;
+  st.v.y = st.v.y + ub.y * t;
+  st.v.z = st.v.z + ub.z * t;
+  st.v.w = st.v.w + ub.w * t;
+}
+
+// VolumeShaderKernel::main (volume_shader.cu:32-81), reference step schedule.
+// The escalation tests compare float alpha with the double literals 0.93 and
+// 0.85; for binary32 inputs `x > 0.93` equals `x >= 0.93f` (0.93f rounds up),
+// likewise for 0.85.
+template <int FMT, bool COUNT, bool TRACE>
+__device__ void march_reference(const VolumeArgs &a, const float *tf, const float *lut, MarchState &st,
+                                unsigned &count, hyd_trace_rec *trace, int max_trace, int &ntrace) {
+  const float cdu = 1.f / comp_max(abs3(st.d));
+  const f3 d = st.d;
+  f3 o = st.o;
+  int nsteps = st.nsteps;
+  int step_size = st.step_size;
+  while (nsteps > 0) {
+    const f3 ip = floor3(o);
+    const int cd = a.cheb[grid_index(ip, a.cdim)];
+    if (cd) {
+      // skip_nblock_steps (volume_shader.cu:5-14) with step * step_size
+      const float sstep = a.step * float(step_size);
+      float tn, tfar;
+      intersect_box(o, d, ip, f3{ip.x + 1.f, ip.y + 1.f, ip.z + 1.f}, tn, tfar);
+      const float di = ceilf((tfar + float(cd - 1) * cdu) / sstep);
+      o = add(o, muls(muls(d, di), sstep));
+      nsteps -= (int)di;
+    } else {
+      const int gi = grid_index(ip, a.vdim);
+      const int pgid = a.vaddr[gi];
+      if (pgid == -1) break;
+      const float4 pg = a.page[pgid];
+      TapIdx taps;
+      const float s = brick_sample<FMT>(a.pool, a.pad, pgid, pg, sub(o, ip), lut, TRACE ? &taps : nullptr);
+      if (TRACE && ntrace < max_trace) {
+        const int gx = gi % a.vdim[0], gy = (gi / a.vdim[0]) % a.vdim[1], gz = gi / (a.vdim[0] * a.vdim[1]);
+        trace[ntrace++] = hyd_trace_rec{gx, gy, gz, pgid, taps.ix, taps.iy, taps.iz, taps.jx, taps.jy, taps.jz};
+      }
+      shade_composite(a, tf, s, pgid, st);
+      if (COUNT) ++count;
+      if (st.v.w >= 0.93f) {
+        step_size = 16;
+      } else if (st.v.w >= 0.85f) {
+        step_size = 4;
+      }
+    }
+    o = add(o, muls(muls(d, a.step), float(step_size)));
+    nsteps -= step_size;
+  }
+  st.o = o;
+  st.nsteps = nsteps;
+  st.step_size = step_size;
+}
+
+// Lattice schedule: sample k at o0 + d * (k * step); o0 stays in st.o, the
+// next lattice index in st.step_size and the remaining budget in st.nsteps.
+template <int FMT, bool COUNT>
+__device__ void march_lattice(const VolumeArgs &a, const float *tf, const float *lut, MarchState &st,
+                              unsigned &count) {
+  const f3 o0 = st.o, d = st.d;
+  int k = st.step_size;
+  const int kend = k + st.nsteps;
+  while (k < kend) {
+    const f3 o = add(o0, muls(d, float(k) * a.step));
+    const f3 ip = floor3(o);
+    const int cd = a.cheb[grid_index(ip, a.cdim)];
+    if (cd) {
+      float tn, tfar;
+      intersect_box(o, d, ip, f3{ip.x + 1.f, ip.y + 1.f, ip.z + 1.f}, tn, tfar);
+      const float cdu = 1.f / comp_max(abs3(d));
+      const float t_exit = float(k) * a.step + tfar + float(cd - 1) * cdu;
+      const int kn = (int)ceilf(t_exit / a.step);
+      k = kn > k + 1 ? kn : k + 1;
+      continue;
+    }
+    const int pgid = a.vaddr[grid_index(ip, a.vdim)];
+    if (pgid == -1) break;
+    const float4 pg = a.page[pgid];
+    const float s = brick_sample<FMT>(a.pool, a.pad, pgid, pg, sub(o, ip), lut, nullptr);
+    shade_composite(a, tf, s, pgid, st);
+    if (COUNT) ++count;
+    ++k;
+    if (a.ert > 0.f && st.v.w > a.ert) { k = kend; break; }
+  }
+  st.nsteps = kend - k > 0 ? kend - k : 0;
+  st.step_size = k;
+}
+
+// ray_emit_shader_impl (shader.hpp:113-137) + VolumeShaderKernel::init.
+__device__ __forceinline__ bool emit(const VolumeArgs &a, int x, int y, MarchState &st) {
+  st.d = pixel_ray_dir(a.trans, a.ctg, a.ray_o, x, y, a.W, a.H);
+  st.o = a.ray_o;
+  st.v = make_float4(0.f, 0.f, 0.f, 0.f);
+  st.theta = f3{0.f, 0.f, 0.f};
+  st.phi = 1.f;
+  float tnear, tfar;
+  if (intersect_box(st.o, st.d, a.bmin, a.bmax, tnear, tfar)) {
+    if (tnear < 0) tnear = 0;
+    if (tfar < 0) tfar = 0;
+    st.o = add(st.o, muls(st.d, tnear));
+    const int n = (int)((tfar - tnear) / a.step);
+    st.nsteps = a.max_steps < n ? a.max_steps : n;
+    st.step_size = a.schedule == HYD_SCHED_LATTICE ? 0 : 1;
+    return true;
+  }
+  st.nsteps = 0;
+  st.step_size = a.schedule == HYD_SCHED_LATTICE ? 0 : 1;
+  return false;
+}
+
+__device__ __forceinline__ void store_pixel(const VolumeArgs &a, size_t idx, const MarchState &st) {
+  if (a.film) {
+    float4 *f = reinterpret_cast<float4 *>(a.film + idx);
+    f[0] = make_float4(st.o.x, st.o.y, st.o.z, st.d.x);
+    f[1] = make_float4(st.d.y, st.d.z, __int_as_float(st.nsteps), __int_as_float(st.step_size));
+    f[2] = st.v;
+    f[3] = make_float4(st.theta.x, st.theta.y, st.theta.z, st.phi);
+  }
+  if (a.rgba) a.rgba[idx] = st.v;
+}
+
+__device__ __forceinline__ void load_pixel(const VolumeArgs &a, size_t idx, MarchState &st) {
+  const float4 *f = reinterpret_cast<const float4 *>(a.film + idx);
+  float4 f0 = f[0], f1 = f[1];
+  st.o = f3{f0.x, f0.y, f0.z};
+  st.d = f3{f0.w, f1.x, f1.y};
+  st.nsteps = __float_as_int(f1.z);
+  st.step_size = __float_as_int(f1.w);
+  st.v = f[2];
+  float4 f3v = f[3];
+  st.theta = f3{f3v.x, f3v.y, f3v.z};
+  st.phi = f3v.w;
+}
+
+template <int FMT, bool COUNT, bool RESUME>
+__global__ __launch_bounds__(64) void volume_kernel(VolumeArgs a) {
+  __shared__ float s_tf[4 * kMaxTf];
+  __shared__ float s_lut[FMT == HYD_FMT_U8 ? 256 : 1];
+  __shared__ unsigned long long s_count;
+  for (int i = threadIdx.x; i < 4 * a.tf_n; i += blockDim.x) s_tf[i] = a.tf[i];
+  if (FMT == HYD_FMT_U8)
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_lut[i] = a.u8_lut[i];
+  if (COUNT && threadIdx.x == 0) s_count = 0;
+  __syncthreads();
+
+  const int x = blockIdx.x * kTile + (threadIdx.x % kTile);
+  const int y = blockIdx.y * kTile + (threadIdx.x / kTile);
+  unsigned count = 0;
+  if (x < a.W && y < a.H) {
+    const size_t idx = (size_t)y * a.W + x;
+    MarchState st;
+    bool live;
+    if (RESUME) {
+      load_pixel(a, idx, st);
+      live = st.nsteps > 0;
+    } else {
+      live = emit(a, x, y, st);
+    }
+    if (live) {
+      int nt = 0;
+      if (a.schedule == HYD_SCHED_LATTICE)
+        march_lattice<FMT, COUNT>(a, s_tf, s_lut, st, count);
+      else
+        march_reference<FMT, COUNT, false>(a, s_tf, s_lut, st, count, nullptr, 0, nt);
+    }
+    store_pixel(a, idx, st);
+  }
+  if (COUNT) {
+    if (count) atomicAdd(&s_count, (unsigned long long)count);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_count) atomicAdd(a.samples, s_count);
+  }
+}
+
+template <int FMT>
+__global__ __launch_bounds__(64) void volume_trace_kernel(VolumeArgs a, int x, int y, hyd_trace_rec *out,
+                                                          int max_recs, int *n_out) {
+  __shared__ float s_tf[4 * kMaxTf];
+  __shared__ float s_lut[256];
+  for (int i = threadIdx.x; i < 4 * a.tf_n; i += blockDim.x) s_tf[i] = a.tf[i];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) s_lut[i] = a.u8_lut ? a.u8_lut[i] : 0.f;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  MarchState st;
+  unsigned count = 0;
+  int nt = 0;
+  if (emit(a, x, y, st)) march_reference<FMT, false, true>(a, s_tf, s_lut, st, count, out, max_recs, nt);
+  *n_out = nt;
+}
+
+template <int FMT>
+hipError_t launch_fmt(const VolumeArgs &a, bool resume, hipStream_t s) {
+  dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile);
+  dim3 block(kTile * kTile);
+  const bool count = a.samples != nullptr;
+  if (resume) {
+    if (count) hipLaunchKernelGGL((volume_kernel<FMT, true, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((volume_kernel<FMT, false, true>), grid, block, 0, s, a);
+  } else {
+    if (count) hipLaunchKernelGGL((volume_kernel<FMT, true, false>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((volume_kernel<FMT, false, false>), grid, block, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_volume(const VolumeArgs &a, int fmt, bool resume, hipStream_t s) {
+  switch (fmt) {
+    case HYD_FMT_U8: return launch_fmt<HYD_FMT_U8>(a, resume, s);
+    case HYD_FMT_F16: return launch_fmt<HYD_FMT_F16>(a, resume, s);
+    default: return launch_fmt<HYD_FMT_F32>(a, resume, s);
+  }
+}
+
+hipError_t launch_volume_trace(const VolumeArgs &a, int fmt, int x, int y, hyd_trace_rec *out_dev, int max_recs,
+                               int *n_dev, hipStream_t s) {
+  switch (fmt) {
+    case HYD_FMT_U8:
+      hipLaunchKernelGGL(volume_trace_kernel<HYD_FMT_U8>, dim3(1), dim3(64), 0, s, a, x, y, out_dev, max_recs, n_dev);
+      break;
+    case HYD_FMT_F16:
+      hipLaunchKernelGGL(volume_trace_kernel<HYD_FMT_F16>, dim3(1), dim3(64), 0, s, a, x, y, out_dev, max_recs, n_dev);
+      break;
+    default:
+      hipLaunchKernelGGL(volume_trace_kernel<HYD_FMT_F32>, dim3(1), dim3(64), 0, s, a, x, y, out_dev, max_recs, n_dev);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace hyd

@@ -1,0 +1,231 @@
+"""Dataset preparation: synthetic volumes, padded bricks, thumbnails, page tables.
+
+Stands in for the reference's varch tooling and paging server
+(build_dataset.sh:79-124, src/slave/paging/lossless_block_paging.cc:15-83):
+volumes are synthesised deterministically (include/hydrant_dataset.h), packed
+into padded bricks in page order, and described by the same BlockPaging
+structures the reference builds (vaddr grid, per-page {k, b} mapping,
+chebyshev empty-space thumbnail, mean thumbnail).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+
+# ----------------------------------------------------------- generators ----
+def gen_blobs(n: int | tuple, n_blobs: int = 16, seed: int = 1, sig=(8.0, 40.0), sigma_scale: float = 1.0,
+              amp=(0.3, 1.0)) -> np.ndarray:
+    """Sum of seeded Gaussian blobs quantised to u8, shape (nz, ny, nx)."""
+    nx, ny, nz = (n, n, n) if isinstance(n, int) else n
+    out = np.empty((nz, ny, nx), np.uint8)
+    L.call("hyd_gen_blobs_u8", nx, ny, nz, n_blobs, seed, sig[0], sig[1], sigma_scale, amp[0], amp[1],
+           out.ctypes.data)
+    return out
+
+
+def gen_perlin(n: int, octaves: int = 5, persistence: float = 0.5, base_period: float = 128.0, gain: float = 2.0,
+               seed: int = 2) -> np.ndarray:
+    """Seeded improved-Perlin fBm volume quantised to u8, shape (n, n, n)."""
+    out = np.empty((n, n, n), np.uint8)
+    L.call("hyd_gen_perlin_u8", n, octaves, persistence, base_period, gain, seed, out.ctypes.data)
+    return out
+
+
+def gen_sdf_spheres(n: int, n_spheres: int = 64, r=(10.0, 60.0), k: float = 8.0, seed: int = 3) -> np.ndarray:
+    """Smooth-union SDF of seeded spheres as IEEE half bits (uint16), shape (n, n, n)."""
+    out = np.empty((n, n, n), np.uint16)
+    L.call("hyd_gen_sdf_spheres_f16", n, n_spheres, r[0], r[1], k, seed, out.ctypes.data)
+    return out
+
+
+def sha256(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def normalised(vol: np.ndarray) -> np.ndarray:
+    """Voxel values as the renderer reads them: float(v)/255 for u8, half->float for f16."""
+    if vol.dtype == np.uint8:
+        return vol.astype(np.float32) / np.float32(255.0)
+    if vol.dtype == np.uint16:
+        return vol.view(np.float16).astype(np.float32)
+    return vol.astype(np.float32)
+
+
+def _fmt_of(vol: np.ndarray) -> int:
+    return {np.dtype(np.uint8): L.FMT_U8, np.dtype(np.uint16): L.FMT_F16, np.dtype(np.float32): L.FMT_F32}[vol.dtype]
+
+
+_NP_OF_FMT = {L.FMT_U8: np.uint8, L.FMT_F16: np.uint16, L.FMT_F32: np.float32}
+
+
+# ------------------------------------------------------ bricked dataset ----
+@dataclass
+class Bricked:
+    """A bricked volume in the reference's BlockPaging terms."""
+    dims: tuple            # raw volume dims (x, y, z)
+    bs: int                # block size (brick edge without apron)
+    apron: int
+    grid: tuple            # brick grid (x, y, z)
+    fmt: int               # pool storage format
+    pool: np.ndarray       # (n_pages, pad, pad, pad) in page order
+    vaddr: np.ndarray      # int32 grid (z, y, x); -1 = not resident
+    page_k: np.ndarray     # float32 (n_pages,)
+    page_b: np.ndarray     # float32 (n_pages, 3)
+    cheb: np.ndarray       # int32 grid
+    mean: np.ndarray       # float32 grid (interior mean, normalised)
+    empty: np.ndarray      # uint8 grid
+    order: np.ndarray      # non-empty bricks in page-assignment order
+    n_nonempty: int
+    lowest_blkcnt: int = 0
+    bmin: np.ndarray = field(default=None)
+    bmax: np.ndarray = field(default=None)
+
+    @property
+    def pad(self) -> int:
+        return self.bs + 2 * self.apron
+
+    @property
+    def n_pages(self) -> int:
+        return int(self.pool.shape[0])
+
+    def pool_values(self) -> np.ndarray:
+        """Normalised float32 voxel values of the pool (what the samplers read)."""
+        return normalised(self.pool)
+
+
+def mapping(bs: int, apron: int) -> tuple[np.float32, np.float32]:
+    """BlockSamplerMapping k, b (lossless_block_paging.cc:21-22): k = bs/pad, b = (apron/bs)*k, binary32."""
+    pad = bs + 2 * apron
+    k = np.float32(bs) / np.float32(pad)
+    b = (np.float32(apron) / np.float32(bs)) * k
+    return np.float32(k), np.float32(b)
+
+
+def brick_stats(vol: np.ndarray, bs: int, apron: int):
+    dims = np.array(vol.shape[::-1], np.int32)
+    grid = [(int(d) + bs - 1) // bs for d in dims]
+    nb = grid[0] * grid[1] * grid[2]
+    bmin = np.empty(nb, np.float32)
+    bmax = np.empty(nb, np.float32)
+    bmean = np.empty(nb, np.float32)
+    vol = np.ascontiguousarray(vol)
+    L.call("hyd_brick_stats", vol.ctypes.data, _fmt_of(vol), L.ptr(dims, C.c_int32), bs, apron,
+           L.ptr(bmin, C.c_float), L.ptr(bmax, C.c_float), L.ptr(bmean, C.c_float))
+    shape = (grid[2], grid[1], grid[0])
+    return grid, bmin.reshape(shape), bmax.reshape(shape), bmean.reshape(shape)
+
+
+def tf_alpha_zero_on(tf: np.ndarray, lo: float, hi: float) -> bool:
+    tf = np.ascontiguousarray(tf, np.float32).reshape(-1)
+    return bool(L.lib().hyd_tf_alpha_zero_on(L.ptr(tf, C.c_float), tf.size // 4, float(lo), float(hi)))
+
+
+def chebyshev(empty: np.ndarray) -> np.ndarray:
+    empty = np.ascontiguousarray(empty, np.uint8)
+    dims = np.array(empty.shape[::-1], np.int32)
+    out = np.empty(empty.shape, np.int32)
+    L.call("hyd_chebyshev", L.ptr(dims, C.c_int32), L.ptr(empty, C.c_uint8), L.ptr(out, C.c_int32))
+    return out
+
+
+def pack(vol: np.ndarray, bs: int, apron: int, fmt_out: int, bricks: np.ndarray | None = None) -> np.ndarray:
+    vol = np.ascontiguousarray(vol)
+    dims = np.array(vol.shape[::-1], np.int32)
+    grid = [(int(d) + bs - 1) // bs for d in dims]
+    count = grid[0] * grid[1] * grid[2] if bricks is None else len(bricks)
+    pad = bs + 2 * apron
+    out = np.empty((count, pad, pad, pad), _NP_OF_FMT[fmt_out])
+    if count == 0:
+        return out
+    bp = None
+    if bricks is not None:
+        bricks = np.ascontiguousarray(bricks, np.int32)
+        bp = L.ptr(bricks, C.c_int32)
+    L.call("hyd_pack_bricks", vol.ctypes.data, _fmt_of(vol), L.ptr(dims, C.c_int32), bs, apron, fmt_out, bp,
+           count, out.ctypes.data)
+    return out
+
+
+def make_bricked(vol: np.ndarray, bs: int, apron: int, fmt_out: int, empty_fn=None, resident: int | float = 1.0,
+                 seed: int = 7) -> Bricked:
+    """Brick ``vol`` (shape (z, y, x)); ``empty_fn(bmin, bmax) -> bool array`` marks empty bricks.
+
+    ``resident``: number (int) or fraction (float) of bricks made resident by the
+    seeded page table (drawn from the non-empty bricks).
+    """
+    grid, bmin, bmax, bmean = brick_stats(vol, bs, apron)
+    empty = np.zeros(bmin.shape, np.uint8) if empty_fn is None else np.asarray(empty_fn(bmin, bmax), np.uint8)
+    cheb = chebyshev(empty)
+    nb = bmin.size
+    n_res = int(round(resident * nb)) if isinstance(resident, float) else int(resident)
+    dims = np.array(grid, np.int32)
+    vaddr = np.empty(bmin.shape, np.int32)
+    order = np.empty(nb, np.int32)
+    nn = C.c_int32(0)
+    empty_c = np.ascontiguousarray(empty)
+    L.call("hyd_seeded_page_table", L.ptr(dims, C.c_int32), L.ptr(empty_c, C.c_uint8), n_res, seed,
+           L.ptr(vaddr, C.c_int32), L.ptr(order, C.c_int32), C.byref(nn))
+    n_pages = min(n_res, nn.value)
+    pool = pack(vol, bs, apron, fmt_out, order[:n_pages])
+    k, b = mapping(bs, apron)
+    page_k = np.full(max(n_pages, 1), k, np.float32)
+    page_b = np.full((max(n_pages, 1), 3), b, np.float32)
+    return Bricked(dims=tuple(int(d) for d in vol.shape[::-1]), bs=bs, apron=apron, grid=tuple(grid), fmt=fmt_out,
+                   pool=pool, vaddr=vaddr, page_k=page_k, page_b=page_b, cheb=cheb, mean=bmean, empty=empty,
+                   order=order[:nn.value].copy(), n_nonempty=nn.value, bmin=bmin, bmax=bmax)
+
+
+def single_brick(vol: np.ndarray, fmt_out: int = L.FMT_F32) -> Bricked:
+    """One brick covering a cubic volume, no apron (block_size = raw dim): k = 1, b = 0."""
+    n = vol.shape[0]
+    assert vol.shape == (n, n, n), "single_brick needs a cubic volume"
+    return make_bricked(vol, n, 0, fmt_out, empty_fn=None, resident=1)
+
+
+def volume_empty_fn(tf: np.ndarray):
+    """Volume-mode emptiness: the TF alpha is zero over the brick's [min, max]."""
+    def fn(bmin, bmax):
+        out = np.zeros(bmin.shape, bool)
+        for idx in np.ndindex(bmin.shape):
+            out[idx] = tf_alpha_zero_on(tf, float(bmin[idx]), float(bmax[idx]))
+        return out
+    return fn
+
+
+def iso_empty_fn(iso: float):
+    """Isosurface emptiness: the iso value lies outside the brick's [min, max]."""
+    def fn(bmin, bmax):
+        return (bmin > iso) | (bmax < iso)
+    return fn
+
+
+# ------------------------------------------------------------- scenes ------
+README_TF = np.array([
+    [0, 0, 0, 0], [1, 0, 0, 1], [1, 0.5, 0, 1], [1, 1, 0, 1], [0, 1, 0, 1],
+    [0, 1, 1, 1], [0, 0, 1, 1], [1, 0, 1, 1], [0, 0, 0, 0]], np.float32)
+"""9-point RGBA transfer function of the README Aneurism config (README.md:30-44)."""
+
+
+@dataclass
+class SceneGeometry:
+    """BasicRenderer::init geometry (include/hydrant/basic_renderer.hpp:57-84), binary32."""
+    f_dim: np.ndarray      # raw / block_size
+    center: np.ndarray     # exhibit centre = f_dim / 2
+    du: np.float32         # 1 / block_size
+    step: np.float32       # du / 4 / sample_rate
+
+
+def geometry(raw_dims, block_size: int, sample_rate: float = 1.0) -> SceneGeometry:
+    raw = np.array(raw_dims, np.float32)
+    blk = np.float32(block_size)
+    f_dim = raw / blk
+    center = f_dim / np.float32(2.0)
+    du = np.float32(1.0) / blk
+    step = (du / np.float32(4.0)) / np.float32(sample_rate)
+    return SceneGeometry(f_dim=f_dim, center=center, du=np.float32(du), step=np.float32(step))

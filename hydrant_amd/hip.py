@@ -1,0 +1,213 @@
+"""Python handle on a libhydrant_hip context.
+
+torch is used only for device memory and streams (outputs are torch tensors
+on ``cuda:<device>``); every kernel runs through the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .dataset import Bricked
+
+
+def _f3(v):
+    return L.f3(*[float(x) for x in v])
+
+
+def orbit_camera(arm, center=(0.0, 0.0, 0.0), fovy: float = 60.0) -> L.Camera:
+    """Camera from CameraOrbit (scene.hpp:68-81) computed by the library."""
+    cam = L.Camera()
+    arm_a = np.asarray(arm, np.float32)
+    ctr = np.asarray(center, np.float32)
+    L.call("hyd_orbit_camera", L.ptr(arm_a, C.c_float), L.ptr(ctr, C.c_float), float(fovy), C.byref(cam))
+    return cam
+
+
+def ptu_camera(position, target=(0, 0, 0), up=(0, 1, 0), fovy: float = 60.0) -> L.Camera:
+    cam = L.Camera()
+    cam.position = _f3(position)
+    cam.target = _f3(target)
+    cam.up = _f3(up)
+    cam.fovy = float(fovy)
+    return cam
+
+
+def make_view(cam: L.Camera, ex_center, ex_size) -> L.View:
+    v = L.View()
+    c = np.asarray(ex_center, np.float32)
+    s = np.asarray(ex_size, np.float32)
+    L.call("hyd_make_view", C.byref(cam), L.ptr(c, C.c_float), L.ptr(s, C.c_float), C.byref(v))
+    return v
+
+
+def view_to_dict(v: L.View) -> dict:
+    return {"trans": np.array(v.trans[:], np.float32), "ctg_fovy_2": np.float32(v.ctg_fovy_2),
+            "ray_o": np.array(v.ray_o[:], np.float32)}
+
+
+def volume_params(bbox_min, bbox_max, step, du, max_steps=4_000_000, mode=L.MODE_DEFAULT, rank=0.0, density=1.0,
+                  schedule=L.SCHED_REFERENCE, ert=0.0) -> L.VolumeParams:
+    p = L.VolumeParams()
+    p.bbox_min = _f3(bbox_min)
+    p.bbox_max = _f3(bbox_max)
+    p.step = float(step)
+    p.du = float(du)
+    p.max_steps = int(max_steps)
+    p.mode = int(mode)
+    p.rank = float(rank)
+    p.density = float(density)
+    p.schedule = int(schedule)
+    p.ert = float(ert)
+    return p
+
+
+class Context:
+    """One libhydrant_hip context on ``cuda:<device>``; work runs on torch's current stream."""
+
+    def __init__(self, device: int = 0):
+        import torch
+
+        self.torch = torch
+        self.device = device
+        self.dev = torch.device("cuda", device)
+        h = C.c_void_p()
+        L.call("hyd_create", device, C.byref(h))
+        self.h = h
+        self.sync_stream()
+
+    def sync_stream(self):
+        """Issue library work on torch's current stream of this device."""
+        s = self.torch.cuda.current_stream(self.dev).cuda_stream
+        L.call("hyd_set_stream", self.h, C.c_void_p(s))
+
+    def close(self):
+        if self.h:
+            L.call("hyd_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------- state ----
+    def upload_pool(self, fmt: int, pad: int, pool: np.ndarray):
+        pool = np.ascontiguousarray(pool)
+        L.call("hyd_upload_pool", self.h, fmt, pad, pool.shape[0], pool.ctypes.data)
+
+    def bind_pool_device(self, fmt: int, pad: int, n_pages: int, ptr: int):
+        L.call("hyd_bind_pool_device", self.h, fmt, pad, n_pages, C.c_void_p(ptr))
+
+    def set_page_table(self, vaddr: np.ndarray, page_k: np.ndarray, page_b: np.ndarray, lowest_blkcnt: int = 0):
+        vaddr = np.ascontiguousarray(vaddr, np.int32)
+        dims = np.array(vaddr.shape[::-1], np.int32)
+        page_k = np.ascontiguousarray(page_k, np.float32)
+        page_b = np.ascontiguousarray(page_b, np.float32)
+        L.call("hyd_set_page_table", self.h, L.ptr(dims, C.c_int32), L.ptr(vaddr, C.c_int32), page_k.shape[0],
+               L.ptr(page_k, C.c_float), L.ptr(page_b, C.c_float), lowest_blkcnt)
+
+    def set_empty_grid(self, cheb: np.ndarray):
+        cheb = np.ascontiguousarray(cheb, np.int32)
+        dims = np.array(cheb.shape[::-1], np.int32)
+        L.call("hyd_set_empty_grid", self.h, L.ptr(dims, C.c_int32), L.ptr(cheb, C.c_int32))
+
+    def set_mean_grid(self, mean: np.ndarray):
+        mean = np.ascontiguousarray(mean, np.float32)
+        dims = np.array(mean.shape[::-1], np.int32)
+        L.call("hyd_set_mean_grid", self.h, L.ptr(dims, C.c_int32), L.ptr(mean, C.c_float))
+
+    def set_tf(self, tf: np.ndarray):
+        tf = np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
+        L.call("hyd_set_tf", self.h, tf.shape[0], L.ptr(tf, C.c_float))
+
+    def load(self, b: Bricked, tf: np.ndarray | None = None):
+        """Make a bricked dataset resident: pool, page table, empty grid, mean grid (and TF)."""
+        self.upload_pool(b.fmt, b.pad, b.pool)
+        self.set_page_table(b.vaddr, b.page_k, b.page_b, b.lowest_blkcnt)
+        self.set_empty_grid(b.cheb)
+        self.set_mean_grid(b.mean)
+        if tf is not None:
+            self.set_tf(tf)
+
+    # --------------------------------------------------------- passes ----
+    def render_volume(self, params: L.VolumeParams, view: L.View, W: int, H: int, film: bool = True,
+                      count: bool = False, rgba=None):
+        t = self.torch
+        self.sync_stream()
+        film_t = t.empty((H, W, 16), dtype=t.float32, device=self.dev) if film else None
+        rgba_t = rgba if rgba is not None else t.empty((H, W, 4), dtype=t.float32, device=self.dev)
+        cnt = t.zeros(1, dtype=t.int64, device=self.dev) if count else None
+        L.call("hyd_render_volume", self.h, C.byref(params), C.byref(view), W, H,
+               C.c_void_p(film_t.data_ptr() if film else 0), C.c_void_p(rgba_t.data_ptr()),
+               C.c_void_p(cnt.data_ptr() if count else 0))
+        return {"film": film_t, "rgba": rgba_t, "samples": cnt}
+
+    def resume_volume(self, params: L.VolumeParams, film, count: bool = False):
+        t = self.torch
+        self.sync_stream()
+        H, W = film.shape[0], film.shape[1]
+        rgba_t = t.empty((H, W, 4), dtype=t.float32, device=self.dev)
+        cnt = t.zeros(1, dtype=t.int64, device=self.dev) if count else None
+        L.call("hyd_resume_volume", self.h, C.byref(params), W, H, C.c_void_p(film.data_ptr()),
+               C.c_void_p(rgba_t.data_ptr()), C.c_void_p(cnt.data_ptr() if count else 0))
+        return {"film": film, "rgba": rgba_t, "samples": cnt}
+
+    def pixel_pass(self, rgba, clear=(0, 0, 0)):
+        t = self.torch
+        self.sync_stream()
+        H, W = rgba.shape[0], rgba.shape[1]
+        out = t.empty((H, W, 3), dtype=t.uint8, device=self.dev)
+        cc = (C.c_uint8 * 3)(*clear)
+        L.call("hyd_pixel_pass", self.h, C.c_void_p(rgba.data_ptr()), W, H, cc, C.c_void_p(out.data_ptr()))
+        return out
+
+    def fetch_pass(self, film):
+        t = self.torch
+        self.sync_stream()
+        H, W = film.shape[0], film.shape[1]
+        out = t.empty((H, W, 8), dtype=t.float32, device=self.dev)
+        L.call("hyd_fetch_pass", self.h, C.c_void_p(film.data_ptr()), W, H, C.c_void_p(out.data_ptr()))
+        return out
+
+    def composite(self, partials, z_order):
+        """partials: (N, H, W, 8) fetch pixels; returns (rgba, rgb8)."""
+        t = self.torch
+        self.sync_stream()
+        n, H, W = partials.shape[0], partials.shape[1], partials.shape[2]
+        z = np.ascontiguousarray(z_order, np.int32)
+        rgba = t.empty((H, W, 4), dtype=t.float32, device=self.dev)
+        rgb = t.empty((H, W, 3), dtype=t.uint8, device=self.dev)
+        L.call("hyd_composite", self.h, n, C.c_void_p(partials.data_ptr()), L.ptr(z, C.c_int32), W, H,
+               C.c_void_p(rgba.data_ptr()), C.c_void_p(rgb.data_ptr()))
+        return rgba, rgb
+
+    def trace_volume(self, params: L.VolumeParams, view: L.View, W: int, H: int, x: int, y: int,
+                     max_recs: int = 8192) -> np.ndarray:
+        recs = (L.TraceRec * max_recs)()
+        n = C.c_int32(0)
+        L.call("hyd_trace_volume", self.h, C.byref(params), C.byref(view), W, H, x, y, recs, max_recs, C.byref(n))
+        arr = np.frombuffer(recs, dtype=np.int32).reshape(max_recs, 10)[: n.value].copy()
+        return arr
+
+    def render_blocks(self, params: L.BlocksParams, view: L.View, W: int, H: int):
+        t = self.torch
+        self.sync_stream()
+        rgba = t.empty((H, W, 4), dtype=t.float32, device=self.dev)
+        L.call("hyd_render_blocks", self.h, C.byref(params), C.byref(view), W, H, C.c_void_p(rgba.data_ptr()))
+        return rgba
+
+    def render_iso(self, params: L.IsoParams, view: L.View, W: int, H: int, count: bool = False):
+        t = self.torch
+        self.sync_stream()
+        out = t.empty((H, W, 6), dtype=t.float32, device=self.dev)
+        cnt = t.zeros(1, dtype=t.int64, device=self.dev) if count else None
+        L.call("hyd_render_iso", self.h, C.byref(params), C.byref(view), W, H, C.c_void_p(out.data_ptr()),
+               C.c_void_p(cnt.data_ptr() if count else 0))
+        return {"px": out, "samples": cnt}
+
+    def synchronize(self):
+        L.call("hyd_synchronize", self.h)

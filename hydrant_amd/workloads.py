@@ -1,0 +1,159 @@
+"""Seeded synthetic workloads for the render path.
+
+Each builder returns a :class:`Scene` that holds the bricked dataset, the
+camera/view and the shader parameters of one benchmark configuration, at
+full size or scaled down (``n`` voxels per axis, ``W`` x ``H`` pixels) for
+parity tests.  Geometry follows BasicRenderer::init/update
+(include/hydrant/basic_renderer.hpp:57-84): object space is in block units,
+bbox = [0, raw / block_size], du = 1 / block_size, step = du / 4 / sample_rate.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from . import dataset as D
+
+SEED_BLOBS = 1001
+SEED_PERLIN = 2002
+SEED_PAGES = 2003
+SEED_SDF = 3003
+SEED_BATCH = 4000
+README_ARM = (25.0, 65.0, 2.2)
+
+
+@dataclass
+class Scene:
+    name: str
+    bricked: D.Bricked
+    tf: np.ndarray
+    W: int
+    H: int
+    arm: tuple
+    fovy: float
+    geom: D.SceneGeometry
+    density: float = 1.0
+    max_steps: int = 4_000_000
+    schedule: int = L.SCHED_REFERENCE
+    ert: float = 0.0
+    mode: int = L.MODE_DEFAULT
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def bbox_min(self):
+        return np.zeros(3, np.float32)
+
+    @property
+    def bbox_max(self):
+        return self.geom.f_dim
+
+    def camera(self):
+        from .hip import orbit_camera
+        return orbit_camera(self.arm, (0.0, 0.0, 0.0), self.fovy)
+
+    def view(self):
+        from .hip import make_view
+        return make_view(self.camera(), self.geom.center, self.geom.f_dim)
+
+    def volume_params(self):
+        from .hip import volume_params
+        return volume_params(self.bbox_min, self.bbox_max, self.geom.step, self.geom.du, self.max_steps,
+                             self.mode, 0.0, self.density, self.schedule, self.ert)
+
+
+def plain_volume(n: int = 256, W: int = 512, H: int = 512, fmt: int = L.FMT_F32, density: float = 1e-2,
+                 sample_rate: float = 0.5, seed: int = SEED_BLOBS) -> Scene:
+    """Plain Volume mode: n^3 u8 sum of 16 Gaussian blobs (sigma 8-40 voxels at n = 256,
+    scaled with n; amplitude U(0.3, 1)), one brick (k = 1, b = 0), README TF and
+    orbit camera, step 0.5 voxel, density 1e-2."""
+    vol = D.gen_blobs(n, 16, seed, sig=(8.0, 40.0), sigma_scale=n / 256.0, amp=(0.3, 1.0))
+    b = D.single_brick(vol, fmt)
+    geom = D.geometry((n, n, n), n, sample_rate)
+    return Scene("plain_volume", b, D.README_TF.copy(), W, H, README_ARM, 60.0, geom, density=density,
+                 extra={"volume": vol})
+
+
+def paged_volume(n: int = 1024, W: int = 512, H: int = 512, fmt: int = L.FMT_U8, bs: int = 64, apron: int = 1,
+                 resident: float = 0.4, density: float = 1e-2, sample_rate: float = 0.5, seed: int = SEED_PERLIN,
+                 page_seed: int = SEED_PAGES, schedule: int = L.SCHED_REFERENCE) -> Scene:
+    """Blocks.Volume paged: n^3 u8 Perlin fBm (5 octaves, persistence 0.5, base period
+    n/8 voxels) in bs^3 bricks + apron, `resident` of the bricks resident through
+    a seeded page table, empty bricks (TF alpha zero over min/max) skipped."""
+    vol = D.gen_perlin(n, 5, 0.5, n / 8.0, 2.0, seed)
+    tf = D.README_TF.copy()
+    nb = ((n + bs - 1) // bs) ** 3
+    n_res = int(round(resident * nb))
+    b = D.make_bricked(vol, bs, apron, fmt, D.volume_empty_fn(tf), resident=n_res, seed=page_seed)
+    geom = D.geometry((n, n, n), bs, sample_rate)
+    return Scene("paged_volume", b, tf, W, H, README_ARM, 60.0, geom, density=density, schedule=schedule,
+                 extra={"volume": vol})
+
+
+def isosurface(n: int = 512, W: int = 1024, H: int = 1024, bs: int = 64, apron: int = 1, n_spheres: int = 64,
+               sample_rate: float = 1.0, seed: int = SEED_SDF, corrected: int = 1) -> Scene:
+    """Isosurface: n^3 FP16 SDF of seeded spheres (radius 10-60 voxels at n = 512,
+    scaled with n), smooth union k = 8, iso 0, bracket + 8 bisections."""
+    s = n / 512.0
+    sdf = D.gen_sdf_spheres(n, n_spheres, (10.0 * s, 60.0 * s), 8.0 * s, seed)
+    b = D.make_bricked(sdf, bs, apron, L.FMT_F16, D.iso_empty_fn(0.0), resident=1.0, seed=1)
+    geom = D.geometry((n, n, n), bs, sample_rate)
+    sc = Scene("isosurface", b, np.zeros((1, 4), np.float32), W, H, README_ARM, 60.0, geom,
+               extra={"volume": sdf, "isovalue": 0.0, "corrected": corrected, "bisect_steps": 8})
+    return sc
+
+
+def iso_params(sc: Scene, mode: int = L.ISO_COLOR):
+    """IsosurfaceRenderer::offline_render_ctxed shader setup (isosurface.cc:108-113)."""
+    import ctypes as C
+
+    cam = sc.camera()
+    c = np.asarray(sc.geom.center, np.float32)
+    s = np.asarray(sc.geom.f_dim, np.float32)
+    iet = np.zeros(16, np.float32)
+    L.call("hyd_exhibit_iet", L.ptr(c, C.c_float), L.ptr(s, C.c_float), L.ptr(iet, C.c_float))
+    to_world = np.zeros(16, np.float32)
+    L.call("hyd_mat4_inverse", L.ptr(iet, C.c_float), L.ptr(to_world, C.c_float))
+    pos = np.array(cam.position[:], np.float32)
+    tgt = np.array(cam.target[:], np.float32)
+    up = np.array(cam.up[:], np.float32)
+    cr = np.array([tgt[1] * up[2] - up[1] * tgt[2], tgt[2] * up[0] - up[2] * tgt[0],
+                   tgt[0] * up[1] - up[0] * tgt[1]], np.float32)
+    light = ((pos + tgt) + up) + cr
+    p = L.IsoParams()
+    p.bbox_min = L.f3(0, 0, 0)
+    p.bbox_max = L.f3(*[float(x) for x in sc.geom.f_dim])
+    p.step = float(sc.geom.step)
+    p.du = float(sc.geom.du)
+    p.max_steps = sc.max_steps
+    p.mode = mode
+    p.to_world = L.f16(*[float(x) for x in to_world])
+    p.eye_pos = L.f3(*[float(x) for x in pos])
+    p.light_pos = L.f3(*[float(x) for x in light])
+    p.surface_color = L.f3(1.0, 1.0, 1.0)
+    p.isovalue = float(sc.extra["isovalue"])
+    p.corrected = int(sc.extra["corrected"])
+    p.bisect_steps = int(sc.extra["bisect_steps"])
+    return p
+
+
+def blocks(n: int = 256, W: int = 512, H: int = 512, bs: int = 32, density: float = 1e-2, seed: int = SEED_BLOBS,
+           sample_rate: float = 1.0) -> Scene:
+    """Blocks.Volume / Blocks.Solid over a blob volume in bs^3 bricks; a brick is
+    empty when its maximum is below 1/255 (no visible data)."""
+    vol = D.gen_blobs(n, 16, seed, sig=(8.0, 40.0), sigma_scale=n / 256.0, amp=(0.3, 1.0))
+    b = D.make_bricked(vol, bs, 0, L.FMT_U8, lambda lo, hi: hi < (1.0 / 255.0), resident=1.0, seed=1)
+    geom = D.geometry((n, n, n), bs, sample_rate)
+    return Scene("blocks", b, D.README_TF.copy(), W, H, README_ARM, 60.0, geom, density=density)
+
+
+def blocks_params(sc: Scene, mode: int = L.BLOCKS_VOLUME):
+    p = L.BlocksParams()
+    p.bbox_min = L.f3(0, 0, 0)
+    p.bbox_max = L.f3(*[float(x) for x in sc.geom.f_dim])
+    p.step = float(sc.geom.step)
+    p.max_steps = sc.max_steps
+    p.mode = mode
+    p.density = float(sc.density)
+    return p

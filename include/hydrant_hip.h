@@ -1,0 +1,234 @@
+/*
+ * hydrant_hip.h - C ABI of the MI355X (gfx950) render hot path.
+ *
+ * This is the thin boundary that replaces the reference's shader-device
+ * backend (ShaderRegistry devices[Cpu|Cuda], include/hydrant/core/shader.hpp:95-109,
+ * 304-396) and its Raycaster passes (include/hydrant/core/raycaster.hpp:32-123).
+ * Everything is plain C: opaque context handle, plain pointers and sizes.
+ *
+ * Conventions
+ *  - Every call returns int32 status (HYD_OK = 0).  On failure a message is
+ *    kept in a thread-local buffer readable through hyd_last_error().
+ *  - Host pointers stay caller-owned and are only read during the call.
+ *    Pool, tables and transfer function are copied into context-owned device
+ *    memory.  Arguments named *_dev are device pointers owned by the caller
+ *    (hipMalloc'd or a torch tensor's data_ptr()).
+ *  - One context per device; a context is not re-entrant.  All work is issued
+ *    on the context's stream (hyd_set_stream), asynchronously unless stated.
+ *  - Floating point follows the reference CPU backend operation by operation
+ *    (FP32, no contraction), so "reference" schedule images match it to the bit
+ *    up to the order of float reductions, which the render path does not have.
+ */
+#ifndef HYDRANT_HIP_H
+#define HYDRANT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hyd_ctx hyd_ctx;
+
+enum {
+  HYD_OK = 0,
+  HYD_ERR_ARG = 1,    /* invalid argument (null, size, range)            */
+  HYD_ERR_HIP = 2,    /* HIP runtime error (message carries hipGetErrorString) */
+  HYD_ERR_STATE = 3,  /* call order: e.g. render before pool/TF were set   */
+  HYD_ERR_RANGE = 4   /* input exceeds a documented limit                  */
+};
+
+/* Brick storage formats.  u8 is read as float(v)/255 (Texture3D<u8>
+ * NormalizedFloat, include/hydrant/bridge/texture_3d.hpp:74-95). */
+enum { HYD_FMT_U8 = 0, HYD_FMT_F16 = 1, HYD_FMT_F32 = 2 };
+
+/* VolumeRenderMode, ui/volume.schema.hpp:11-14 */
+enum { HYD_MODE_DEFAULT = 0, HYD_MODE_PARTITION = 1, HYD_MODE_PAGING = 2 };
+
+/* Step schedules.  REFERENCE = volume_shader.cu:32-81 (step escalation x4 at
+ * alpha > 0.85, x16 at > 0.93, no early termination).  LATTICE = sample k at
+ * entry + d*(k*step), no escalation, early termination when alpha > ert
+ * (brick-order independent; used by sort-last segments and the batched form). */
+enum { HYD_SCHED_REFERENCE = 0, HYD_SCHED_LATTICE = 1 };
+
+/* BlocksRenderMode, ui/blocks.schema.hpp:11-12 */
+enum { HYD_BLOCKS_VOLUME = 0, HYD_BLOCKS_SOLID = 1 };
+
+/* IsosurfaceRenderMode, ui/isosurface.schema.hpp:11-12 */
+enum { HYD_ISO_COLOR = 0, HYD_ISO_NORMAL = 1, HYD_ISO_POSITION = 2 };
+
+/* ViewArgs, include/hydrant/core/shader.hpp:63-68 */
+typedef struct {
+  float trans[16];  /* column-major: iet * inverse(lookAt(...)) */
+  float ctg_fovy_2; /* 1 / tan(fovy / 2) */
+  float ray_o[3];   /* camera position in object (block) space */
+} hyd_view;
+
+/* Camera, include/hydrant/core/scene.hpp:32-82 */
+typedef struct {
+  float position[3];
+  float target[3];
+  float up[3];
+  float fovy;       /* degrees (CameraPerspective.fovy, default 60) */
+} hyd_camera;
+
+/* Volume shader fields (IShaderTypeErased shader.hpp:16-22 + VolumeShader
+ * volume_shader.hpp:24-31) plus the density scale on the TF alpha
+ * (1.0 reproduces the reference, which has no density in Volume mode). */
+typedef struct {
+  float bbox_min[3];
+  float bbox_max[3];
+  float step;       /* BasicRenderer::update: du / 4 / sample_rate */
+  float du;         /* 1 / block_size */
+  int32_t max_steps;
+  int32_t mode;     /* HYD_MODE_* */
+  float rank;       /* Partition mode colour parameter */
+  float density;
+  int32_t schedule; /* HYD_SCHED_* */
+  float ert;        /* LATTICE early-termination alpha; <= 0 disables */
+} hyd_volume_params;
+
+/* VolumePixel film record, 64 bytes (volume_shader.hpp:11-15): the resumable
+ * per-ray state.  In LATTICE schedule step_size holds the next lattice index. */
+typedef struct {
+  float o[3];
+  float d[3];
+  int32_t nsteps;
+  int32_t step_size;
+  float v[4];
+  float theta[3];
+  float phi;
+} hyd_volume_pixel;
+
+/* VolumeFetchPixel, volume_shader.hpp:17-22 (32 bytes) */
+typedef struct {
+  float theta[3];
+  float phi;
+  float val[4];
+} hyd_fetch_pixel;
+
+/* One record of the per-sample trace: brick cell, page id and the 6 filter
+ * tap indices (CpuSampler::filter_linear, cpu_sampler.hpp:88-104). */
+typedef struct {
+  int32_t ix, iy, iz, pgid;
+  int32_t tix, tiy, tiz, tjx, tjy, tjz;
+} hyd_trace_rec;
+
+typedef struct {
+  float bbox_min[3];
+  float bbox_max[3];
+  float step;
+  int32_t max_steps;
+  int32_t mode;     /* HYD_BLOCKS_* */
+  float density;
+} hyd_blocks_params;
+
+typedef struct {
+  float bbox_min[3];
+  float bbox_max[3];
+  float step;
+  float du;
+  int32_t max_steps;
+  int32_t mode;          /* HYD_ISO_* */
+  float to_world[16];    /* inverse(iet), column-major (isosurface.cc:108) */
+  float eye_pos[3];
+  float light_pos[3];
+  float surface_color[3];
+  float isovalue;
+  int32_t corrected;     /* 0: reference crossing test; 1: bracket + bisection */
+  int32_t bisect_steps;  /* bisection iterations when corrected (8) */
+} hyd_iso_params;
+
+/* Isosurface output per pixel: rgba, depth (block units, +inf on miss), hit */
+typedef struct {
+  float v[4];
+  float depth;
+  int32_t hit;
+} hyd_iso_pixel;
+
+/* ---------------------------------------------------------------- core -- */
+const char *hyd_last_error(void);
+int32_t hyd_version(void);
+
+/* IRenderer device selection (basic_renderer.hpp:43-55): one context per device. */
+int32_t hyd_create(int32_t device, hyd_ctx **out);
+int32_t hyd_destroy(hyd_ctx *ctx);
+/* Issue all work on this HIP stream (a hipStream_t; NULL selects the device's
+ * null stream).  A new context uses a private non-blocking stream. */
+int32_t hyd_set_stream(hyd_ctx *ctx, void *stream);
+int32_t hyd_synchronize(hyd_ctx *ctx);
+
+/* ------------------------------------------------------ camera (host) -- */
+/* Camera::update_params(CameraOrbit), scene.hpp:68-81 (+ perspective fovy). */
+int32_t hyd_orbit_camera(const float arm[3], const float center[3], float fovy, hyd_camera *out);
+/* Raycaster::fill_ray_emit_args, raycaster.hpp:126-135, with Exhibit::get_iet
+ * (scene.hpp:20-29) for an exhibit of the given centre and size. */
+int32_t hyd_make_view(const hyd_camera *cam, const float ex_center[3], const float ex_size[3], hyd_view *out);
+/* Exhibit::get_iet and glm inverse (for the isosurface to_world). */
+int32_t hyd_exhibit_iet(const float ex_center[3], const float ex_size[3], float out[16]);
+int32_t hyd_mat4_inverse(const float m[16], float out[16]);
+
+/* --------------------------------------------------- resident state ---- */
+/* Brick pool: n_pages padded bricks of pad_edge^3 voxels, x fastest, brick
+ * after brick (LosslessBlockPagingServer block_storage,
+ * src/slave/paging/lossless_block_paging.cc:15-83).  Copied to the device. */
+int32_t hyd_upload_pool(hyd_ctx *ctx, int32_t fmt, int32_t pad_edge, int32_t n_pages, const void *host);
+/* Same, but the pool already lives on the device; the context keeps the pointer
+ * (not a copy), which must stay valid while the context renders from it. */
+int32_t hyd_bind_pool_device(hyd_ctx *ctx, int32_t fmt, int32_t pad_edge, int32_t n_pages, const void *pool_dev);
+/* Page table (BlockPaging, block_paging.hpp:38-44): vaddr grid of dims[0]*dims[1]*dims[2]
+ * int32 (x fastest, -1 = not resident) and per-page BlockSamplerMapping {k, b[3]}
+ * (block_paging.hpp:11-22). */
+int32_t hyd_set_page_table(hyd_ctx *ctx, const int32_t dims[3], const int32_t *vaddr, int32_t n_pages,
+                           const float *page_k, const float *page_b, int32_t lowest_blkcnt);
+/* Empty-space grid (chebyshev thumbnail, volume.cc:47-51): 0 = non-empty brick,
+ * k > 0 = L-inf brick distance to the nearest non-empty brick. */
+int32_t hyd_set_empty_grid(hyd_ctx *ctx, const int32_t dims[3], const int32_t *chebyshev);
+/* Per-brick mean thumbnail for Blocks.Volume (blocks.cc:44-50). */
+int32_t hyd_set_mean_grid(hyd_ctx *ctx, const int32_t dims[3], const float *mean);
+/* Transfer function table: n RGBA float entries (TransferFn, transfer_fn.hpp:14-28). */
+int32_t hyd_set_tf(hyd_ctx *ctx, int32_t n, const float *rgba);
+
+/* ----------------------------------------------------------- passes ---- */
+/* Ray-emit pass with the march fused (ray_emit_kernel_impl + VolumeShaderKernel::main,
+ * cuda_shading.cu:14-37, volume_shader.cu:32-81).  film_dev (hyd_volume_pixel
+ * per pixel) and rgba_dev (float4 per pixel) may each be NULL.  samples_dev, if
+ * not NULL, is a device uint64 to which the number of composited samples is
+ * added (slower counting variant). */
+int32_t hyd_render_volume(hyd_ctx *ctx, const hyd_volume_params *p, const hyd_view *view, int32_t width,
+                          int32_t height, hyd_volume_pixel *film_dev, float *rgba_dev,
+                          unsigned long long *samples_dev);
+/* Ray-march pass: resume every film pixel after the page table changed
+ * (ray_march_kernel_impl, cuda_shading.cu:41-56; shader.hpp:146-157). */
+int32_t hyd_resume_volume(hyd_ctx *ctx, const hyd_volume_params *p, int32_t width, int32_t height,
+                          hyd_volume_pixel *film_dev, float *rgba_dev, unsigned long long *samples_dev);
+/* Pixel pass (StdVec4Pixel::write_to, pixel_template.hpp:11-20): RGBA float
+ * -> RGB8, clear colour where alpha == 0. */
+int32_t hyd_pixel_pass(hyd_ctx *ctx, const float *rgba_dev, int32_t width, int32_t height,
+                       const uint8_t clear_color[3], uint8_t *rgb_dev);
+/* Fetch pass (VolumeShaderKernel::fetch, volume_shader.cu:23-30). */
+int32_t hyd_fetch_pass(hyd_ctx *ctx, const hyd_volume_pixel *film_dev, int32_t width, int32_t height,
+                       hyd_fetch_pixel *out_dev);
+/* In-GPU sort-last composite of n partial images (hyd_fetch_pixel, n images
+ * of width*height back to back) in z_order (host array of n indices), then
+ * sRGB + saturate (volume.cc:267-292).  rgba_dev / rgb_dev may be NULL. */
+int32_t hyd_composite(hyd_ctx *ctx, int32_t n, const hyd_fetch_pixel *partials_dev, const int32_t *z_order,
+                      int32_t width, int32_t height, float *rgba_dev, uint8_t *rgb_dev);
+/* Per-sample trace of one pixel (debug; synchronous; FP32 reference schedule):
+ * writes up to max_recs records to host memory, *n_out = count. */
+int32_t hyd_trace_volume(hyd_ctx *ctx, const hyd_volume_params *p, const hyd_view *view, int32_t width,
+                         int32_t height, int32_t x, int32_t y, hyd_trace_rec *out, int32_t max_recs,
+                         int32_t *n_out);
+
+/* Blocks.Volume / Blocks.Solid (VisBlocksShaderKernel::main, blocks_shader.cu:16-50). */
+int32_t hyd_render_blocks(hyd_ctx *ctx, const hyd_blocks_params *p, const hyd_view *view, int32_t width,
+                          int32_t height, float *rgba_dev);
+/* Isosurface (IsosurfaceShaderKernel::main, isosurface_shader.cu:50-125). */
+int32_t hyd_render_iso(hyd_ctx *ctx, const hyd_iso_params *p, const hyd_view *view, int32_t width,
+                       int32_t height, hyd_iso_pixel *out_dev, unsigned long long *samples_dev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYDRANT_HIP_H */

@@ -1,0 +1,295 @@
+"""ctypes wrapper of the oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.  The oracle is a
+C++ restatement of the reference's multi-thread CPU backend (oracle.hpp);
+precision 32 follows the reference's binary32 arithmetic operation by
+operation, precision 64 is the FP64 ground truth.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+d3 = C.c_double * 3
+
+
+class OView(C.Structure):
+    _fields_ = [("trans", C.c_double * 16), ("ctg_fovy_2", C.c_double), ("ray_o", d3)]
+
+
+class OVolumeArgs(C.Structure):
+    _fields_ = [
+        ("W", C.c_int), ("H", C.c_int), ("bbox_min", d3), ("bbox_max", d3), ("step", C.c_double),
+        ("du", C.c_double), ("max_steps", C.c_int), ("mode", C.c_int), ("rank", C.c_double),
+        ("density", C.c_double), ("schedule", C.c_int), ("ert", C.c_double), ("tf_n", C.c_int),
+        ("tf", C.POINTER(C.c_float)), ("cheb_dim", C.c_int * 3), ("cheb", C.POINTER(C.c_int)),
+        ("vaddr_dim", C.c_int * 3), ("vaddr", C.POINTER(C.c_int)), ("lowest_blkcnt", C.c_int),
+        ("page_k", C.POINTER(C.c_float)), ("page_b", C.POINTER(C.c_float)), ("pool", C.POINTER(C.c_float)),
+        ("pad", C.c_int),
+    ]
+
+
+class OBlocksArgs(C.Structure):
+    _fields_ = [
+        ("W", C.c_int), ("H", C.c_int), ("bbox_min", d3), ("bbox_max", d3), ("step", C.c_double),
+        ("max_steps", C.c_int), ("mode", C.c_int), ("density", C.c_double), ("dim", C.c_int * 3),
+        ("cheb", C.POINTER(C.c_int)), ("mean", C.POINTER(C.c_float)),
+    ]
+
+
+class OIsoArgs(C.Structure):
+    _fields_ = [
+        ("W", C.c_int), ("H", C.c_int), ("bbox_min", d3), ("bbox_max", d3), ("step", C.c_double),
+        ("du", C.c_double), ("max_steps", C.c_int), ("mode", C.c_int), ("to_world", C.c_double * 16),
+        ("eye_pos", d3), ("light_pos", d3), ("surface_color", d3), ("isovalue", C.c_double),
+        ("cheb_dim", C.c_int * 3), ("cheb", C.POINTER(C.c_int)), ("vaddr_dim", C.c_int * 3),
+        ("vaddr", C.POINTER(C.c_int)), ("page_k", C.POINTER(C.c_float)), ("page_b", C.POINTER(C.c_float)),
+        ("pool", C.POINTER(C.c_float)), ("pad", C.c_int), ("corrected", C.c_int), ("bisect_steps", C.c_int),
+    ]
+
+
+class OTrace(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("ix", "iy", "iz", "pgid", "tix", "tiy", "tiz", "tjx", "tjy", "tjz")]
+
+
+_lib = None
+
+
+def build() -> Path:
+    res = subprocess.run(["make", "-C", str(HERE)], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{res.stdout}\n{res.stderr}")
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        PD = C.POINTER(C.c_double)
+        L.orc_orbit_position.argtypes = [C.c_int, PD, PD]
+        L.orc_make_view.argtypes = [C.c_int, PD, PD, PD, C.c_double, PD, PD, C.POINTER(OView)]
+        L.orc_inverse.argtypes = [C.c_int, PD, PD]
+        L.orc_exhibit_iet.argtypes = [C.c_int, PD, PD, PD]
+        L.orc_gen_ray.argtypes = [C.c_int, C.POINTER(OView), C.c_int, C.c_int, C.c_int, C.c_int, PD]
+        L.orc_intersect.argtypes = [C.c_int, PD, PD, PD, PD, PD]
+        L.orc_brick_sample.argtypes = [C.c_int, C.POINTER(C.c_float), C.c_int, PD, C.POINTER(C.c_int)]
+        L.orc_brick_sample.restype = C.c_double
+        L.orc_tf_sample.argtypes = [C.c_int, C.POINTER(C.c_float), C.c_int, C.c_double, PD]
+        L.orc_volume_render.argtypes = [C.c_int, C.c_int, C.POINTER(OView), C.POINTER(OVolumeArgs), PD, PD,
+                                        C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.orc_volume_resume.argtypes = [C.c_int, C.c_int, C.POINTER(OVolumeArgs), PD, C.POINTER(C.c_longlong)]
+        L.orc_volume_trace.argtypes = [C.POINTER(OView), C.POINTER(OVolumeArgs), C.c_int, C.c_int,
+                                       C.POINTER(OTrace), C.c_int]
+        L.orc_pixel_pass.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_ubyte),
+                                     C.POINTER(C.c_ubyte)]
+        L.orc_composite.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_float), C.POINTER(C.c_ubyte)]
+        L.orc_blocks_render.argtypes = [C.c_int, C.c_int, C.POINTER(OView), C.POINTER(OBlocksArgs), PD]
+        L.orc_iso_render.argtypes = [C.c_int, C.c_int, C.POINTER(OView), C.POINTER(OIsoArgs), PD,
+                                     C.POINTER(C.c_longlong)]
+        _lib = L
+    return _lib
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _pf(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _pi(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def orbit_position(arm, prec=32) -> np.ndarray:
+    out = np.zeros(3)
+    lib().orc_orbit_position(prec, _pd(np.asarray(arm, np.float64)), _pd(out))
+    return out
+
+
+def make_view(pos, target, up, fovy, ex_center, ex_size, prec=32) -> OView:
+    v = OView()
+    a = [np.asarray(x, np.float64) for x in (pos, target, up, ex_center, ex_size)]
+    lib().orc_make_view(prec, _pd(a[0]), _pd(a[1]), _pd(a[2]), float(fovy), _pd(a[3]), _pd(a[4]), C.byref(v))
+    return v
+
+
+def view_from_dict(d: dict) -> OView:
+    v = OView()
+    v.trans[:] = [float(x) for x in d["trans"]]
+    v.ctg_fovy_2 = float(d["ctg_fovy_2"])
+    v.ray_o[:] = [float(x) for x in d["ray_o"]]
+    return v
+
+
+def view_to_dict(v: OView) -> dict:
+    return {"trans": np.array(v.trans[:]), "ctg_fovy_2": v.ctg_fovy_2, "ray_o": np.array(v.ray_o[:])}
+
+
+def gen_ray(view: OView, x, y, W, H, prec=32) -> np.ndarray:
+    out = np.zeros(6)
+    lib().orc_gen_ray(prec, C.byref(view), x, y, W, H, _pd(out))
+    return out
+
+
+def intersect(o, d, bmin, bmax, prec=32):
+    out = np.zeros(2)
+    a = [np.asarray(x, np.float64) for x in (o, d, bmin, bmax)]
+    hit = lib().orc_intersect(prec, _pd(a[0]), _pd(a[1]), _pd(a[2]), _pd(a[3]), _pd(out))
+    return bool(hit), out[0], out[1]
+
+
+def brick_sample(data: np.ndarray, u, prec=32):
+    data = np.ascontiguousarray(data, np.float32)
+    taps = np.zeros(6, np.int32)
+    r = lib().orc_brick_sample(prec, _pf(data), data.shape[0], _pd(np.asarray(u, np.float64)), _pi(taps))
+    return r, taps
+
+
+def tf_sample(tf: np.ndarray, s: float, prec=32) -> np.ndarray:
+    tf = np.ascontiguousarray(tf, np.float32)
+    out = np.zeros(4)
+    lib().orc_tf_sample(prec, _pf(tf), tf.size // 4, float(s), _pd(out))
+    return out
+
+
+def inverse(m: np.ndarray, prec=32) -> np.ndarray:
+    out = np.zeros(16)
+    lib().orc_inverse(prec, _pd(np.ascontiguousarray(m, np.float64)), _pd(out))
+    return out
+
+
+def exhibit_iet(center, size, prec=32) -> np.ndarray:
+    out = np.zeros(16)
+    lib().orc_exhibit_iet(prec, _pd(np.asarray(center, np.float64)), _pd(np.asarray(size, np.float64)), _pd(out))
+    return out
+
+
+class VolumeScene:
+    """Arguments of one volume render held alive for the oracle."""
+
+    def __init__(self, W, H, bbox_min, bbox_max, step, du, tf, cheb, vaddr, page_k, page_b, pool_values, pad,
+                 max_steps=4_000_000, mode=0, rank=0.0, density=1.0, schedule=0, ert=0.0, lowest_blkcnt=0):
+        self.keep = [np.ascontiguousarray(tf, np.float32).reshape(-1), np.ascontiguousarray(cheb, np.int32),
+                     np.ascontiguousarray(vaddr, np.int32), np.ascontiguousarray(page_k, np.float32),
+                     np.ascontiguousarray(page_b, np.float32).reshape(-1),
+                     np.ascontiguousarray(pool_values, np.float32)]
+        tfa, ch, va, pk, pb, pv = self.keep
+        a = OVolumeArgs()
+        a.W, a.H = W, H
+        a.bbox_min[:] = [float(x) for x in bbox_min]
+        a.bbox_max[:] = [float(x) for x in bbox_max]
+        a.step, a.du = float(step), float(du)
+        a.max_steps, a.mode, a.rank, a.density = int(max_steps), int(mode), float(rank), float(density)
+        a.schedule, a.ert = int(schedule), float(ert)
+        a.tf_n, a.tf = tfa.size // 4, _pf(tfa)
+        a.cheb_dim[:] = list(ch.shape[::-1])
+        a.cheb = _pi(ch)
+        a.vaddr_dim[:] = list(va.shape[::-1])
+        a.vaddr = _pi(va)
+        a.lowest_blkcnt = int(lowest_blkcnt)
+        a.page_k, a.page_b, a.pool, a.pad = _pf(pk), _pf(pb), _pf(pv), int(pad)
+        self.args = a
+
+    def render(self, view: OView, prec=32, nthreads=0, film=True, counts=False):
+        a = self.args
+        P = a.W * a.H
+        film_a = np.zeros((a.H, a.W, 16)) if film else None
+        rgba = np.zeros((a.H, a.W, 4))
+        cnt = np.zeros((a.H, a.W), np.int64) if counts else None
+        tot = C.c_longlong(0)
+        lib().orc_volume_render(prec, nthreads, C.byref(view), C.byref(a), _pd(film_a) if film else None, _pd(rgba),
+                                cnt.ctypes.data_as(C.POINTER(C.c_longlong)) if counts else None, C.byref(tot))
+        return {"film": film_a, "rgba": rgba, "counts": cnt, "samples": tot.value}
+
+    def resume(self, film: np.ndarray, prec=32, nthreads=0):
+        film = np.ascontiguousarray(film, np.float64).copy()
+        tot = C.c_longlong(0)
+        lib().orc_volume_resume(prec, nthreads, C.byref(self.args), _pd(film), C.byref(tot))
+        return {"film": film, "rgba": film[..., 8:12].copy(), "samples": tot.value}
+
+    def trace(self, view: OView, x, y, max_recs=8192) -> np.ndarray:
+        recs = (OTrace * max_recs)()
+        n = lib().orc_volume_trace(C.byref(view), C.byref(self.args), x, y, recs, max_recs)
+        return np.frombuffer(recs, dtype=np.int32).reshape(max_recs, 10)[:n].copy()
+
+
+def pixel_pass(rgba: np.ndarray, clear=(0, 0, 0)) -> np.ndarray:
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    H, W = rgba.shape[:2]
+    out = np.zeros((H, W, 3), np.uint8)
+    cc = (C.c_ubyte * 3)(*clear)
+    lib().orc_pixel_pass(W, H, _pf(rgba), cc, out.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    return out
+
+
+def composite(partials: np.ndarray, z_order) -> tuple[np.ndarray, np.ndarray]:
+    partials = np.ascontiguousarray(partials, np.float32)
+    N, H, W = partials.shape[:3]
+    z = np.ascontiguousarray(z_order, np.int32)
+    rgba = np.zeros((H, W, 4), np.float32)
+    rgb = np.zeros((H, W, 3), np.uint8)
+    lib().orc_composite(N, W, H, _pf(partials), _pi(z), _pf(rgba), rgb.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    return rgba, rgb
+
+
+def blocks_render(view: OView, W, H, bbox_min, bbox_max, step, cheb, mean, mode=0, density=1e-2,
+                  max_steps=4_000_000, prec=32, nthreads=0) -> np.ndarray:
+    ch = np.ascontiguousarray(cheb, np.int32)
+    mn = np.ascontiguousarray(mean, np.float32)
+    a = OBlocksArgs()
+    a.W, a.H = W, H
+    a.bbox_min[:] = [float(x) for x in bbox_min]
+    a.bbox_max[:] = [float(x) for x in bbox_max]
+    a.step, a.max_steps, a.mode, a.density = float(step), int(max_steps), int(mode), float(density)
+    a.dim[:] = list(ch.shape[::-1])
+    a.cheb, a.mean = _pi(ch), _pf(mn)
+    rgba = np.zeros((H, W, 4))
+    lib().orc_blocks_render(prec, nthreads, C.byref(view), C.byref(a), _pd(rgba))
+    return rgba
+
+
+class IsoScene:
+    def __init__(self, W, H, bbox_min, bbox_max, step, du, to_world, eye_pos, light_pos, surface_color, isovalue,
+                 cheb, vaddr, page_k, page_b, pool_values, pad, mode=0, corrected=0, bisect_steps=8,
+                 max_steps=4_000_000):
+        self.keep = [np.ascontiguousarray(cheb, np.int32), np.ascontiguousarray(vaddr, np.int32),
+                     np.ascontiguousarray(page_k, np.float32), np.ascontiguousarray(page_b, np.float32).reshape(-1),
+                     np.ascontiguousarray(pool_values, np.float32)]
+        ch, va, pk, pb, pv = self.keep
+        a = OIsoArgs()
+        a.W, a.H = W, H
+        a.bbox_min[:] = [float(x) for x in bbox_min]
+        a.bbox_max[:] = [float(x) for x in bbox_max]
+        a.step, a.du, a.max_steps, a.mode = float(step), float(du), int(max_steps), int(mode)
+        a.to_world[:] = [float(x) for x in np.asarray(to_world).reshape(-1)]
+        a.eye_pos[:] = [float(x) for x in eye_pos]
+        a.light_pos[:] = [float(x) for x in light_pos]
+        a.surface_color[:] = [float(x) for x in surface_color]
+        a.isovalue = float(isovalue)
+        a.cheb_dim[:] = list(ch.shape[::-1])
+        a.cheb = _pi(ch)
+        a.vaddr_dim[:] = list(va.shape[::-1])
+        a.vaddr = _pi(va)
+        a.page_k, a.page_b, a.pool, a.pad = _pf(pk), _pf(pb), _pf(pv), int(pad)
+        a.corrected, a.bisect_steps = int(corrected), int(bisect_steps)
+        self.args = a
+
+    def render(self, view: OView, prec=32, nthreads=0):
+        a = self.args
+        out = np.zeros((a.H, a.W, 6))
+        tot = C.c_longlong(0)
+        lib().orc_iso_render(prec, nthreads, C.byref(view), C.byref(a), _pd(out), C.byref(tot))
+        return {"px": out, "samples": tot.value}
