@@ -1,0 +1,46 @@
+"""The C-ABI library loads and exports every symbol the public headers declare
+(no compute calls: this runs without a GPU)."""
+import re
+from pathlib import Path
+
+from hydrant_amd import _lib as L
+
+INCLUDE = Path(__file__).resolve().parent.parent / "include"
+
+
+def declared_symbols():
+    names = set()
+    for h in INCLUDE.glob("*.h"):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(hyd_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_headers_declare_symbols():
+    names = declared_symbols()
+    assert {"hyd_create", "hyd_render_volume", "hyd_composite", "hyd_pack_bricks"} <= names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    missing = [n for n in sorted(declared_symbols()) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_table_covers_headers():
+    assert declared_symbols() <= set(L.SIGNATURES), declared_symbols() - set(L.SIGNATURES)
+
+
+def test_host_only_calls_without_gpu():
+    import ctypes as C
+    import numpy as np
+
+    cam = L.Camera()
+    arm = np.array([25, 65, 2.2], np.float32)
+    L.call("hyd_orbit_camera", L.ptr(arm, C.c_float), None, 60.0, C.byref(cam))
+    assert abs(cam.position[0] - 0.842649) < 1e-5
+    assert L.lib().hyd_version() == 1
+    code = L.lib().hyd_orbit_camera(None, None, 60.0, None)
+    assert code == L.HYD_ERR_ARG and b"null" in L.lib().hyd_last_error()
