@@ -104,6 +104,93 @@ def bench_plain(torch, args):
     return res
 
 
+def bench_batched(torch, args, plain_ms_per_sample=None):
+    """96 views (resolutions in {256..1024}^2) of three blob volumes (256^3, 512^3, 768^3)
+    as FP16 bricks, one persistent launch replayed from a HIP graph; launch shape from
+    the autotuner."""
+    from hydrant_amd import _lib as L
+    from hydrant_amd import workloads as W
+    from hydrant_amd.hip import Context
+
+    t0 = time.perf_counter()
+    wl = W.batched(fmt=L.FMT_F16)
+    t_gen = time.perf_counter() - t0
+    ctx = Context(0)
+    ctx.upload_pool(wl.fmt, wl.bs + 2 * wl.apron, wl.pool)
+    ctx.set_tf(wl.tf)
+    ctx.set_volumes(wl.volume_descs())
+    b = ctx.batch(wl.hip_views())
+    out = b.new_output()
+    if args.config:
+        from hydrant_amd.hip import batch_config
+
+        tw, un, wpc = (int(x) for x in args.config.split(","))
+        best, tune_ms = batch_config(tw, un, wpc), None
+    else:
+        best, tune_ms = b.autotune(wl.density, wl.ert, out, reps=2)
+    _, cnt = b.render(best, wl.density, wl.ert, out=out, count=True)
+    torch.cuda.synchronize()
+    n_samples = int(cnt.item())
+    b.capture(best, wl.density, wl.ert, out)
+
+    def step():
+        b.replay()
+
+    wall, ev = time_kernel(torch, step, args.steps, args.warmup)
+    ms_step = wall * 1e3 / args.steps
+    k_ms = float(np.mean(ev))
+    n_px = b.n_pixels
+    alg = n_samples * 8 * 2 + n_px * 16
+    res = {"n_samples": n_samples, "ms_per_step": ms_step, "kernel_ms": k_ms,
+           "msamples_per_s": n_samples / (ms_step * 1e-3) / 1e6, "alg_bytes": alg,
+           "achieved_gbs": alg / (k_ms * 1e-3) / 1e9, "config": best.as_dict(), "autotune_best_ms": tune_ms,
+           "n_pixels": n_px, "gen_s": t_gen, "wl": wl, "batch": b, "out": out, "ctx": ctx}
+    return res
+
+
+def batched_errors(res, n_views: int = 6):
+    """Per-view max-abs error of the batched FP16 output against the FP64 oracle (exact u8
+    source values) on the first n_views views: full frame up to 512^2, else a 64x64 crop
+    plus every 7th pixel."""
+    from tests.helpers import error_pixels, exact_pool_values, oracle_view, oracle_volume
+
+    wl, b, out = res["wl"], res["batch"], res["out"]
+    errs = []
+    for i in range(min(n_views, len(wl.views))):
+        sc = wl.scene(i)
+        xy = error_pixels(sc.W, sc.H)
+        osc = oracle_volume(sc, pool_values=exact_pool_values(sc), exact_mapping=True)
+        ref, _ = osc.render_pixels(oracle_view(sc), xy, prec=64, nthreads=cpu_threads())
+        img = b.view_image(out, i).cpu().numpy()
+        got = img[xy[:, 1], xy[:, 0]].astype(np.float64)
+        errs.append({"view": i, "res": sc.W, "volume": wl.views[i]["volume"], "pixels": int(xy.shape[0]),
+                     "max_abs": float(np.max(np.abs(got - ref)))})
+    return errs
+
+
+def cpu_baseline_batched(res, budget_s: float = 12.0):
+    """FP32 oracle (lattice schedule) on whole views of the batched workload until the budget
+    is spent; FP16 storage is replaced by the exact u8 values, which costs the CPU the same."""
+    from tests.helpers import exact_pool_values, oracle_view, oracle_volume
+
+    wl = res["wl"]
+    nthreads = cpu_threads()
+    samples, t, views = 0, 0.0, 0
+    for i in range(len(wl.views)):
+        sc = wl.scene(i)
+        osc = oracle_volume(sc, pool_values=exact_pool_values(sc), exact_mapping=True)
+        t0 = time.perf_counter()
+        r = osc.render(oracle_view(sc), 32, nthreads=nthreads, film=False)
+        t += time.perf_counter() - t0
+        samples += r["samples"]
+        views += 1
+        if t > budget_s:
+            break
+    return {"value": round(samples / t / 1e6, 3), "unit": "Msamples/s", "cores": nthreads, "kind": "port",
+            "sample": f"first {views} of {len(wl.views)} batched views rendered whole by the FP32 oracle "
+                      f"(lattice schedule, {samples} samples in {t:.1f} s, {cpu_model()})"}
+
+
 def cpu_baseline_plain(sc, budget_s: float = 12.0):
     """Oracle FP32 multi-thread path (the reference CPU backend restated) on the same
     workload; whole frames, 1 warm-up, then as many frames as fit the budget."""
@@ -130,10 +217,17 @@ def cpu_baseline_plain(sc, budget_s: float = 12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain", choices=["plain"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="batched", choices=["batched", "plain"],
+                    help="headline workload (the other one is reported alongside)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg and the FP64 error check")
+    ap.add_argument("--err-views", type=int, default=6, help="batched views checked against the FP64 oracle")
+    ap.add_argument("--only", choices=["plain", "batched"],
+                    help="profiling aid: run only this workload's timed launches and print its line")
+    ap.add_argument("--config", help="batched launch shape tile_w,unroll,waves_per_cu (skips the autotuner)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="profiling aid: copy a MIB-sized buffer 3 times (scales FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -146,38 +240,80 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
 
-    res = bench_plain(torch, args)
-    sc = res["scene"]
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4}), flush=True)
+        return
+    if args.only:
+        res = bench_plain(torch, args) if args.only == "plain" else bench_batched(torch, args)
+        print(json.dumps({k: v for k, v in res.items() if isinstance(v, (int, float, str, dict))}), flush=True)
+        return
+
+    plain = bench_plain(torch, args)
+    print(f"# plain fp32: {plain['msamples_per_s']:.1f} Msamples/s, {plain['ms_per_step']:.3f} ms/frame",
+          file=sys.stderr, flush=True)
+    batched = bench_batched(torch, args)
+    print(f"# batched fp16: {batched['msamples_per_s']:.1f} Msamples/s, {batched['ms_per_step']:.3f} ms/batch, "
+          f"config {batched['config']}", file=sys.stderr, flush=True)
+
+    head = batched if args.workload == "batched" else plain
+    plain_info = {"msamples_per_s": round(plain["msamples_per_s"], 3), "ms_per_frame": round(plain["ms_per_step"], 4),
+                  "samples_per_frame": plain["n_samples"], "kernel": "volume_kernel<f32> (reference schedule)",
+                  "kernel_ms": round(plain["kernel_ms"], 4)}
+    batched_info = {"msamples_per_s": round(batched["msamples_per_s"], 3),
+                    "ms_per_batch": round(batched["ms_per_step"], 4), "samples_per_batch": batched["n_samples"],
+                    "views": len(batched["wl"].views), "pixels": batched["n_pixels"], "volumes": [256, 512, 768],
+                    "brick_format": "f16", "launch_config": batched["config"],
+                    "kernel": "batch_kernel<f16> (lattice schedule, HIP graph replay)",
+                    "kernel_ms": round(batched["kernel_ms"], 4),
+                    "ratio_vs_plain_fp32": round(batched["msamples_per_s"] / plain["msamples_per_s"], 3)}
+    if head is batched:
+        workload = ("batched_fp16: 96 views (seeded resolutions in {256,384,512,768,1024}^2, seeded orbit "
+                    "angles) of three Gaussian-blob volumes 256^3/512^3/768^3 as FP16 64^3+1 bricks, one "
+                    "persistent launch replayed from a HIP graph, lattice schedule (ERT 0.9995), README TF, "
+                    "density 1e-2, step 0.5 voxel")
+        bytes_per_voxel, kern = 2, "batch_kernel<f16>"
+    else:
+        workload = ("plain_volume_fp32: 256^3 u8 Gaussian-blob volume as one FP32 brick, 512x512, orbit "
+                    "[25,65,2.2], README 9-point TF, step 0.5 voxel, density 1e-2, reference schedule")
+        bytes_per_voxel, kern = 4, "volume_kernel<f32>"
     out = {
         "metric": METRIC,
-        "value": round(res["msamples_per_s"], 3),
+        "value": round(head["msamples_per_s"], 3),
         "unit": "Msamples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(res["ms_per_step"], 4),
+        "ms_per_step": round(head["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "plain_volume_fp32: 256^3 u8 Gaussian-blob volume as one FP32 brick, 512x512, "
-                               "orbit [25,65,2.2], README 9-point TF, step 0.5 voxel, density 1e-2, "
-                               "reference schedule",
-                   "resolution": [sc.W, sc.H], "volume": [256, 256, 256], "brick_format": "f32",
-                   "samples_per_frame": res["n_samples"]},
-        "ms_per_frame": round(res["ms_per_step"], 4),
-        "roofline": {"bound": "hbm", "achieved": round(res["achieved_gbs"], 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(res["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "volume_kernel<f32>", "kernel_ms": round(res["kernel_ms"], 4),
-                     "alg_bytes_per_launch": res["alg_bytes"]},
+        "config": {"workload": workload, "samples_per_step": head["n_samples"]},
+        "plain_fp32": plain_info,
+        "batched_fp16": batched_info,
+        "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": kern, "kernel_ms": round(head["kernel_ms"], 4),
+                     "alg_bytes_per_launch": head["alg_bytes"],
+                     "alg_bytes_rule": f"samples x 8 taps x {bytes_per_voxel} B + pixels x 16 B"},
     }
     if not args.no_cpu:
-        cb = cpu_baseline_plain(sc)
-        err = float(np.max(np.abs(res["rgba"].cpu().numpy().astype(np.float64) - cb["rgba"])))
-        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
-        out["cpu_baseline"]["value"] = round(cb["value"], 3)
-        out["max_abs_vs_cpu"] = err
+        errs = batched_errors(batched, args.err_views)
+        out["batched_fp16"]["max_abs_vs_fp64"] = max(e["max_abs"] for e in errs)
+        out["batched_fp16"]["per_view_error"] = errs
+        if head is batched:
+            out["cpu_baseline"] = cpu_baseline_batched(batched)
+        else:
+            cb = cpu_baseline_plain(plain["scene"])
+            out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+            out["cpu_baseline"]["value"] = round(cb["value"], 3)
     print(json.dumps(out), flush=True)
 
 

@@ -75,6 +75,23 @@ I32 = C.c_int32
 PI32 = C.POINTER(C.c_int32)
 PF = C.POINTER(C.c_float)
 
+
+class VolumeDesc(C.Structure):
+    _fields_ = [("grid", i3), ("bs", I32), ("apron", I32), ("bbox_max", f3), ("step", C.c_float),
+                ("value_scale", C.c_float), ("vaddr", PI32), ("chebyshev", PI32)]
+
+
+class BatchView(C.Structure):
+    _fields_ = [("view", View), ("width", I32), ("height", I32), ("volume", I32), ("reserved", I32),
+                ("out_offset", C.c_int64)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("tile_w", I32), ("unroll", I32), ("waves_per_cu", I32)]
+
+    def as_dict(self):
+        return {"tile_w": self.tile_w, "tile_h": 8, "unroll": self.unroll, "waves_per_cu": self.waves_per_cu}
+
 # name -> (restype, argtypes); every symbol the public headers declare.
 SIGNATURES = {
     # hydrant_hip.h
@@ -103,6 +120,15 @@ SIGNATURES = {
                                C.POINTER(TraceRec), I32, PI32]),
     "hyd_render_blocks": (I32, [P, C.POINTER(BlocksParams), C.POINTER(View), I32, I32, P]),
     "hyd_render_iso": (I32, [P, C.POINTER(IsoParams), C.POINTER(View), I32, I32, P, P]),
+    "hyd_set_volumes": (I32, [P, I32, C.POINTER(VolumeDesc)]),
+    "hyd_batch_create": (I32, [P, C.POINTER(BatchView), I32, C.POINTER(P)]),
+    "hyd_batch_destroy": (I32, [P]),
+    "hyd_batch_output_pixels": (I32, [P, C.POINTER(C.c_int64)]),
+    "hyd_batch_render": (I32, [P, P, C.POINTER(BatchConfig), C.c_float, C.c_float, P, P]),
+    "hyd_batch_capture": (I32, [P, P, C.POINTER(BatchConfig), C.c_float, C.c_float, P]),
+    "hyd_batch_replay": (I32, [P, P]),
+    "hyd_batch_autotune": (I32, [P, P, C.c_float, C.c_float, P, I32, C.POINTER(BatchConfig),
+                                 C.POINTER(C.c_float)]),
     # hydrant_dataset.h
     "hyd_gen_blobs_u8": (I32, [I32, I32, I32, I32, C.c_uint64, C.c_double, C.c_double, C.c_double,
                                C.c_double, C.c_double, P]),
@@ -110,7 +136,7 @@ SIGNATURES = {
     "hyd_gen_sdf_spheres_f16": (I32, [I32, I32, C.c_float, C.c_float, C.c_float, C.c_uint64, P]),
     "hyd_f32_to_f16": (C.c_uint16, [C.c_float]),
     "hyd_f16_to_f32": (C.c_float, [C.c_uint16]),
-    "hyd_pack_bricks": (I32, [P, I32, PI32, I32, I32, I32, PI32, I32, P]),
+    "hyd_pack_bricks": (I32, [P, I32, PI32, I32, I32, I32, I32, PI32, I32, P]),
     "hyd_brick_stats": (I32, [P, I32, PI32, I32, I32, PF, PF, PF]),
     "hyd_tf_alpha_zero_on": (I32, [PF, I32, C.c_float, C.c_float]),
     "hyd_chebyshev": (I32, [PI32, C.POINTER(C.c_uint8), PI32]),

@@ -29,12 +29,6 @@ template <class T> void free_dev(T *&p) {
   p = nullptr;
 }
 
-int32_t activate(Ctx *c) {
-  HYD_REQUIRE(c, HYD_ERR_ARG, "null context");
-  HYD_CHECK_HIP(hipSetDevice(c->device));
-  return HYD_OK;
-}
-
 template <class T> int32_t upload(T *&dst, const void *src, size_t bytes, hipStream_t s) {
   free_dev(dst);
   HYD_CHECK_HIP(hipMalloc((void **)&dst, bytes));
@@ -90,11 +84,16 @@ void fill_view(const hyd_view *v, float trans[16], float &ctg, f3 &ray_o) {
 }
 
 }  // namespace
+
+int32_t activate(Ctx *c) {
+  HYD_REQUIRE(c, HYD_ERR_ARG, "null context");
+  HYD_CHECK_HIP(hipSetDevice(c->device));
+  return HYD_OK;
+}
+
 }  // namespace hyd
 
 using namespace hyd;
-
-struct hyd_ctx : Ctx {};
 
 extern "C" {
 
@@ -111,6 +110,9 @@ int32_t hyd_create(int32_t device, hyd_ctx **out) {
   HYD_CHECK_HIP(hipSetDevice(device));
   hyd_ctx *c = new hyd_ctx();
   c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cus = prop.multiProcessorCount;
+  if (c->n_cus <= 0) c->n_cus = 256;
   if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     set_error("hipStreamCreateWithFlags failed");
@@ -139,6 +141,8 @@ int32_t hyd_destroy(hyd_ctx *c) {
   free_dev(c->own_mean);
   free_dev(c->own_tf);
   free_dev(c->own_lut);
+  free_dev(c->own_vols);
+  free_dev(c->own_cells);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return HYD_OK;

@@ -1,0 +1,292 @@
+// C ABI of the batched renderer: multi-volume set, batch plans, HIP-graph
+// capture/replay and the launch-shape autotuner.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "hyd_batch.hpp"
+
+namespace hyd {
+namespace {
+
+int32_t dev_upload(void **dst, const void *src, size_t bytes, hipStream_t s) {
+  if (*dst) (void)hipFree(*dst);
+  *dst = nullptr;
+  HYD_CHECK_HIP(hipMalloc(dst, bytes));
+  HYD_CHECK_HIP(hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, s));
+  HYD_CHECK_HIP(hipStreamSynchronize(s));
+  return HYD_OK;
+}
+
+bool valid_config(const hyd_batch_config *c) {
+  return c && (c->tile_w == 8 || c->tile_w == 16 || c->tile_w == 32) &&
+         (c->unroll == 1 || c->unroll == 2 || c->unroll == 4) && c->waves_per_cu >= 1 && c->waves_per_cu <= 32;
+}
+
+int tw_slot(int tw) { return tw == 8 ? 0 : tw == 16 ? 1 : 2; }
+
+}  // namespace
+}  // namespace hyd
+
+using namespace hyd;
+
+struct hyd_batch {
+  std::vector<hyd_batch_view> views;
+  BatchViewDev *d_views = nullptr;
+  uint32_t *d_tiles[3] = {nullptr, nullptr, nullptr};   // per tile width 8 / 16 / 32
+  std::vector<int32_t> chunks[3];
+  unsigned *d_counters = nullptr;
+  int64_t total_px = 0;
+  // captured graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capture_stream = nullptr;
+};
+
+namespace {
+
+int32_t ensure_tiles(hyd_batch *b, int tw, hipStream_t s) {
+  const int slot = tw_slot(tw);
+  if (b->d_tiles[slot]) return HYD_OK;
+  std::vector<uint32_t> tiles;
+  for (size_t v = 0; v < b->views.size(); ++v) {
+    const int nx = (b->views[v].width + tw - 1) / tw, ny = (b->views[v].height + 7) / 8;
+    HYD_REQUIRE(nx <= 256 && ny <= 256, HYD_ERR_RANGE, "view %zu: %d x %d tiles exceeds 256 per axis", v, nx, ny);
+    for (int ty = 0; ty < ny; ++ty)
+      for (int tx = 0; tx < nx; ++tx) tiles.push_back((uint32_t)v << 16 | (uint32_t)ty << 8 | (uint32_t)tx);
+  }
+  std::vector<int32_t> &ch = b->chunks[slot];
+  ch.resize(kXcds + 1);
+  for (int i = 0; i <= kXcds; ++i) ch[i] = (int32_t)((int64_t)tiles.size() * i / kXcds);
+  void *p = nullptr;
+  int32_t st = dev_upload(&p, tiles.data(), tiles.size() * sizeof(uint32_t), s);
+  if (st) return st;
+  b->d_tiles[slot] = static_cast<uint32_t *>(p);
+  return HYD_OK;
+}
+
+int32_t fill_args(Ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float density, float ert, float *rgba_dev,
+                  unsigned long long *samples_dev, BatchArgs &a, int &grid) {
+  HYD_REQUIRE(valid_config(cfg), HYD_ERR_ARG, "invalid batch config (tile_w 8/16/32, unroll 1/2/4, waves_per_cu 1..32)");
+  HYD_REQUIRE(rgba_dev, HYD_ERR_ARG, "null output");
+  HYD_REQUIRE(c->own_cells && c->n_vols > 0, HYD_ERR_STATE, "no volume set: call hyd_set_volumes first");
+  HYD_REQUIRE(c->pool.data && c->tf, HYD_ERR_STATE, "batched render needs a pool and a transfer function");
+  HYD_REQUIRE(c->pool.pad == c->vol_pad, HYD_ERR_STATE, "pool pad edge %d differs from the volume set's %d",
+              c->pool.pad, c->vol_pad);
+  int32_t st = ensure_tiles(b, cfg->tile_w, c->stream);
+  if (st) return st;
+  const int slot = tw_slot(cfg->tile_w);
+  std::memset(&a, 0, sizeof(a));
+  a.tiles = b->d_tiles[slot];
+  for (int i = 0; i <= kXcds; ++i) a.chunk_start[i] = b->chunks[slot][i];
+  a.counters = b->d_counters;
+  a.views = b->d_views;
+  a.vols = static_cast<const BatchVolDev *>(c->own_vols);
+  a.cells = c->own_cells;
+  a.pool = c->pool.data;
+  a.pad = c->pool.pad;
+  a.page_elems = (uint32_t)c->pool.pad * c->pool.pad * c->pool.pad;
+  a.tf = c->tf;
+  a.tf_n = c->tf_n;
+  a.density = density;
+  a.ert = ert;
+  a.max_steps = 4000000;
+  a.out = reinterpret_cast<float4 *>(rgba_dev);
+  a.samples = samples_dev;
+  grid = c->n_cus * cfg->waves_per_cu;
+  return HYD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t hyd_set_volumes(hyd_ctx *c, int32_t n, const hyd_volume_desc *vols) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(n >= 1 && vols, HYD_ERR_ARG, "empty volume set");
+  std::vector<BatchVolDev> dv(n);
+  std::vector<int32_t> cells;
+  int pad = -1;
+  for (int v = 0; v < n; ++v) {
+    const hyd_volume_desc &d = vols[v];
+    HYD_REQUIRE(d.grid[0] > 0 && d.grid[1] > 0 && d.grid[2] > 0 && d.vaddr && d.chebyshev, HYD_ERR_ARG,
+                "volume %d: invalid grid", v);
+    HYD_REQUIRE(d.bs > 0 && d.apron >= 1, HYD_ERR_ARG, "volume %d: block size > 0 and apron >= 1 required", v);
+    HYD_REQUIRE(d.step > 0.f, HYD_ERR_ARG, "volume %d: step must be > 0", v);
+    const int p = d.bs + 2 * d.apron;
+    HYD_REQUIRE(pad < 0 || pad == p, HYD_ERR_ARG, "volume %d: every volume must share one padded brick edge", v);
+    pad = p;
+    BatchVolDev &o = dv[v];
+    std::memcpy(o.grid, d.grid, sizeof(o.grid));
+    o.cell_off = (int32_t)cells.size();
+    o.bs = d.bs;
+    o.apron = d.apron;
+    std::memcpy(o.bmax, d.bbox_max, sizeof(o.bmax));
+    o.step = d.step;
+    HYD_REQUIRE(d.value_scale > 0.f, HYD_ERR_ARG, "volume %d: value_scale must be > 0", v);
+    o.value_scale = d.value_scale;
+    const size_t nc = (size_t)d.grid[0] * d.grid[1] * d.grid[2];
+    for (size_t i = 0; i < nc; ++i) {
+      const int32_t cd = d.chebyshev[i], pg = d.vaddr[i];
+      HYD_REQUIRE(cd >= 0 && pg >= -1, HYD_ERR_RANGE, "volume %d cell %zu: invalid chebyshev/vaddr", v, i);
+      if (cd == 0 && c->pool.data)
+        HYD_REQUIRE(pg < c->pool.n_pages, HYD_ERR_RANGE, "volume %d cell %zu: page %d beyond the pool", v, i, pg);
+      cells.push_back(cd > 0 ? -1 - cd : pg);
+    }
+  }
+  if (c->own_vols) (void)hipFree(c->own_vols);
+  if (c->own_cells) (void)hipFree(c->own_cells);
+  c->own_vols = nullptr;
+  c->own_cells = nullptr;
+  c->n_vols = 0;
+  void *pv = nullptr;
+  if ((st = dev_upload(&pv, dv.data(), dv.size() * sizeof(BatchVolDev), c->stream))) return st;
+  c->own_vols = pv;
+  void *pcells = nullptr;
+  if ((st = dev_upload(&pcells, cells.data(), cells.size() * sizeof(int32_t), c->stream))) return st;
+  c->own_cells = static_cast<int32_t *>(pcells);
+  c->n_vols = n;
+  c->vol_pad = pad;
+  return HYD_OK;
+}
+
+int32_t hyd_batch_create(hyd_ctx *c, const hyd_batch_view *views, int32_t n_views, hyd_batch **out) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(out && views && n_views >= 1 && n_views <= 65535, HYD_ERR_ARG, "1..65535 views required");
+  HYD_REQUIRE(c->n_vols > 0, HYD_ERR_STATE, "no volume set: call hyd_set_volumes first");
+  hyd_batch *b = new hyd_batch();
+  b->views.assign(views, views + n_views);
+  std::vector<BatchViewDev> dv(n_views);
+  for (int i = 0; i < n_views; ++i) {
+    const hyd_batch_view &v = views[i];
+    if (!(v.width > 0 && v.height > 0 && v.volume >= 0 && v.volume < c->n_vols && v.out_offset >= 0)) {
+      delete b;
+      set_error("view %d: invalid size, volume or output offset", i);
+      return HYD_ERR_ARG;
+    }
+    std::memcpy(dv[i].trans, v.view.trans, sizeof(dv[i].trans));
+    dv[i].ctg = v.view.ctg_fovy_2;
+    std::memcpy(dv[i].ray_o, v.view.ray_o, sizeof(dv[i].ray_o));
+    dv[i].W = v.width;
+    dv[i].H = v.height;
+    dv[i].volume = v.volume;
+    dv[i].out_offset = v.out_offset;
+    b->total_px = std::max<int64_t>(b->total_px, v.out_offset + (int64_t)v.width * v.height);
+  }
+  void *p = nullptr;
+  if ((st = dev_upload(&p, dv.data(), dv.size() * sizeof(BatchViewDev), c->stream))) { delete b; return st; }
+  b->d_views = static_cast<BatchViewDev *>(p);
+  if (hipMalloc((void **)&b->d_counters, kXcds * sizeof(unsigned)) != hipSuccess) {
+    (void)hipFree(b->d_views);
+    delete b;
+    set_error("hipMalloc of the work counters failed");
+    return HYD_ERR_HIP;
+  }
+  *out = b;
+  return HYD_OK;
+}
+
+int32_t hyd_batch_destroy(hyd_batch *b) {
+  if (!b) return HYD_OK;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+  (void)hipFree(b->d_views);
+  for (auto *t : b->d_tiles) if (t) (void)hipFree(t);
+  (void)hipFree(b->d_counters);
+  delete b;
+  return HYD_OK;
+}
+
+int32_t hyd_batch_output_pixels(const hyd_batch *b, int64_t *n_out) {
+  HYD_REQUIRE(b && n_out, HYD_ERR_ARG, "null argument");
+  *n_out = b->total_px;
+  return HYD_OK;
+}
+
+int32_t hyd_batch_render(hyd_ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float density, float ert,
+                         float *rgba_dev, unsigned long long *samples_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(b, HYD_ERR_ARG, "null batch");
+  BatchArgs a;
+  int grid = 0;
+  if ((st = fill_args(c, b, cfg, density, ert, rgba_dev, samples_dev, a, grid))) return st;
+  HYD_CHECK_HIP(launch_batch(a, c->pool.fmt, cfg->tile_w, cfg->unroll, grid, c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_batch_capture(hyd_ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float density, float ert,
+                          float *rgba_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(b, HYD_ERR_ARG, "null batch");
+  BatchArgs a;
+  int grid = 0;
+  if ((st = fill_args(c, b, cfg, density, ert, rgba_dev, nullptr, a, grid))) return st;
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  if (!b->capture_stream) HYD_CHECK_HIP(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
+  HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
+  HYD_CHECK_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = launch_batch(a, c->pool.fmt, cfg->tile_w, cfg->unroll, grid, b->capture_stream);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(b->capture_stream, &g);
+  HYD_CHECK_HIP(e);
+  HYD_CHECK_HIP(e2);
+  b->graph = g;
+  HYD_CHECK_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return HYD_OK;
+}
+
+int32_t hyd_batch_replay(hyd_ctx *c, hyd_batch *b) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(b && b->exec, HYD_ERR_STATE, "no captured graph: call hyd_batch_capture first");
+  HYD_CHECK_HIP(hipGraphLaunch(b->exec, c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_batch_autotune(hyd_ctx *c, hyd_batch *b, float density, float ert, float *rgba_dev, int32_t reps,
+                           hyd_batch_config *best, float *best_ms) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(b && best && reps >= 1, HYD_ERR_ARG, "invalid autotune arguments");
+  hipEvent_t e0, e1;
+  HYD_CHECK_HIP(hipEventCreate(&e0));
+  HYD_CHECK_HIP(hipEventCreate(&e1));
+  float best_t = 1e30f;
+  hyd_batch_config best_c{8, 1, 8};
+  const int tws[3] = {8, 16, 32};
+  const int unrolls[3] = {1, 2, 4};
+  const int wpcs[3] = {8, 16, 32};
+  for (int tw : tws)
+    for (int un : unrolls)
+      for (int wpc : wpcs) {
+        hyd_batch_config cfg{tw, un, wpc};
+        BatchArgs a;
+        int grid = 0;
+        if ((st = fill_args(c, b, &cfg, density, ert, rgba_dev, nullptr, a, grid))) goto done;
+        // one untimed launch (first-use costs), then reps timed launches
+        if (launch_batch(a, c->pool.fmt, tw, un, grid, c->stream) != hipSuccess) { st = HYD_ERR_HIP; goto done; }
+        (void)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps; ++r) (void)launch_batch(a, c->pool.fmt, tw, un, grid, c->stream);
+        (void)hipEventRecord(e1, c->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) { st = HYD_ERR_HIP; set_error("autotune launch failed"); goto done; }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)reps;
+        if (ms < best_t) { best_t = ms; best_c = cfg; }
+      }
+done:
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  *best = best_c;
+  if (best_ms) *best_ms = best_t;
+  return HYD_OK;
+}
+
+}  // extern "C"

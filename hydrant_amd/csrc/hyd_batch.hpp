@@ -1,0 +1,50 @@
+// Internal declarations of the batched persistent renderer.
+#pragma once
+
+#include "hyd_internal.hpp"
+
+namespace hyd {
+
+constexpr int kXcds = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+
+struct BatchViewDev {
+  float trans[16];
+  float ctg;
+  float ray_o[3];
+  int32_t W, H;
+  int32_t volume;
+  int32_t pad_;
+  int64_t out_offset;
+};
+
+struct BatchVolDev {
+  int32_t grid[3];
+  int32_t cell_off;
+  int32_t bs, apron;
+  float bmax[3];
+  float step;
+  float value_scale;
+};
+
+struct BatchArgs {
+  const uint32_t *tiles;       // view << 16 | ty << 8 | tx
+  int32_t chunk_start[kXcds + 1];
+  unsigned *counters;          // kXcds work counters, zeroed by reset_counters
+  const BatchViewDev *views;
+  const BatchVolDev *vols;
+  const int32_t *cells;        // concatenated cell codes of every volume
+  const void *pool;
+  int32_t pad;
+  uint32_t page_elems;         // pad^3
+  const float *tf;
+  int32_t tf_n;
+  float density;
+  float ert;
+  int32_t max_steps;
+  float4 *out;
+  unsigned long long *samples;
+};
+
+hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int grid, hipStream_t s);
+
+}  // namespace hyd

@@ -79,6 +79,12 @@ struct Ctx {
   float *own_tf = nullptr;
   const float *u8_lut = nullptr;  // 256-entry float(v)/255 table
   float *own_lut = nullptr;
+  // multi-volume set of the batched renderer
+  void *own_vols = nullptr;       // BatchVolDev[n_vols]
+  int32_t *own_cells = nullptr;   // concatenated cell codes
+  int32_t n_vols = 0;
+  int32_t vol_pad = 0;            // padded brick edge shared by every volume
+  int32_t n_cus = 0;
 };
 
 // Flattened kernel arguments for the volume march (passed by value).
@@ -166,4 +172,9 @@ struct IsoArgs {
 };
 hipError_t launch_iso(const IsoArgs &a, int fmt, hipStream_t s);
 
+int32_t activate(Ctx *c);
+
 }  // namespace hyd
+
+// The opaque handle of the C ABI.
+struct hyd_ctx : hyd::Ctx {};

@@ -1,0 +1,319 @@
+// Batched persistent volume renderer for gfx950.
+//
+// Many views of several volumes (uneven resolutions) are rendered by ONE
+// launch: a grid of persistent single-wave workgroups drains a queue of
+// super-tiles (TW x 8 pixels of one view), each marched as 8x8 ray packets.
+// The queue is split into 8 contiguous chunks, one per XCD (workgroups are
+// dispatched round-robin over the XCDs, so workgroup i runs on XCD i % 8);
+// a wave drains its own chunk first, so neighbouring tiles of a view - which
+// touch the same bricks - share that XCD's L2, and then steals from the others.
+//
+// Schedule: the lattice schedule (sample k at the midpoint of its coverage,
+// sample weighted by its coverage, empty regions skipped, early termination at
+// ert).  The march is a brick-level DDA: the cell code (page / not resident /
+// empty distance) is looked up once per brick, the number of lattice samples
+// inside the brick comes from the ray's exit distance, and the samples inside
+// a brick step a precomputed local frame q(k) = qo + qd*(k + 0.5) in padded
+// voxel units, i.e. fx = (p - ip)*bs + (apron - 0.5) - the reference mapping
+// pad*(k*x + b) - 0.5 with k = bs/pad, b = apron/pad, whose Wrap mod is the
+// identity for an apron >= 1.  A sample that lands on a brick face may be read
+// from either neighbour: the apron holds the neighbour's voxels, so the value
+// is the same up to rounding.  UNROLL samples are resolved per inner
+// iteration so their tap loads are in flight together; compositing stays
+// strictly in order.  FP16 / u8 / FP32 bricks, FP32 accumulation.
+#include <hip/hip_fp16.h>
+
+#include "hyd_batch.hpp"
+
+namespace hyd {
+namespace {
+
+constexpr int kMaxTf = 256;
+
+// Raw stored values; the interpolated value is multiplied once by the
+// volume's value_scale (1/255 for bricks that hold 8-bit counts).
+template <int FMT> struct BVox;
+template <> struct BVox<HYD_FMT_F32> {
+  static __device__ __forceinline__ float at(const void *p, uint32_t i) { return static_cast<const float *>(p)[i]; }
+};
+template <> struct BVox<HYD_FMT_F16> {
+  static __device__ __forceinline__ float at(const void *p, uint32_t i) {
+    return __half2float(static_cast<const __half *>(p)[i]);
+  }
+};
+template <> struct BVox<HYD_FMT_U8> {
+  static __device__ __forceinline__ float at(const void *p, uint32_t i) {
+    return float(static_cast<const uint8_t *>(p)[i]);
+  }
+};
+
+struct RayState {
+  float ox, oy, oz, dx, dy, dz;
+  float ix, iy, iz;   // 1 / d
+  float lf;
+  int k, kend;
+  float cdu;
+  float4 v;
+};
+
+// One resolved sample: tap base index and trilinear weights.
+struct Tap {
+  uint32_t off;
+  float ax, ay, az;
+};
+
+__device__ __forceinline__ Tap local_tap(float qx, float qy, float qz, float hi, uint32_t pad, uint32_t base) {
+  const float fx = __builtin_amdgcn_fmed3f(qx, 0.f, hi);
+  const float fy = __builtin_amdgcn_fmed3f(qy, 0.f, hi);
+  const float fz = __builtin_amdgcn_fmed3f(qz, 0.f, hi);
+  const float flx = floorf(fx), fly = floorf(fy), flz = floorf(fz);
+  Tap t;
+  t.ax = fx - flx;
+  t.ay = fy - fly;
+  t.az = fz - flz;
+  t.off = base + ((uint32_t)flz * pad + (uint32_t)fly) * pad + (uint32_t)flx;
+  return t;
+}
+
+struct Taps8 { float v[8]; };
+
+template <int FMT>
+__device__ __forceinline__ Taps8 fetch(const void *pool, uint32_t off, uint32_t p1, uint32_t p2) {
+  Taps8 r;
+  r.v[0] = BVox<FMT>::at(pool, off);
+  r.v[1] = BVox<FMT>::at(pool, off + 1);
+  r.v[2] = BVox<FMT>::at(pool, off + p1);
+  r.v[3] = BVox<FMT>::at(pool, off + p1 + 1);
+  r.v[4] = BVox<FMT>::at(pool, off + p2);
+  r.v[5] = BVox<FMT>::at(pool, off + p2 + 1);
+  r.v[6] = BVox<FMT>::at(pool, off + p2 + p1);
+  r.v[7] = BVox<FMT>::at(pool, off + p2 + p1 + 1);
+  return r;
+}
+
+__device__ __forceinline__ float lerp8(const Taps8 &v, const Tap &t) {
+  const float x00 = fmaf(t.ax, v.v[1] - v.v[0], v.v[0]);
+  const float x10 = fmaf(t.ax, v.v[3] - v.v[2], v.v[2]);
+  const float x01 = fmaf(t.ax, v.v[5] - v.v[4], v.v[4]);
+  const float x11 = fmaf(t.ax, v.v[7] - v.v[6], v.v[6]);
+  const float y0 = fmaf(t.ay, x10 - x00, x00);
+  const float y1 = fmaf(t.ay, x11 - x01, x01);
+  return fmaf(t.az, y1 - y0, y0);
+}
+
+// TF lookup on the edge-extended table ext[0] = tf[0], ext[1..n] = tf[0..n-1],
+// ext[n+1] = tf[n-1]: fx = n*s - 0.5 in [-0.5, n - 0.5] for s in [0, 1], so
+// floor(fx) + 1 indexes ext without clamping and reproduces the reference's
+// clamp-to-edge; s outside [0, 1] is the Border zero.
+__device__ __forceinline__ void composite(const float4 *ext, float fn, float density, float s, float w,
+                                          float4 &v) {
+  const float fx = fmaf(fn, s, -0.5f);
+  const bool inside = (0.f <= s) & (s <= 1.f);
+  const float flr = floorf(fx);
+  const float t = fx - flr;
+  const int i = inside ? (int)flr + 1 : 0;
+  const float4 p = ext[i], q = ext[i + 1];
+  const float a = inside ? fmaf(t, q.w - p.w, p.w) : 0.f;
+  const float alpha = a * density * w;
+  const float tr = (1.f - v.w) * alpha;
+  v.x = fmaf(fmaf(t, q.x - p.x, p.x), tr, v.x);
+  v.y = fmaf(fmaf(t, q.y - p.y, p.y), tr, v.y);
+  v.z = fmaf(fmaf(t, q.z - p.z, p.z), tr, v.z);
+  v.w = fmaf(1.f - v.w, alpha, v.w);
+}
+
+// Cell code: >= 0 page id, -1 not resident, <= -2 empty with chebyshev -1 - code.
+__device__ __forceinline__ int cell_code(const BatchVolDev &vol, const int32_t *cells, float ipx, float ipy,
+                                         float ipz) {
+  const int ix = min(max((int)ipx, 0), vol.grid[0] - 1);
+  const int iy = min(max((int)ipy, 0), vol.grid[1] - 1);
+  const int iz = min(max((int)ipz, 0), vol.grid[2] - 1);
+  return cells[vol.cell_off + (iz * vol.grid[1] + iy) * vol.grid[0] + ix];
+}
+
+// Ray parameter (from o0) at which the ray leaves the unit box [ip, ip + 1].
+// The exit face follows the sign of 1/d, so d = +0 / -0 give +inf slabs.
+__device__ __forceinline__ float box_exit(const RayState &r, float ipx, float ipy, float ipz) {
+  const float tx = ((r.ix > 0.f ? ipx + 1.f : ipx) - r.ox) * r.ix;
+  const float ty = ((r.iy > 0.f ? ipy + 1.f : ipy) - r.oy) * r.iy;
+  const float tz = ((r.iz > 0.f ? ipz + 1.f : ipz) - r.oz) * r.iz;
+  return fminf(fminf(tx, ty), tz);
+}
+
+template <int FMT, int UNROLL, bool COUNT>
+__device__ void march(const BatchArgs &a, const BatchVolDev &vol, const float4 *ext, RayState &r,
+                      unsigned &count) {
+  const float step = vol.step, inv_step = 1.f / vol.step;
+  const float fn = float(a.tf_n);
+  const uint32_t pad = (uint32_t)a.pad, p2 = pad * pad;
+  const float hi = float(a.pad) - 1.0001f;
+  const float fbs = float(vol.bs), off = float(vol.apron) - 0.5f;
+  while (r.k < r.kend) {
+    const float t0 = fmaf(0.5f, fminf(1.f, r.lf - float(r.k)), float(r.k)) * step;
+    const float ipx = floorf(fmaf(r.dx, t0, r.ox));
+    const float ipy = floorf(fmaf(r.dy, t0, r.oy));
+    const float ipz = floorf(fmaf(r.dz, t0, r.oz));
+    const int code = cell_code(vol, a.cells, ipx, ipy, ipz);
+    const float t_exit = box_exit(r, ipx, ipy, ipz);
+    if (code <= -2) {   // empty: jump to the first lattice point past the empty region
+      const int kn = (int)ceilf(fmaf(t_exit + float(-1 - code - 1) * r.cdu, inv_step, -0.5f));
+      r.k = kn > r.k + 1 ? kn : r.k + 1;
+      continue;
+    }
+    if (code == -1) { r.k = r.kend; break; }   // page fault: the ray stops (reference semantics)
+    // lattice samples with t < t_exit belong to this brick
+    int kb = (int)ceilf(fmaf(t_exit, inv_step, -0.5f));
+    kb = kb > r.k + 1 ? kb : r.k + 1;
+    kb = kb < r.kend ? kb : r.kend;
+    const uint32_t base = (uint32_t)code * a.page_elems;
+    // local frame in padded-voxel units: q(k) = qo + qd * (k + 0.5)
+    const float qox = fmaf(r.ox - ipx, fbs, off), qoy = fmaf(r.oy - ipy, fbs, off), qoz = fmaf(r.oz - ipz, fbs, off);
+    const float qdx = r.dx * fbs * step, qdy = r.dy * fbs * step, qdz = r.dz * fbs * step;
+    int k = r.k;
+    bool done = false;
+    while (k < kb) {
+      if (UNROLL > 1 && k + UNROLL <= kb) {
+        Tap tp[UNROLL];
+        Taps8 tv[UNROLL];
+        float wv[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+          wv[u] = fminf(1.f, r.lf - float(k + u));
+          const float kf = fmaf(0.5f, wv[u], float(k + u));
+          tp[u] = local_tap(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), hi, pad, base);
+          tv[u] = fetch<FMT>(a.pool, tp[u].off, pad, p2);
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+          const float s = lerp8(tv[u], tp[u]) * vol.value_scale;
+          composite(ext, fn, a.density, s, wv[u], r.v);
+          if (COUNT) ++count;
+          ++k;
+          if (a.ert > 0.f && r.v.w > a.ert) { done = true; break; }
+        }
+        if (done) break;
+      } else {
+        const float w = fminf(1.f, r.lf - float(k));
+        const float kf = fmaf(0.5f, w, float(k));
+        const Tap tp = local_tap(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), hi, pad, base);
+        const Taps8 tv = fetch<FMT>(a.pool, tp.off, pad, p2);
+        const float s = lerp8(tv, tp) * vol.value_scale;
+        composite(ext, fn, a.density, s, w, r.v);
+        if (COUNT) ++count;
+        ++k;
+        if (a.ert > 0.f && r.v.w > a.ert) { done = true; break; }
+      }
+    }
+    r.k = done ? r.kend : k;
+  }
+}
+
+__device__ __forceinline__ unsigned claim(unsigned *counter) {
+  unsigned t = 0;
+  if (__lane_id() == 0) t = atomicAdd(counter, 1u);
+  return __shfl(t, 0);
+}
+
+template <int FMT, int TW, int UNROLL, bool COUNT>
+__global__ __launch_bounds__(64) void batch_kernel(BatchArgs a) {
+  __shared__ float4 s_ext[kMaxTf + 2];
+  __shared__ unsigned long long s_count;
+  const float4 *tf4 = reinterpret_cast<const float4 *>(a.tf);
+  for (int i = threadIdx.x; i < a.tf_n + 2; i += 64) {
+    const int j = i == 0 ? 0 : (i > a.tf_n ? a.tf_n - 1 : i - 1);
+    s_ext[i] = tf4[j];
+  }
+  if (COUNT && threadIdx.x == 0) s_count = 0;
+  __syncthreads();
+  unsigned count = 0;
+  const int home = blockIdx.x % kXcds;
+  for (int c = 0; c < kXcds; ++c) {
+    const int chunk = (home + c) % kXcds;
+    const unsigned base = (unsigned)a.chunk_start[chunk];
+    const unsigned len = (unsigned)(a.chunk_start[chunk + 1] - a.chunk_start[chunk]);
+    while (true) {
+      const unsigned t = claim(a.counters + chunk);
+      if (t >= len) break;
+      const uint32_t desc = a.tiles[base + t];
+      const int view = (int)(desc >> 16);
+      const int tx = (int)(desc & 0xffu), ty = (int)((desc >> 8) & 0xffu);
+      const BatchViewDev &vw = a.views[view];
+      const BatchVolDev &vol = a.vols[vw.volume];
+#pragma unroll 1
+      for (int sub = 0; sub < TW / 8; ++sub) {
+        const int x = tx * TW + sub * 8 + (int)(threadIdx.x & 7);
+        const int y = ty * 8 + (int)(threadIdx.x >> 3);
+        if (x >= vw.W || y >= vw.H) continue;
+        const f3 ro = f3{vw.ray_o[0], vw.ray_o[1], vw.ray_o[2]};
+        const f3 d = pixel_ray_dir(vw.trans, vw.ctg, ro, x, y, vw.W, vw.H);
+        RayState r;
+        r.v = make_float4(0.f, 0.f, 0.f, 0.f);
+        float tnear, tfar;
+        if (intersect_box(ro, d, f3{0.f, 0.f, 0.f}, f3{vol.bmax[0], vol.bmax[1], vol.bmax[2]}, tnear, tfar)) {
+          if (tnear < 0) tnear = 0;
+          if (tfar < 0) tfar = 0;
+          r.ox = ro.x + d.x * tnear; r.oy = ro.y + d.y * tnear; r.oz = ro.z + d.z * tnear;
+          r.dx = d.x; r.dy = d.y; r.dz = d.z;
+          r.ix = 1.f / d.x; r.iy = 1.f / d.y; r.iz = 1.f / d.z;
+          r.lf = fminf((tfar - tnear) / vol.step, float(a.max_steps));
+          r.k = 0;
+          r.kend = (int)ceilf(r.lf);
+          r.cdu = 1.f / comp_max(abs3(d));
+          march<FMT, UNROLL, COUNT>(a, vol, s_ext, r, count);
+        }
+        a.out[vw.out_offset + (int64_t)y * vw.W + x] = r.v;
+      }
+    }
+  }
+  if (COUNT) {
+    if (count) atomicAdd(&s_count, (unsigned long long)count);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_count) atomicAdd(a.samples, s_count);
+  }
+}
+
+__global__ void reset_counters(unsigned *c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
+}
+
+template <int FMT, int TW, int UNROLL>
+hipError_t launch3(const BatchArgs &a, int grid, hipStream_t s) {
+  if (a.samples)
+    hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, true>), dim3(grid), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, false>), dim3(grid), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int FMT, int TW>
+hipError_t launch2(const BatchArgs &a, int unroll, int grid, hipStream_t s) {
+  switch (unroll) {
+    case 4: return launch3<FMT, TW, 4>(a, grid, s);
+    case 2: return launch3<FMT, TW, 2>(a, grid, s);
+    default: return launch3<FMT, TW, 1>(a, grid, s);
+  }
+}
+
+template <int FMT>
+hipError_t launch1(const BatchArgs &a, int tw, int unroll, int grid, hipStream_t s) {
+  switch (tw) {
+    case 32: return launch2<FMT, 32>(a, unroll, grid, s);
+    case 16: return launch2<FMT, 16>(a, unroll, grid, s);
+    default: return launch2<FMT, 8>(a, unroll, grid, s);
+  }
+}
+
+}  // namespace
+
+hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(reset_counters, dim3(1), dim3(kXcds), 0, s, a.counters, kXcds);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  switch (fmt) {
+    case HYD_FMT_U8: return launch1<HYD_FMT_U8>(a, tile_w, unroll, grid, s);
+    case HYD_FMT_F16: return launch1<HYD_FMT_F16>(a, tile_w, unroll, grid, s);
+    default: return launch1<HYD_FMT_F32>(a, tile_w, unroll, grid, s);
+  }
+}
+
+}  // namespace hyd

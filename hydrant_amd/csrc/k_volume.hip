@@ -120,10 +120,12 @@ struct MarchState {
 
 // Shade one resolved sample and composite it front to back
 // (volume_shader.cu:52-70, plus the density scale on the TF alpha).
-__device__ __forceinline__ void shade_composite(const VolumeArgs &a, const float *tf, float s, int pgid,
-                                                MarchState &st) {
+template <bool WEIGHTED = false>
+__device__ __forceinline__ void shade_composite_impl(const VolumeArgs &a, const float *tf, float s, int pgid,
+                                                     MarchState &st, float w) {
   float4 ub = tf_sample(tf, a.tf_n, s);
   ub.w = ub.w * a.density;
+  if (WEIGHTED) ub.w = ub.w * w;
   if (a.mode == HYD_MODE_PARTITION) {
     float l = length3(f3{ub.x, ub.y, ub.z});
     float r = a.rank, q = 1.f - r;
@@ -142,6 +144,15 @@ __device__ __forceinline__ void shade_composite(const VolumeArgs &a, const float
   st.v.y = st.v.y + ub.y * t;
   st.v.z = st.v.z + ub.z * t;
   st.v.w = st.v.w + ub.w * t;
+}
+
+__device__ __forceinline__ void shade_composite(const VolumeArgs &a, const float *tf, float s, int pgid,
+                                                MarchState &st) {
+  shade_composite_impl<false>(a, tf, s, pgid, st, 1.f);
+}
+__device__ __forceinline__ void shade_composite(const VolumeArgs &a, const float *tf, float s, int pgid,
+                                                MarchState &st, float w) {
+  shade_composite_impl<true>(a, tf, s, pgid, st, w);
 }
 
 // VolumeShaderKernel::main (volume_shader.cu:32-81), reference step schedule.
@@ -194,24 +205,31 @@ __device__ void march_reference(const VolumeArgs &a, const float *tf, const floa
   st.step_size = step_size;
 }
 
-// Lattice schedule: sample k at o0 + d * (k * step); o0 stays in st.o, the
-// next lattice index in st.step_size and the remaining budget in st.nsteps.
+// Lattice schedule: sample k covers w = min(1, L - k) steps (L the ray extent
+// in steps, k < ceil(L)), sits at the covered segment's midpoint
+// o0 + d * ((k + w/2) * step) and its opacity is scaled by w; empty regions are
+// skipped to the next lattice point; early
+// termination once v.w > ert.  Film: o = o0, nsteps slot = bits of L (float),
+// step_size = next lattice index.
 template <int FMT, bool COUNT>
 __device__ void march_lattice(const VolumeArgs &a, const float *tf, const float *lut, MarchState &st,
                               unsigned &count) {
   const f3 o0 = st.o, d = st.d;
+  const float lf = __int_as_float(st.nsteps);
   int k = st.step_size;
-  const int kend = k + st.nsteps;
+  const int kend = (int)ceilf(lf);
+  const float cdu = 1.f / comp_max(abs3(d));
   while (k < kend) {
-    const f3 o = add(o0, muls(d, float(k) * a.step));
+    const float w = gmin(1.f, lf - float(k));       // coverage of sample k
+    const float t = (float(k) + 0.5f * w) * a.step;  // midpoint of the covered segment
+    const f3 o = add(o0, muls(d, t));
     const f3 ip = floor3(o);
     const int cd = a.cheb[grid_index(ip, a.cdim)];
     if (cd) {
       float tn, tfar;
       intersect_box(o, d, ip, f3{ip.x + 1.f, ip.y + 1.f, ip.z + 1.f}, tn, tfar);
-      const float cdu = 1.f / comp_max(abs3(d));
-      const float t_exit = float(k) * a.step + tfar + float(cd - 1) * cdu;
-      const int kn = (int)ceilf(t_exit / a.step);
+      const float t_exit = t + tfar + float(cd - 1) * cdu;
+      const int kn = (int)ceilf(t_exit / a.step - 0.5f);
       k = kn > k + 1 ? kn : k + 1;
       continue;
     }
@@ -219,12 +237,11 @@ __device__ void march_lattice(const VolumeArgs &a, const float *tf, const float 
     if (pgid == -1) break;
     const float4 pg = a.page[pgid];
     const float s = brick_sample<FMT>(a.pool, a.pad, pgid, pg, sub(o, ip), lut, nullptr);
-    shade_composite(a, tf, s, pgid, st);
+    shade_composite(a, tf, s, pgid, st, w);
     if (COUNT) ++count;
     ++k;
     if (a.ert > 0.f && st.v.w > a.ert) { k = kend; break; }
   }
-  st.nsteps = kend - k > 0 ? kend - k : 0;
   st.step_size = k;
 }
 
@@ -240,12 +257,17 @@ __device__ __forceinline__ bool emit(const VolumeArgs &a, int x, int y, MarchSta
     if (tnear < 0) tnear = 0;
     if (tfar < 0) tfar = 0;
     st.o = add(st.o, muls(st.d, tnear));
-    const int n = (int)((tfar - tnear) / a.step);
-    st.nsteps = a.max_steps < n ? a.max_steps : n;
-    st.step_size = a.schedule == HYD_SCHED_LATTICE ? 0 : 1;
+    if (a.schedule == HYD_SCHED_LATTICE) {
+      st.nsteps = __float_as_int(gmin((tfar - tnear) / a.step, float(a.max_steps)));
+      st.step_size = 0;
+    } else {
+      const int n = (int)((tfar - tnear) / a.step);
+      st.nsteps = a.max_steps < n ? a.max_steps : n;
+      st.step_size = 1;
+    }
     return true;
   }
-  st.nsteps = 0;
+  st.nsteps = 0;   // also the bits of 0.f for the lattice schedule
   st.step_size = a.schedule == HYD_SCHED_LATTICE ? 0 : 1;
   return false;
 }

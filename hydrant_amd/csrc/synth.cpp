@@ -241,7 +241,8 @@ int32_t hyd_gen_sdf_spheres_f16(int32_t n, int32_t n_spheres, float r_lo, float 
 }
 
 int32_t hyd_pack_bricks(const void *vol, int32_t fmt_in, const int32_t dims[3], int32_t bs, int32_t apron,
-                        int32_t fmt_out, const int32_t *bricks, int32_t n_bricks, void *out) {
+                        int32_t fmt_out, int32_t normalize, const int32_t *bricks, int32_t n_bricks, void *out) {
+  const bool raw_counts = fmt_in == HYD_FMT_U8 && !normalize;
   HYD_REQUIRE(vol && dims && out && bs > 0 && apron >= 0, HYD_ERR_ARG, "hyd_pack_bricks: invalid arguments");
   HYD_REQUIRE(fmt_in >= HYD_FMT_U8 && fmt_in <= HYD_FMT_F32 && fmt_out >= HYD_FMT_U8 && fmt_out <= HYD_FMT_F32,
               HYD_ERR_ARG, "hyd_pack_bricks: unknown format");
@@ -265,13 +266,15 @@ int32_t hyd_pack_bricks(const void *vol, int32_t fmt_in, const int32_t dims[3], 
           const int x = std::min(std::max(bx * bs + i - apron, 0), dims[0] - 1);
           const size_t src = ((size_t)z * dims[1] + y) * dims[0] + x;
           const size_t dst = (size_t)p * pv + ((size_t)k * pad + j) * pad + i;
+          const float value = raw_counts ? float(static_cast<const uint8_t *>(vol)[src]) : 0.f;
           if (fmt_out == HYD_FMT_U8) {
             static_cast<uint8_t *>(out)[dst] = static_cast<const uint8_t *>(vol)[src];
           } else if (fmt_out == HYD_FMT_F16) {
             static_cast<uint16_t *>(out)[dst] =
-                fmt_in == HYD_FMT_F16 ? static_cast<const uint16_t *>(vol)[src] : hyd_f32_to_f16(read_norm(vol, fmt_in, src));
+                fmt_in == HYD_FMT_F16 ? static_cast<const uint16_t *>(vol)[src]
+                                      : hyd_f32_to_f16(raw_counts ? value : read_norm(vol, fmt_in, src));
           } else {
-            static_cast<float *>(out)[dst] = read_norm(vol, fmt_in, src);
+            static_cast<float *>(out)[dst] = raw_counts ? value : read_norm(vol, fmt_in, src);
           }
         }
       }

@@ -85,6 +85,7 @@ class Bricked:
     lowest_blkcnt: int = 0
     bmin: np.ndarray = field(default=None)
     bmax: np.ndarray = field(default=None)
+    raw_counts: bool = False   # f16/f32 pool holds 8-bit counts (renderer scale 1/255)
 
     @property
     def pad(self) -> int:
@@ -94,8 +95,16 @@ class Bricked:
     def n_pages(self) -> int:
         return int(self.pool.shape[0])
 
+    @property
+    def value_scale(self) -> float:
+        """Multiplier of interpolated pool values in the batched renderer."""
+        return 1.0 / 255.0 if (self.raw_counts or self.fmt == L.FMT_U8) else 1.0
+
     def pool_values(self) -> np.ndarray:
         """Normalised float32 voxel values of the pool (what the samplers read)."""
+        if self.raw_counts:
+            return self.pool.view(np.float16).astype(np.float32) / np.float32(255) if self.fmt == L.FMT_F16 \
+                else self.pool.astype(np.float32) / np.float32(255)
         return normalised(self.pool)
 
 
@@ -134,7 +143,8 @@ def chebyshev(empty: np.ndarray) -> np.ndarray:
     return out
 
 
-def pack(vol: np.ndarray, bs: int, apron: int, fmt_out: int, bricks: np.ndarray | None = None) -> np.ndarray:
+def pack(vol: np.ndarray, bs: int, apron: int, fmt_out: int, bricks: np.ndarray | None = None,
+         normalize: bool = True) -> np.ndarray:
     vol = np.ascontiguousarray(vol)
     dims = np.array(vol.shape[::-1], np.int32)
     grid = [(int(d) + bs - 1) // bs for d in dims]
@@ -147,13 +157,13 @@ def pack(vol: np.ndarray, bs: int, apron: int, fmt_out: int, bricks: np.ndarray 
     if bricks is not None:
         bricks = np.ascontiguousarray(bricks, np.int32)
         bp = L.ptr(bricks, C.c_int32)
-    L.call("hyd_pack_bricks", vol.ctypes.data, _fmt_of(vol), L.ptr(dims, C.c_int32), bs, apron, fmt_out, bp,
-           count, out.ctypes.data)
+    L.call("hyd_pack_bricks", vol.ctypes.data, _fmt_of(vol), L.ptr(dims, C.c_int32), bs, apron, fmt_out,
+           int(bool(normalize)), bp, count, out.ctypes.data)
     return out
 
 
 def make_bricked(vol: np.ndarray, bs: int, apron: int, fmt_out: int, empty_fn=None, resident: int | float = 1.0,
-                 seed: int = 7) -> Bricked:
+                 seed: int = 7, normalize: bool = True) -> Bricked:
     """Brick ``vol`` (shape (z, y, x)); ``empty_fn(bmin, bmax) -> bool array`` marks empty bricks.
 
     ``resident``: number (int) or fraction (float) of bricks made resident by the
@@ -172,13 +182,14 @@ def make_bricked(vol: np.ndarray, bs: int, apron: int, fmt_out: int, empty_fn=No
     L.call("hyd_seeded_page_table", L.ptr(dims, C.c_int32), L.ptr(empty_c, C.c_uint8), n_res, seed,
            L.ptr(vaddr, C.c_int32), L.ptr(order, C.c_int32), C.byref(nn))
     n_pages = min(n_res, nn.value)
-    pool = pack(vol, bs, apron, fmt_out, order[:n_pages])
+    pool = pack(vol, bs, apron, fmt_out, order[:n_pages], normalize=normalize)
     k, b = mapping(bs, apron)
     page_k = np.full(max(n_pages, 1), k, np.float32)
     page_b = np.full((max(n_pages, 1), 3), b, np.float32)
+    raw = (not normalize) and vol.dtype == np.uint8 and fmt_out != L.FMT_U8
     return Bricked(dims=tuple(int(d) for d in vol.shape[::-1]), bs=bs, apron=apron, grid=tuple(grid), fmt=fmt_out,
                    pool=pool, vaddr=vaddr, page_k=page_k, page_b=page_b, cheb=cheb, mean=bmean, empty=empty,
-                   order=order[:nn.value].copy(), n_nonempty=nn.value, bmin=bmin, bmax=bmax)
+                   order=order[:nn.value].copy(), n_nonempty=nn.value, bmin=bmin, bmax=bmax, raw_counts=raw)
 
 
 def single_brick(vol: np.ndarray, fmt_out: int = L.FMT_F32) -> Bricked:

@@ -211,3 +211,96 @@ class Context:
 
     def synchronize(self):
         L.call("hyd_synchronize", self.h)
+
+    # -------------------------------------------------------- batched ----
+    def set_volumes(self, descs: list[dict]):
+        """descs: dicts with grid (x, y, z), bs, apron, bbox_max, step, vaddr, cheb (numpy grids)."""
+        arr = (L.VolumeDesc * len(descs))()
+        keep = []
+        for i, d in enumerate(descs):
+            va = np.ascontiguousarray(d["vaddr"], np.int32)
+            ch = np.ascontiguousarray(d["cheb"], np.int32)
+            keep += [va, ch]
+            arr[i].grid = L.i3(*[int(x) for x in d["grid"]])
+            arr[i].bs, arr[i].apron = int(d["bs"]), int(d["apron"])
+            arr[i].bbox_max = _f3(d["bbox_max"])
+            arr[i].step = float(d["step"])
+            arr[i].value_scale = float(d.get("value_scale", 1.0))
+            arr[i].vaddr = L.ptr(va, C.c_int32)
+            arr[i].chebyshev = L.ptr(ch, C.c_int32)
+        L.call("hyd_set_volumes", self.h, len(descs), arr)
+
+    def batch(self, views: list[tuple]) -> "Batch":
+        """views: (hyd View, width, height, volume index) tuples, packed back to back in the output."""
+        return Batch(self, views)
+
+
+class Batch:
+    """A batched-render plan: many views, one persistent launch (optionally a HIP graph)."""
+
+    def __init__(self, ctx: Context, views: list[tuple]):
+        self.ctx = ctx
+        arr = (L.BatchView * len(views))()
+        off = 0
+        self.offsets = []
+        for i, (v, w, h, vol) in enumerate(views):
+            arr[i].view = v
+            arr[i].width, arr[i].height, arr[i].volume = int(w), int(h), int(vol)
+            arr[i].out_offset = off
+            self.offsets.append((off, int(w), int(h)))
+            off += int(w) * int(h)
+        self.n_pixels = off
+        h = C.c_void_p()
+        L.call("hyd_batch_create", ctx.h, arr, len(views), C.byref(h))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            L.call("hyd_batch_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def new_output(self):
+        t = self.ctx.torch
+        return t.empty((self.n_pixels, 4), dtype=t.float32, device=self.ctx.dev)
+
+    def render(self, cfg: L.BatchConfig, density: float, ert: float, out=None, count: bool = False):
+        t = self.ctx.torch
+        self.ctx.sync_stream()
+        out = self.new_output() if out is None else out
+        cnt = t.zeros(1, dtype=t.int64, device=self.ctx.dev) if count else None
+        L.call("hyd_batch_render", self.ctx.h, self.h, C.byref(cfg), float(density), float(ert),
+               C.c_void_p(out.data_ptr()), C.c_void_p(cnt.data_ptr() if count else 0))
+        return out, cnt
+
+    def capture(self, cfg: L.BatchConfig, density: float, ert: float, out):
+        self.ctx.sync_stream()
+        L.call("hyd_batch_capture", self.ctx.h, self.h, C.byref(cfg), float(density), float(ert),
+               C.c_void_p(out.data_ptr()))
+
+    def replay(self):
+        self.ctx.sync_stream()
+        L.call("hyd_batch_replay", self.ctx.h, self.h)
+
+    def autotune(self, density: float, ert: float, out, reps: int = 2):
+        self.ctx.sync_stream()
+        best = L.BatchConfig()
+        ms = C.c_float(0)
+        L.call("hyd_batch_autotune", self.ctx.h, self.h, float(density), float(ert), C.c_void_p(out.data_ptr()),
+               int(reps), C.byref(best), C.byref(ms))
+        return best, ms.value
+
+    def view_image(self, out, i: int):
+        off, w, h = self.offsets[i]
+        return out[off:off + w * h].view(h, w, 4)
+
+
+def batch_config(tile_w=8, unroll=1, waves_per_cu=8) -> L.BatchConfig:
+    c = L.BatchConfig()
+    c.tile_w, c.unroll, c.waves_per_cu = tile_w, unroll, waves_per_cu
+    return c

@@ -148,6 +148,79 @@ def blocks(n: int = 256, W: int = 512, H: int = 512, bs: int = 32, density: floa
     return Scene("blocks", b, D.README_TF.copy(), W, H, README_ARM, 60.0, geom, density=density)
 
 
+@dataclass
+class BatchedWorkload:
+    """Many views of several volumes sharing one brick pool (the batched form)."""
+    sizes: tuple
+    fmt: int
+    volumes: list            # per volume: dict(vol, bricked, geom, page_base)
+    pool: np.ndarray         # all volumes' pages, page order
+    views: list              # per view: dict(volume, W, H, arm)
+    tf: np.ndarray
+    density: float
+    ert: float
+    bs: int
+    apron: int
+
+    def volume_descs(self):
+        out = []
+        for v in self.volumes:
+            b = v["bricked"]
+            vaddr = b.vaddr.copy()
+            vaddr[vaddr >= 0] += v["page_base"]
+            out.append({"grid": b.grid, "bs": self.bs, "apron": self.apron, "bbox_max": v["geom"].f_dim,
+                        "step": v["geom"].step, "value_scale": b.value_scale, "vaddr": vaddr, "cheb": b.cheb})
+        return out
+
+    def hip_views(self):
+        from .hip import make_view, orbit_camera
+        out = []
+        for v in self.views:
+            g = self.volumes[v["volume"]]["geom"]
+            out.append((make_view(orbit_camera(v["arm"]), g.center, g.f_dim), v["W"], v["H"], v["volume"]))
+        return out
+
+    def scene(self, i: int) -> Scene:
+        """View i as a single-view lattice Scene over its own volume (for the oracle)."""
+        v = self.views[i]
+        vol = self.volumes[v["volume"]]
+        sc = Scene(f"batched_view{i}", vol["bricked"], self.tf, v["W"], v["H"], v["arm"], 60.0, vol["geom"],
+                   density=self.density, schedule=L.SCHED_LATTICE, ert=self.ert,
+                   extra={"volume": vol["vol"], "map_bs": self.bs})
+        return sc
+
+
+def batched(sizes=(256, 512, 768), n_views: int = 96, res_choices=(256, 384, 512, 768, 1024), fmt: int = L.FMT_F16,
+            bs: int = 64, apron: int = 1, density: float = 1e-2, ert: float = 0.9995, sample_rate: float = 0.5,
+            seed: int = SEED_BATCH, normalize: bool = False) -> BatchedWorkload:
+    """Batched precision form: seeded Gaussian-blob volumes (sigma scaled by size/256) stored
+    as fmt bricks of bs^3 + apron, n_views views with resolutions drawn from res_choices and
+    seeded orbit angles (arm.x ~ U[0, 360), arm.y ~ U[-80, 80], distance 2.2).
+    normalize=False stores the 8-bit counts (exact in FP16) and renders with scale 1/255;
+    normalize=True stores float(v)/255 rounded to the brick format."""
+    tf = D.README_TF.copy()
+    vols = []
+    pools = []
+    base = 0
+    for i, n in enumerate(sizes):
+        vol = D.gen_blobs(n, 16, seed + i, sig=(8.0, 40.0), sigma_scale=n / 256.0, amp=(0.3, 1.0))
+        b = D.make_bricked(vol, bs, apron, fmt, D.volume_empty_fn(tf), resident=1.0, seed=seed + 10 + i,
+                           normalize=normalize)
+        geom = D.geometry((n, n, n), bs, sample_rate)
+        vols.append({"vol": vol, "bricked": b, "geom": geom, "page_base": base, "n": n})
+        pools.append(b.pool)
+        base += b.n_pages
+    pool = np.concatenate(pools, axis=0)
+    rng = np.random.default_rng(seed)
+    views = []
+    for _ in range(n_views):
+        vi = int(rng.integers(0, len(sizes)))
+        r = int(res_choices[int(rng.integers(0, len(res_choices)))])
+        arm = (float(rng.uniform(0.0, 360.0)), float(rng.uniform(-80.0, 80.0)), 2.2)
+        views.append({"volume": vi, "W": r, "H": r, "arm": arm})
+    return BatchedWorkload(tuple(sizes), fmt, vols, pool, views, tf, density, ert, bs, apron)
+
+
 def blocks_params(sc: Scene, mode: int = L.BLOCKS_VOLUME):
     p = L.BlocksParams()
     p.bbox_min = L.f3(0, 0, 0)

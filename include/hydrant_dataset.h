@@ -51,12 +51,14 @@ float hyd_f16_to_f32(uint16_t h);
  * after brick in grid order (x fastest), apron voxels taken from the
  * neighbours with clamp-to-edge at the volume border.
  *   fmt_in:  HYD_FMT_U8 (uint8) or HYD_FMT_F16 (uint16 half bits) or HYD_FMT_F32
- *   fmt_out: HYD_FMT_U8 (only from u8), HYD_FMT_F16 or HYD_FMT_F32; u8 input
- *            is normalised as float(v) / 255 first.
+ *   fmt_out: HYD_FMT_U8 (only from u8), HYD_FMT_F16 or HYD_FMT_F32.
+ * normalize: for u8 input and f16/f32 output, 1 stores float(v) / 255 (the
+ *            reference's NormalizedFloat read), 0 stores the count v itself,
+ *            which f16 holds exactly (renderers then scale by 1/255).
  * bricks: optional list of n_bricks linear brick indices to pack (page order);
  * NULL packs every brick of the grid. */
 int32_t hyd_pack_bricks(const void *vol, int32_t fmt_in, const int32_t dims[3], int32_t bs, int32_t apron,
-                        int32_t fmt_out, const int32_t *bricks, int32_t n_bricks, void *out);
+                        int32_t fmt_out, int32_t normalize, const int32_t *bricks, int32_t n_bricks, void *out);
 
 /* Per-brick min / max / mean of normalised values.  min/max cover the padded
  * brick (every voxel a sample in the brick can touch); mean covers the bs^3

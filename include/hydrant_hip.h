@@ -89,7 +89,9 @@ typedef struct {
 } hyd_volume_params;
 
 /* VolumePixel film record, 64 bytes (volume_shader.hpp:11-15): the resumable
- * per-ray state.  In LATTICE schedule step_size holds the next lattice index. */
+ * per-ray state.  In LATTICE schedule o is the clipped entry point, nsteps
+ * holds the bits of the float ray extent in steps and step_size the next
+ * lattice index. */
 typedef struct {
   float o[3];
   float d[3];
@@ -226,6 +228,65 @@ int32_t hyd_render_blocks(hyd_ctx *ctx, const hyd_blocks_params *p, const hyd_vi
 /* Isosurface (IsosurfaceShaderKernel::main, isosurface_shader.cu:50-125). */
 int32_t hyd_render_iso(hyd_ctx *ctx, const hyd_iso_params *p, const hyd_view *view, int32_t width,
                        int32_t height, hyd_iso_pixel *out_dev, unsigned long long *samples_dev);
+
+/* --------------------------------------------------- batched form ------ */
+/* The reference renders one view per offline_render call (volume.cc:95-134),
+ * re-launching per pass.  The batched form renders many views of several
+ * volumes in ONE persistent launch over the shared brick pool, with the
+ * LATTICE schedule (no step escalation; FP32 accumulation over u8/f16/f32
+ * bricks).  It has no counterpart in the reference. */
+
+/* One volume of the set.  Page ids in vaddr index the shared pool. */
+typedef struct {
+  int32_t grid[3];           /* brick grid (x, y, z) */
+  int32_t bs;                /* block size (brick edge without apron) */
+  int32_t apron;             /* apron voxels per side, >= 1 (bs + 2*apron = pool pad edge) */
+  float bbox_max[3];         /* raw dims / bs, block units (bbox_min = 0) */
+  float step;                /* lattice step, block units */
+  float value_scale;         /* multiplies the interpolated brick value: 1/255 for bricks that hold
+                                8-bit counts (u8, or f16 packed with normalize = 0), else 1 */
+  const int32_t *vaddr;      /* host: grid cells, page id or -1 */
+  const int32_t *chebyshev;  /* host: grid cells, 0 = non-empty */
+} hyd_volume_desc;
+
+typedef struct {
+  hyd_view view;
+  int32_t width;
+  int32_t height;
+  int32_t volume;            /* index into the volume set */
+  int32_t reserved;
+  int64_t out_offset;        /* first pixel of this view in the RGBA output */
+} hyd_batch_view;
+
+/* Launch shape (results do not depend on it). */
+typedef struct {
+  int32_t tile_w;            /* ray-packet super-tile: 8, 16 or 32 pixels wide, 8 high */
+  int32_t unroll;            /* samples resolved per inner iteration: 1, 2 or 4 */
+  int32_t waves_per_cu;      /* persistent single-wave workgroups per CU: 1..32 */
+} hyd_batch_config;
+
+typedef struct hyd_batch hyd_batch;
+
+/* Define the volume set (copied to the device; cells are re-encoded). */
+int32_t hyd_set_volumes(hyd_ctx *ctx, int32_t n, const hyd_volume_desc *vols);
+/* Build a plan for n_views views (<= 65535, <= 256 tiles per axis). */
+int32_t hyd_batch_create(hyd_ctx *ctx, const hyd_batch_view *views, int32_t n_views, hyd_batch **out);
+int32_t hyd_batch_destroy(hyd_batch *batch);
+/* Pixels the output buffer must hold (max over views of out_offset + w*h). */
+int32_t hyd_batch_output_pixels(const hyd_batch *batch, int64_t *n_out);
+/* One persistent launch (a counter-reset kernel + the march); RGBA float out.
+ * samples_dev, if not NULL, accumulates the composited-sample count. */
+int32_t hyd_batch_render(hyd_ctx *ctx, hyd_batch *batch, const hyd_batch_config *cfg, float density, float ert,
+                         float *rgba_dev, unsigned long long *samples_dev);
+/* Record that launch into a HIP graph owned by the batch (no memset nodes). */
+int32_t hyd_batch_capture(hyd_ctx *ctx, hyd_batch *batch, const hyd_batch_config *cfg, float density, float ert,
+                          float *rgba_dev);
+/* Replay the captured graph on the context's stream. */
+int32_t hyd_batch_replay(hyd_ctx *ctx, hyd_batch *batch);
+/* Time every launch shape (tile_w x unroll x waves_per_cu) with `reps`
+ * launches each and return the fastest; writes the output buffer. */
+int32_t hyd_batch_autotune(hyd_ctx *ctx, hyd_batch *batch, float density, float ert, float *rgba_dev, int32_t reps,
+                           hyd_batch_config *best, float *best_ms);
 
 #ifdef __cplusplus
 }

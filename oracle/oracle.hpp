@@ -321,6 +321,7 @@ struct Paging {
   const float *page_b;    // per page, 3 floats
   const float *pool;      // n_pages * pad^3 normalised voxel values
   int pad;                // padded brick edge
+  int map_bs = 0;         // > 0: lattice uses the exact mapping for block size map_bs
 };
 
 // ---------------------------------------------------------------- shaders --
@@ -349,6 +350,7 @@ template <class T> struct VolumePixel {
   V4<T> v;
   V3<T> theta;
   T phi;
+  T lf;      // lattice schedule: ray extent in steps
 };
 
 struct TraceRec { int ix, iy, iz, pgid; Taps t; };
@@ -432,43 +434,59 @@ void volume_main(const VolumeShader<T> &sh, VolumePixel<T> &px_io, long long *ns
   if (nsamples) *nsamples += ns;
 }
 
-// Lattice schedule (build-defined, for brick-order-independent sort-last
-// compositing and for the batched form): sample k sits at o0 + d * (k * step)
-// with k counted from the clipped entry point, empty bricks are skipped to the
-// first lattice point past their exit, no step escalation, optional early ray
-// termination once v.w > ert.  px.nsteps holds the remaining budget and
-// px.step_size the next lattice index (k).
+// Lattice schedule (build-defined; used by brick-order-independent sort-last
+// compositing and by the batched form).  With o0 the clipped entry point and
+// L = (tfar - tnear) / step the ray extent in steps:
+//   sample k covers [k, min(k + 1, L)] steps: coverage w_k = min(1, L - k),
+//   it sits at the covered segment's midpoint o0 + d * ((k + w_k / 2) * step)
+//   (inside the box even for the last, partial segment), k = 0 .. ceil(L) - 1,
+//   and its opacity is scaled by w_k,
+//   empty bricks are skipped to the first lattice point past the empty region,
+//   no step escalation, early termination once v.w > ert (ert > 0).
+// Every decision is continuous in the ray geometry (a sample that appears or is
+// skipped carries ~zero opacity), so FP32 and FP64 renders agree to rounding.
+// Film: o = o0, lf = L, step_size = next lattice index k.
 template <class T>
-void volume_main_lattice(const VolumeShader<T> &sh, VolumePixel<T> &px_io, V3<T> o0, long long *nsamples) {
+void volume_main_lattice(const VolumeShader<T> &sh, VolumePixel<T> &px_io, long long *nsamples) {
   VolumePixel<T> px = px_io;
+  const V3<T> o0 = px.o;
+  const T lf = px.lf;
   long long ns = 0;
   int k = px.step_size;
-  int kend = k + px.nsteps;
+  const int kend = (int)std::ceil(lf);
+  const T cdu = T(1) / comp_max(vabs(px.d));
   while (k < kend) {
-    V3<T> o = o0 + px.d * (T(k) * sh.step);
+    const T w = std::min(T(1), lf - T(k));         // coverage of sample k
+    const T t = (T(k) + T(0.5f) * w) * sh.step;   // midpoint of the covered segment
+    V3<T> o = o0 + px.d * t;
     V3<T> ip = vfloor(o);
     int cd = grid_lookup(sh.cheb, sh.cheb_dim, ip);
     if (cd) {
-      // exit of the empty region along the ray, measured from o0
       Ray<T> r{o, px.d};
       T tn, tf;
       r.intersect(Box<T>{ip, ip + T(1)}, tn, tf);
-      T cdu = T(1) / comp_max(vabs(px.d));
-      T t_exit = T(k) * sh.step + tf + T(cd - 1) * cdu;
-      int kn = (int)std::ceil(t_exit / sh.step);
+      T t_exit = t + tf + T(cd - 1) * cdu;
+      int kn = (int)std::ceil(t_exit / sh.step - T(0.5f));
       k = std::max(kn, k + 1);
       continue;
     }
     int pgid = grid_lookup(sh.paging.vaddr, sh.paging.dim, ip);
     if (pgid == -1) break;
     V3<T> x = o - ip;
-    T kk = T(sh.paging.page_k[pgid]);
-    V3<T> b{T(sh.paging.page_b[3 * pgid]), T(sh.paging.page_b[3 * pgid + 1]), T(sh.paging.page_b[3 * pgid + 2])};
-    V3<T> u = x * kk + b;
+    T kk, bx, by, bz;
+    if (sh.paging.map_bs > 0) {   // exact mapping k = bs/pad, b = (apron/bs) * k in T
+      const int pad = sh.paging.pad;
+      kk = T(sh.paging.map_bs) / T(pad);
+      bx = by = bz = (T((pad - sh.paging.map_bs) / 2) / T(sh.paging.map_bs)) * kk;
+    } else {
+      kk = T(sh.paging.page_k[pgid]);
+      bx = T(sh.paging.page_b[3 * pgid]); by = T(sh.paging.page_b[3 * pgid + 1]); bz = T(sh.paging.page_b[3 * pgid + 2]);
+    }
+    V3<T> u = x * kk + V3<T>{bx, by, bz};
     int pad = sh.paging.pad;
     T s = brick_sample(sh.paging.pool + (size_t)pgid * pad * pad * pad, pad, u);
     V4<T> ub = tf_sample(sh.tf, sh.tf_n, s);
-    ub.w = ub.w * sh.density;
+    ub.w = ub.w * sh.density * w;
     ub *= V4<T>{ub.w, ub.w, ub.w, T(1)};
     px.theta += V3<T>{ub.x, ub.y, ub.z} * px.phi;
     px.phi *= T(1) - ub.w;
@@ -477,9 +495,8 @@ void volume_main_lattice(const VolumeShader<T> &sh, VolumePixel<T> &px_io, V3<T>
     ++k;
     if (sh.ert > T(0) && px.v.w > sh.ert) { k = kend; break; }
   }
-  px.nsteps = std::max(kend - k, 0);
   px.step_size = k;
-  px.o = o0;   // the film keeps the lattice origin so the march can resume
+  px.nsteps = kend - k;
   px_io = px;
   if (nsamples) *nsamples += ns;
 }
@@ -501,9 +518,9 @@ void volume_emit(const VolumeShader<T> &sh, const Ray<T> &ray_in, VolumePixel<T>
     px.o = ray.o;
     px.d = ray.d;
     if (sh.schedule == SCHED_LATTICE) {
-      px.nsteps = std::min(sh.max_steps, int((tfar - tnear) / sh.step));
+      px.lf = std::min((tfar - tnear) / sh.step, T(sh.max_steps));
       px.step_size = 0;
-      volume_main_lattice(sh, px, ray.o, nsamples);
+      volume_main_lattice(sh, px, nsamples);
     } else {
       px.nsteps = std::min(sh.max_steps, int((tfar - tnear) / sh.step));
       px.step_size = 1;
@@ -513,6 +530,7 @@ void volume_emit(const VolumeShader<T> &sh, const Ray<T> &ray_in, VolumePixel<T>
     px.o = ray.o;
     px.d = ray.d;
     px.nsteps = 0;
+    px.lf = T(0);
     px.step_size = sh.schedule == SCHED_LATTICE ? 0 : 1;  // lattice: next index 0
   }
   out = px;

@@ -32,7 +32,7 @@ class OVolumeArgs(C.Structure):
         ("tf", C.POINTER(C.c_float)), ("cheb_dim", C.c_int * 3), ("cheb", C.POINTER(C.c_int)),
         ("vaddr_dim", C.c_int * 3), ("vaddr", C.POINTER(C.c_int)), ("lowest_blkcnt", C.c_int),
         ("page_k", C.POINTER(C.c_float)), ("page_b", C.POINTER(C.c_float)), ("pool", C.POINTER(C.c_float)),
-        ("pad", C.c_int),
+        ("pad", C.c_int), ("map_bs", C.c_int),
     ]
 
 
@@ -88,6 +88,8 @@ def lib() -> C.CDLL:
         L.orc_volume_render.argtypes = [C.c_int, C.c_int, C.POINTER(OView), C.POINTER(OVolumeArgs), PD, PD,
                                         C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.orc_volume_resume.argtypes = [C.c_int, C.c_int, C.POINTER(OVolumeArgs), PD, C.POINTER(C.c_longlong)]
+        L.orc_volume_render_pixels.argtypes = [C.c_int, C.c_int, C.POINTER(OView), C.POINTER(OVolumeArgs), C.c_int,
+                                               C.POINTER(C.c_int), PD, C.POINTER(C.c_longlong)]
         L.orc_volume_trace.argtypes = [C.POINTER(OView), C.POINTER(OVolumeArgs), C.c_int, C.c_int,
                                        C.POINTER(OTrace), C.c_int]
         L.orc_pixel_pass.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_ubyte),
@@ -181,7 +183,8 @@ class VolumeScene:
     """Arguments of one volume render held alive for the oracle."""
 
     def __init__(self, W, H, bbox_min, bbox_max, step, du, tf, cheb, vaddr, page_k, page_b, pool_values, pad,
-                 max_steps=4_000_000, mode=0, rank=0.0, density=1.0, schedule=0, ert=0.0, lowest_blkcnt=0):
+                 max_steps=4_000_000, mode=0, rank=0.0, density=1.0, schedule=0, ert=0.0, lowest_blkcnt=0,
+                 map_bs=0):
         self.keep = [np.ascontiguousarray(tf, np.float32).reshape(-1), np.ascontiguousarray(cheb, np.int32),
                      np.ascontiguousarray(vaddr, np.int32), np.ascontiguousarray(page_k, np.float32),
                      np.ascontiguousarray(page_b, np.float32).reshape(-1),
@@ -201,6 +204,7 @@ class VolumeScene:
         a.vaddr = _pi(va)
         a.lowest_blkcnt = int(lowest_blkcnt)
         a.page_k, a.page_b, a.pool, a.pad = _pf(pk), _pf(pb), _pf(pv), int(pad)
+        a.map_bs = int(map_bs)
         self.args = a
 
     def render(self, view: OView, prec=32, nthreads=0, film=True, counts=False):
@@ -213,6 +217,15 @@ class VolumeScene:
         lib().orc_volume_render(prec, nthreads, C.byref(view), C.byref(a), _pd(film_a) if film else None, _pd(rgba),
                                 cnt.ctypes.data_as(C.POINTER(C.c_longlong)) if counts else None, C.byref(tot))
         return {"film": film_a, "rgba": rgba, "counts": cnt, "samples": tot.value}
+
+    def render_pixels(self, view: OView, xy: np.ndarray, prec=64, nthreads=0):
+        """RGBA of the listed pixels (xy: (n, 2) int array of x, y)."""
+        xy = np.ascontiguousarray(xy, np.int32)
+        out = np.zeros((xy.shape[0], 4))
+        tot = C.c_longlong(0)
+        lib().orc_volume_render_pixels(prec, nthreads, C.byref(view), C.byref(self.args), xy.shape[0], _pi(xy),
+                                       _pd(out), C.byref(tot))
+        return out, tot.value
 
     def resume(self, film: np.ndarray, prec=32, nthreads=0):
         film = np.ascontiguousarray(film, np.float64).copy()

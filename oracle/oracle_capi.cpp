@@ -34,6 +34,7 @@ typedef struct {
   const float *page_b;
   const float *pool;
   int pad;
+  int map_bs;   // > 0: lattice schedule uses the exact mapping for this block size
 } orc_volume_args;
 
 typedef struct {
@@ -106,22 +107,23 @@ template <class T> VolumeShader<T> to_shader(const orc_volume_args *a) {
   s.tf = a->tf; s.tf_n = a->tf_n;
   s.cheb = a->cheb; s.cheb_dim = I3{a->cheb_dim[0], a->cheb_dim[1], a->cheb_dim[2]};
   s.paging = Paging{I3{a->vaddr_dim[0], a->vaddr_dim[1], a->vaddr_dim[2]}, a->vaddr, a->lowest_blkcnt,
-                    a->page_k, a->page_b, a->pool, a->pad};
+                    a->page_k, a->page_b, a->pool, a->pad, a->map_bs};
   s.schedule = a->schedule; s.ert = T(a->ert);
   return s;
 }
 
 // film layout per pixel (16 doubles): o[3] d[3] nsteps step_size v[4] theta[3] phi
-template <class T> void store_pixel(const VolumePixel<T> &p, double *f) {
+// (lattice schedule: f[6] holds the ray extent lf instead of nsteps)
+template <class T> void store_pixel(const VolumePixel<T> &p, double *f, bool lattice) {
   f[0] = p.o.x; f[1] = p.o.y; f[2] = p.o.z; f[3] = p.d.x; f[4] = p.d.y; f[5] = p.d.z;
-  f[6] = p.nsteps; f[7] = p.step_size;
+  f[6] = lattice ? double(p.lf) : double(p.nsteps); f[7] = p.step_size;
   f[8] = p.v.x; f[9] = p.v.y; f[10] = p.v.z; f[11] = p.v.w;
   f[12] = p.theta.x; f[13] = p.theta.y; f[14] = p.theta.z; f[15] = p.phi;
 }
-template <class T> VolumePixel<T> load_pixel(const double *f) {
+template <class T> VolumePixel<T> load_pixel(const double *f, bool lattice) {
   VolumePixel<T> p;
   p.o = V3<T>{T(f[0]), T(f[1]), T(f[2])}; p.d = V3<T>{T(f[3]), T(f[4]), T(f[5])};
-  p.nsteps = int(f[6]); p.step_size = int(f[7]);
+  p.nsteps = lattice ? 0 : int(f[6]); p.lf = lattice ? T(f[6]) : T(0); p.step_size = int(f[7]);
   p.v = V4<T>{T(f[8]), T(f[9]), T(f[10]), T(f[11])};
   p.theta = V3<T>{T(f[12]), T(f[13]), T(f[14])}; p.phi = T(f[15]);
   return p;
@@ -141,7 +143,7 @@ int volume_render(int nthreads, const orc_view *view, const orc_volume_args *a, 
       long long ns = 0;
       volume_emit(sh, r, px, &ns, nullptr);
       size_t i = (size_t)y * a->W + x;
-      if (film) store_pixel(px, film + 16 * i);
+      if (film) store_pixel(px, film + 16 * i, sh.schedule == SCHED_LATTICE);
       if (rgba) { rgba[4 * i] = px.v.x; rgba[4 * i + 1] = px.v.y; rgba[4 * i + 2] = px.v.z; rgba[4 * i + 3] = px.v.w; }
       if (counts) counts[i] = ns;
       row += ns;
@@ -160,12 +162,13 @@ int volume_resume(int nthreads, const orc_volume_args *a, double *film, long lon
     long long row = 0;
     for (int x = 0; x < a->W; ++x) {
       size_t i = (size_t)y * a->W + x;
-      VolumePixel<T> px = load_pixel<T>(film + 16 * i);
+      const bool lat = sh.schedule == SCHED_LATTICE;
+      VolumePixel<T> px = load_pixel<T>(film + 16 * i, lat);
       long long ns = 0;
       // ray_march_shader_impl, shader.hpp:146-157
-      if (sh.schedule == SCHED_LATTICE) volume_main_lattice(sh, px, px.o, &ns);
+      if (lat) volume_main_lattice(sh, px, &ns);
       else volume_main(sh, px, &ns, nullptr);
-      store_pixel(px, film + 16 * i);
+      store_pixel(px, film + 16 * i, lat);
       row += ns;
     }
     tot += row;
@@ -310,6 +313,33 @@ int orc_volume_render(int prec, int nthreads, const orc_view *view, const orc_vo
                       double *rgba, long long *counts, long long *total) {
   if (prec == 32) return volume_render<float>(nthreads, view, a, film, rgba, counts, total);
   return volume_render<double>(nthreads, view, a, film, rgba, counts, total);
+}
+
+// Render only the listed pixels (xy: n pairs); rgba gets 4 doubles per listed pixel.
+int orc_volume_render_pixels(int prec, int nthreads, const orc_view *view, const orc_volume_args *a, int n,
+                             const int *xy, double *rgba, long long *total) {
+  std::atomic<long long> tot{0};
+  auto run = [&](auto tag) {
+    using T = decltype(tag);
+    View<T> v = to_view<T>(view);
+    VolumeShader<T> sh = to_shader<T>(a);
+    const int nt = resolve_threads(nthreads);
+    const int rows = (n + 63) / 64;
+    row_interleave(rows, nt, [&](int r) {
+      long long acc = 0;
+      for (int i = r * 64; i < std::min(n, r * 64 + 64); ++i) {
+        VolumePixel<T> px;
+        long long ns = 0;
+        volume_emit(sh, gen_ray(v, xy[2 * i], xy[2 * i + 1], a->W, a->H), px, &ns, nullptr);
+        rgba[4 * i] = px.v.x; rgba[4 * i + 1] = px.v.y; rgba[4 * i + 2] = px.v.z; rgba[4 * i + 3] = px.v.w;
+        acc += ns;
+      }
+      tot += acc;
+    });
+  };
+  if (prec == 32) run(float{}); else run(double{});
+  if (total) *total = tot.load();
+  return 0;
 }
 
 int orc_volume_resume(int prec, int nthreads, const orc_volume_args *a, double *film, long long *total) {

@@ -1,0 +1,119 @@
+"""GPU tests of the batched persistent renderer (hyd_batch_*).
+
+Bar: per-view error against the FP64 oracle (lattice schedule, exact u8 source
+values) <= 1e-3 max-abs for FP16 and u8 bricks; FP32 bricks agree with the FP32
+oracle to 1e-4; graph replays equal the eager launch bitwise; the launch shape
+(autotuner knobs) never changes the output.
+"""
+import numpy as np
+import pytest
+
+from hydrant_amd import _lib as L
+from hydrant_amd import workloads as W
+from hydrant_amd.hip import batch_config
+from tests.helpers import error_pixels, exact_pool_values, oracle_view, oracle_volume
+
+pytestmark = pytest.mark.gpu
+
+
+def _small(fmt, n_views=6, normalize=False):
+    return W.batched(sizes=(64, 96, 128), n_views=n_views, res_choices=(64, 96, 128, 160), fmt=fmt, bs=32,
+                     apron=1, seed=77, normalize=normalize)
+
+
+def _setup(ctx, wl):
+    ctx.upload_pool(wl.fmt, wl.bs + 2 * wl.apron, wl.pool)
+    ctx.set_tf(wl.tf)
+    ctx.set_volumes(wl.volume_descs())
+    return ctx.batch(wl.hip_views())
+
+
+def _view_errors(wl, out, batch, prec, exact, views=None):
+    errs = []
+    for i in (range(len(wl.views)) if views is None else views):
+        sc = wl.scene(i)
+        img = batch.view_image(out, i).cpu().numpy()
+        xy = error_pixels(sc.W, sc.H)
+        pv = exact_pool_values(sc) if exact else None
+        osc = oracle_volume(sc, pool_values=pv, exact_mapping=exact)
+        ref, _ = osc.render_pixels(oracle_view(sc), xy, prec=prec)
+        got = img[xy[:, 1], xy[:, 0]]
+        errs.append(float(np.max(np.abs(got.astype(np.float64) - ref))))
+    return errs
+
+
+@pytest.mark.parametrize("fmt", [L.FMT_F16, L.FMT_U8, L.FMT_F32])
+def test_batch_vs_fp64_oracle(gpu_ctx, fmt):
+    wl = _small(fmt)
+    b = _setup(gpu_ctx, wl)
+    out, cnt = b.render(batch_config(8, 1, 8), wl.density, wl.ert, count=True)
+    gpu_ctx.synchronize()
+    errs = _view_errors(wl, out, b, 64, exact=True)
+    print(f"fmt {fmt}: per-view max-abs vs FP64 {['%.2e' % e for e in errs]}, samples {int(cnt.item())}", flush=True)
+    assert max(errs) <= 1e-3
+    if fmt == L.FMT_F32:
+        e32 = _view_errors(wl, out, b, 32, exact=False)
+        assert max(e32) <= 1e-4, e32
+    b.close()
+
+
+def test_batch_normalised_fp16_storage_error(gpu_ctx):
+    """FP16 bricks holding float(v)/255 (instead of the exact count v) carry up to 2.4e-4
+    rounding per voxel; the image error against FP64 is reported and bounded loosely."""
+    wl = _small(L.FMT_F16, normalize=True)
+    b = _setup(gpu_ctx, wl)
+    out, _ = b.render(batch_config(8, 1, 8), wl.density, wl.ert)
+    gpu_ctx.synchronize()
+    errs = _view_errors(wl, out, b, 64, exact=True)
+    print(f"normalised fp16: per-view max-abs vs FP64 {['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) <= 1e-2
+    b.close()
+
+
+def test_launch_shape_does_not_change_output(gpu_ctx):
+    import torch
+
+    wl = _small(L.FMT_F16, n_views=4)
+    b = _setup(gpu_ctx, wl)
+    ref, c0 = b.render(batch_config(8, 1, 8), wl.density, wl.ert, count=True)
+    for tw, un, wpc in [(16, 2, 4), (32, 1, 16), (32, 2, 2), (8, 4, 32), (16, 4, 8)]:
+        out, c1 = b.render(batch_config(tw, un, wpc), wl.density, wl.ert, count=True)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), (tw, un, wpc)
+        assert int(c1.item()) == int(c0.item())
+    best, ms = b.autotune(wl.density, wl.ert, b.new_output(), reps=1)
+    assert best.tile_w in (8, 16, 32) and best.unroll in (1, 2, 4) and ms > 0
+    b.close()
+
+
+def test_graph_replay_matches_eager(gpu_ctx):
+    """Capture the persistent launch in a HIP graph; every replay (with eager work in
+    between) equals the eager result bitwise."""
+    import torch
+
+    wl = _small(L.FMT_F16, n_views=5)
+    b = _setup(gpu_ctx, wl)
+    cfg = batch_config(16, 2, 8)
+    ref, _ = b.render(cfg, wl.density, wl.ert)
+    out = b.new_output()
+    b.capture(cfg, wl.density, wl.ert, out)
+    for rep in range(3):
+        out.fill_(-1.0)
+        junk = torch.randn(1024, device="cuda").sum()  # eager work between replays
+        b.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), rep
+        del junk
+    b.close()
+
+
+def test_batch_rejects_bad_config(gpu_ctx):
+    import ctypes as C
+
+    wl = _small(L.FMT_F16, n_views=2)
+    b = _setup(gpu_ctx, wl)
+    out = b.new_output()
+    bad = batch_config(12, 1, 8)
+    code = L.lib().hyd_batch_render(gpu_ctx.h, b.h, C.byref(bad), 0.01, 0.9995, C.c_void_p(out.data_ptr()), None)
+    assert code == L.HYD_ERR_ARG
+    b.close()

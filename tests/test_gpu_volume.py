@@ -36,7 +36,10 @@ def _compare(ctx, sc, trace_pixels=(), check_film=True):
     if check_film:
         for k in ("v", "theta", "phi", "o"):
             assert max_abs(g[k], o[k]) <= TOL, k
-        assert np.array_equal(g["nsteps"], o["nsteps"])
+        if sc.schedule == L.SCHED_LATTICE:   # nsteps slot holds the float ray extent
+            assert np.array_equal(g["nsteps"].view(np.float32), r["film"][..., 6].astype(np.float32))
+        else:
+            assert np.array_equal(g["nsteps"], o["nsteps"])
         assert np.array_equal(g["step_size"], o["step_size"])
     for (x, y) in trace_pixels:
         gt = ctx.trace_volume(p, view, sc.W, sc.H, x, y)

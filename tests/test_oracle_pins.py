@@ -279,6 +279,41 @@ def test_fp32_vs_fp64_reference_schedule_is_discontinuous():
     assert r32["samples"] > 0
 
 
+def test_lattice_schedule_is_continuous_fp32_vs_fp64():
+    """The lattice schedule (midpoint samples, coverage-weighted last sample, lattice skips,
+    ERT) has no whole-sample flips: FP32 and FP64 renders of the same data agree to ~1e-5."""
+    from hydrant_amd import _lib as L
+    from hydrant_amd import workloads as W
+    from tests.helpers import oracle_view, oracle_volume
+
+    sc = W.paged_volume(n=96, W=48, H=48, bs=32, resident=1.0, schedule=L.SCHED_LATTICE)
+    sc.ert = 0.9995
+    ov = oracle_view(sc)
+    r32 = oracle_volume(sc, exact_mapping=True).render(ov, 32, film=False)
+    r64 = oracle_volume(sc, exact_mapping=True).render(ov, 64, film=False)
+    assert np.max(np.abs(r32["rgba"] - r64["rgba"])) < 2e-5
+
+
+def test_lattice_coverage_weight_closed_form():
+    """Constant field along an axis ray: L = chord / step samples-worth of opacity, i.e.
+    v.w = 1 - (1 - a)^floor(L) * (1 - a * frac(L)) with a the per-sample opacity."""
+    from hydrant_amd import _lib as L
+
+    pool = np.full((1, 16, 16, 16), np.float32(0.5), np.float32)
+    tf = np.tile(np.asarray([0.2, 0.4, 0.6, 1.0], np.float32), (4, 1))
+    step = 0.013
+    v = O.make_view((3, 0, 0), (0, 0, 0), (0, 1, 0), 60.0, (0.5, 0.5, 0.5), (1, 1, 1), 64)
+    sc = O.VolumeScene(2, 2, (0, 0, 0), (1, 1, 1), step, 1 / 16, tf, np.zeros((1, 1, 1)), np.zeros((1, 1, 1)),
+                       np.ones(1), np.zeros((1, 3)), pool, 16, density=0.05, schedule=L.SCHED_LATTICE)
+    r = sc.render(v, 64, film=True, counts=True)
+    lf = 1.0 / step
+    a = 0.05
+    want = 1 - (1 - a) ** np.floor(lf) * (1 - a * (lf - np.floor(lf)))
+    assert abs(r["rgba"][1, 1, 3] - want) < 1e-12
+    assert r["counts"][1, 1] == int(np.ceil(lf))
+    assert abs(r["film"][1, 1, 6] - lf) < 1e-12
+
+
 # ---------------------------------------------------------- resume (A16) ----
 def test_oracle_resume_bit_exact():
     """Partial residency then full residency + resume == one uninterrupted march, bitwise."""
