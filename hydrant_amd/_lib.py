@@ -116,6 +116,9 @@ SIGNATURES = {
     "hyd_pixel_pass": (I32, [P, P, I32, I32, C.POINTER(C.c_uint8), P]),
     "hyd_fetch_pass": (I32, [P, P, I32, I32, P]),
     "hyd_composite": (I32, [P, I32, P, PI32, I32, I32, P, P]),
+    "hyd_segments_begin": (I32, [P, I32, I32, I32, P]),
+    "hyd_segments_pass": (I32, [P, C.POINTER(VolumeParams), C.POINTER(View), I32, I32, I32, P, P, P]),
+    "hyd_segments_composite": (I32, [P, I32, I32, I32, P, P, P]),
     "hyd_trace_volume": (I32, [P, C.POINTER(VolumeParams), C.POINTER(View), I32, I32, I32, I32,
                                C.POINTER(TraceRec), I32, PI32]),
     "hyd_render_blocks": (I32, [P, C.POINTER(BlocksParams), C.POINTER(View), I32, I32, P]),

@@ -327,6 +327,49 @@ int32_t hyd_composite(hyd_ctx *c, int32_t n, const hyd_fetch_pixel *partials_dev
   return HYD_OK;
 }
 
+int32_t hyd_segments_begin(hyd_ctx *c, int32_t width, int32_t height, int32_t max_slots, float *slots_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(slots_dev && width > 0 && height > 0 && max_slots > 0, HYD_ERR_ARG, "invalid segment buffers");
+  HYD_CHECK_HIP(launch_segments_clear(reinterpret_cast<float4 *>(slots_dev), (size_t)width * height * max_slots,
+                                      c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_segments_pass(hyd_ctx *c, const hyd_volume_params *p, const hyd_view *view, int32_t width,
+                          int32_t height, int32_t max_slots, float *slots_dev, int32_t *nslots_dev,
+                          unsigned long long *samples_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  if ((st = check_volume_state(c))) return st;
+  HYD_REQUIRE(view && slots_dev && nslots_dev && width > 0 && height > 0, HYD_ERR_ARG, "invalid segment pass");
+  const int32_t need = c->paging.vdim[0] + c->paging.vdim[1] + c->paging.vdim[2];
+  HYD_REQUIRE(max_slots >= need, HYD_ERR_RANGE, "max_slots %d < %d (sum of the brick grid dims)", max_slots, need);
+  VolumeArgs a;
+  if ((st = fill_volume_args(c, p, a))) return st;
+  HYD_REQUIRE(a.schedule == HYD_SCHED_LATTICE, HYD_ERR_ARG, "segments need the lattice schedule");
+  fill_view(view, a.trans, a.ctg, a.ray_o);
+  a.W = width;
+  a.H = height;
+  a.slots = reinterpret_cast<float4 *>(slots_dev);
+  a.nslots = nslots_dev;
+  a.max_slots = max_slots;
+  a.samples = samples_dev;
+  HYD_CHECK_HIP(launch_segments_pass(a, c->pool.fmt, c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_segments_composite(hyd_ctx *c, int32_t width, int32_t height, int32_t max_slots,
+                               const float *slots_dev, const int32_t *nslots_dev, float *rgba_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(slots_dev && nslots_dev && rgba_dev && width > 0 && height > 0 && max_slots > 0, HYD_ERR_ARG,
+              "invalid segment composite");
+  HYD_CHECK_HIP(launch_segments_composite(reinterpret_cast<const float4 *>(slots_dev), nslots_dev, max_slots,
+                                          width * height, reinterpret_cast<float4 *>(rgba_dev), c->stream));
+  return HYD_OK;
+}
+
 int32_t hyd_trace_volume(hyd_ctx *c, const hyd_volume_params *p, const hyd_view *view, int32_t width,
                          int32_t height, int32_t x, int32_t y, hyd_trace_rec *out, int32_t max_recs,
                          int32_t *n_out) {

@@ -119,9 +119,17 @@ struct VolumeArgs {
   hyd_volume_pixel *film;
   float4 *rgba;
   unsigned long long *samples;
+  // sort-last segments: slots[ord * W*H + ray], nslots[ray]
+  float4 *slots;
+  int32_t *nslots;
+  int32_t max_slots;
 };
 
 hipError_t launch_volume(const VolumeArgs &a, int fmt, bool resume, hipStream_t s);
+hipError_t launch_segments_clear(float4 *slots, size_t n, hipStream_t s);
+hipError_t launch_segments_pass(const VolumeArgs &a, int fmt, hipStream_t s);
+hipError_t launch_segments_composite(const float4 *slots, const int32_t *nslots, int max_slots, int P, float4 *rgba,
+                                     hipStream_t s);
 hipError_t launch_volume_trace(const VolumeArgs &a, int fmt, int x, int y, hyd_trace_rec *out_dev, int max_recs,
                                int *n_dev, hipStream_t s);
 hipError_t launch_pixel_pass(const float4 *rgba, int W, int H, uchar3 clear, uint8_t *rgb, hipStream_t s);

@@ -352,6 +352,115 @@ __global__ __launch_bounds__(64) void volume_trace_kernel(VolumeArgs a, int x, i
   *n_out = nt;
 }
 
+// ---------------------------------------------------------------------------
+// Sort-last segments.  The lattice walk of a ray visits cells in order (every
+// coordinate of o0 + d*t is monotone in t, so a cell is never revisited); the
+// k-th visited cell owns slot k of the ray.  A pass composites, front to back
+// from zero, the samples of every visited cell whose brick is resident in the
+// current page table into its slot (each sample reads the brick of its own
+// floor(p), exactly as the single march does); empty cells write a transparent
+// slot; cells not resident now are left for a later pass.  The final fold
+// composites the slots in order and stops at the first slot no pass filled
+// (the reference stops a ray at its first non-resident brick).
+template <int FMT, bool COUNT>
+__global__ __launch_bounds__(64) void segment_kernel(VolumeArgs a) {
+  __shared__ float s_tf[4 * kMaxTf];
+  __shared__ float s_lut[FMT == HYD_FMT_U8 ? 256 : 1];
+  __shared__ unsigned long long s_count;
+  for (int i = threadIdx.x; i < 4 * a.tf_n; i += blockDim.x) s_tf[i] = a.tf[i];
+  if (FMT == HYD_FMT_U8)
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_lut[i] = a.u8_lut[i];
+  if (COUNT && threadIdx.x == 0) s_count = 0;
+  __syncthreads();
+  const int x = blockIdx.x * kTile + (threadIdx.x % kTile);
+  const int y = blockIdx.y * kTile + (threadIdx.x / kTile);
+  unsigned count = 0;
+  if (x < a.W && y < a.H) {
+    const size_t P = (size_t)a.W * a.H, ray = (size_t)y * a.W + x;
+    MarchState st;
+    int ord = 0;
+    if (emit(a, x, y, st)) {
+      const f3 o0 = st.o, d = st.d;
+      const float lf = __int_as_float(st.nsteps);
+      const int kend = (int)ceilf(lf);
+      const float cdu = 1.f / comp_max(abs3(d));
+      int k = 0;
+      while (k < kend && ord < a.max_slots) {
+        float w = gmin(1.f, lf - float(k));
+        float t = (float(k) + 0.5f * w) * a.step;
+        f3 o = add(o0, muls(d, t));
+        const f3 ip = floor3(o);
+        const int cd = a.cheb[grid_index(ip, a.cdim)];
+        if (cd) {   // empty cell: same skip as the single lattice march
+          float tn, tfar;
+          intersect_box(o, d, ip, f3{ip.x + 1.f, ip.y + 1.f, ip.z + 1.f}, tn, tfar);
+          const float t_exit = t + tfar + float(cd - 1) * cdu;
+          const int kn = (int)ceilf(t_exit / a.step - 0.5f);
+          k = kn > k + 1 ? kn : k + 1;
+          a.slots[(size_t)ord * P + ray] = make_float4(0.f, 0.f, 0.f, 0.f);
+          ++ord;
+          continue;
+        }
+        const int pgid = a.vaddr[grid_index(ip, a.vdim)];
+        MarchState seg;
+        seg.v = make_float4(0.f, 0.f, 0.f, 0.f);
+        seg.theta = f3{0.f, 0.f, 0.f};
+        seg.phi = 1.f;
+        const float4 pg = pgid >= 0 ? a.page[pgid] : make_float4(0.f, 0.f, 0.f, 0.f);
+        while (true) {
+          if (pgid >= 0) {
+            const float s = brick_sample<FMT>(a.pool, a.pad, pgid, pg, sub(o, ip), s_lut, nullptr);
+            shade_composite(a, s_tf, s, pgid, seg, w);
+            if (COUNT) ++count;
+          }
+          if (++k >= kend) break;
+          w = gmin(1.f, lf - float(k));
+          t = (float(k) + 0.5f * w) * a.step;
+          o = add(o0, muls(d, t));
+          const f3 nip = floor3(o);
+          if (nip.x != ip.x || nip.y != ip.y || nip.z != ip.z) break;
+        }
+        if (pgid >= 0) a.slots[(size_t)ord * P + ray] = seg.v;
+        ++ord;
+      }
+    }
+    a.nslots[ray] = ord;
+  }
+  if (COUNT) {
+    if (count) atomicAdd(&s_count, (unsigned long long)count);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_count) atomicAdd(a.samples, s_count);
+  }
+}
+
+__global__ __launch_bounds__(256) void segment_clear_kernel(float4 *slots, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) slots[i] = make_float4(0.f, 0.f, 0.f, -1.f);   // alpha -1: not filled yet
+}
+
+__global__ __launch_bounds__(256) void segment_fold_kernel(const float4 *slots, const int32_t *nslots,
+                                                           int max_slots, int P, float4 *rgba) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P) return;
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  const int n = min(nslots[i], max_slots);
+  for (int ord = 0; ord < n; ++ord) {
+    const float4 s = slots[(size_t)ord * P + i];
+    if (s.w < 0.f) break;
+    const float t = 1.f - v.w;
+    v = make_float4(v.x + s.x * t, v.y + s.y * t, v.z + s.z * t, v.w + s.w * t);
+  }
+  rgba[i] = v;
+}
+
+template <int FMT>
+hipError_t launch_segments_fmt(const VolumeArgs &a, hipStream_t s) {
+  dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile);
+  if (a.samples) hipLaunchKernelGGL((segment_kernel<FMT, true>), grid, dim3(64), 0, s, a);
+  else hipLaunchKernelGGL((segment_kernel<FMT, false>), grid, dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
 template <int FMT>
 hipError_t launch_fmt(const VolumeArgs &a, bool resume, hipStream_t s) {
   dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile);
@@ -375,6 +484,25 @@ hipError_t launch_volume(const VolumeArgs &a, int fmt, bool resume, hipStream_t 
     case HYD_FMT_F16: return launch_fmt<HYD_FMT_F16>(a, resume, s);
     default: return launch_fmt<HYD_FMT_F32>(a, resume, s);
   }
+}
+
+hipError_t launch_segments_clear(float4 *slots, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(segment_clear_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, slots, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_segments_pass(const VolumeArgs &a, int fmt, hipStream_t s) {
+  switch (fmt) {
+    case HYD_FMT_U8: return launch_segments_fmt<HYD_FMT_U8>(a, s);
+    case HYD_FMT_F16: return launch_segments_fmt<HYD_FMT_F16>(a, s);
+    default: return launch_segments_fmt<HYD_FMT_F32>(a, s);
+  }
+}
+
+hipError_t launch_segments_composite(const float4 *slots, const int32_t *nslots, int max_slots, int P, float4 *rgba,
+                                     hipStream_t s) {
+  hipLaunchKernelGGL(segment_fold_kernel, dim3((P + 255) / 256), dim3(256), 0, s, slots, nslots, max_slots, P, rgba);
+  return hipGetLastError();
 }
 
 hipError_t launch_volume_trace(const VolumeArgs &a, int fmt, int x, int y, hyd_trace_rec *out_dev, int max_recs,

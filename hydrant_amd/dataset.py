@@ -192,6 +192,24 @@ def make_bricked(vol: np.ndarray, bs: int, apron: int, fmt_out: int, empty_fn=No
                    order=order[:nn.value].copy(), n_nonempty=nn.value, bmin=bmin, bmax=bmax, raw_counts=raw)
 
 
+def residency_batches(vol: np.ndarray, b: Bricked, pages_per_batch: int, n_batches: int | None = None):
+    """Split the non-empty bricks (in the page-table order) into residency batches of at most
+    pages_per_batch pages, like LosslessBlockPagingState::next (lossless_block_paging.cc:137-174):
+    each batch gets its own pool (pages 0..n-1) and a page table mapping only its bricks.
+    Returns (fmt, pad, pool, vaddr, page_k, page_b) tuples."""
+    out = []
+    order = b.order
+    nb = (len(order) + pages_per_batch - 1) // pages_per_batch
+    for i in range(nb if n_batches is None else min(nb, n_batches)):
+        bricks = order[i * pages_per_batch:(i + 1) * pages_per_batch]
+        pool = pack(vol, b.bs, b.apron, b.fmt, bricks, normalize=not b.raw_counts)
+        vaddr = np.full(b.vaddr.shape, -1, np.int32)
+        vaddr.reshape(-1)[bricks] = np.arange(len(bricks), dtype=np.int32)
+        out.append((b.fmt, b.pad, pool, vaddr, b.page_k[:1].repeat(len(bricks)),
+                    np.repeat(b.page_b[:1], len(bricks), axis=0)))
+    return out
+
+
 def single_brick(vol: np.ndarray, fmt_out: int = L.FMT_F32) -> Bricked:
     """One brick covering a cubic volume, no apron (block_size = raw dim): k = 1, b = 0."""
     n = vol.shape[0]

@@ -185,6 +185,30 @@ class Context:
                C.c_void_p(rgba.data_ptr()), C.c_void_p(rgb.data_ptr()))
         return rgba, rgb
 
+    def render_sort_last(self, params: L.VolumeParams, view: L.View, W: int, H: int, batches, grid_dims,
+                         count: bool = False):
+        """Sort-last render over residency batches: batches is a list of (fmt, pad, pool, vaddr, page_k,
+        page_b); each batch is made resident in turn and its bricks' per-ray partials are written,
+        then the partials are composited on the GPU.  Returns (rgba, samples)."""
+        t = self.torch
+        self.sync_stream()
+        max_slots = int(sum(grid_dims))
+        slots = t.empty((max_slots, H, W, 4), dtype=t.float32, device=self.dev)
+        nslots = t.zeros((H, W), dtype=t.int32, device=self.dev)
+        cnt = t.zeros(1, dtype=t.int64, device=self.dev) if count else None
+        L.call("hyd_segments_begin", self.h, W, H, max_slots, C.c_void_p(slots.data_ptr()))
+        for fmt, pad, pool, vaddr, page_k, page_b in batches:
+            self.upload_pool(fmt, pad, pool)
+            self.set_page_table(vaddr, page_k, page_b)
+            self.sync_stream()
+            L.call("hyd_segments_pass", self.h, C.byref(params), C.byref(view), W, H, max_slots,
+                   C.c_void_p(slots.data_ptr()), C.c_void_p(nslots.data_ptr()),
+                   C.c_void_p(cnt.data_ptr() if count else 0))
+        rgba = t.empty((H, W, 4), dtype=t.float32, device=self.dev)
+        L.call("hyd_segments_composite", self.h, W, H, max_slots, C.c_void_p(slots.data_ptr()),
+               C.c_void_p(nslots.data_ptr()), C.c_void_p(rgba.data_ptr()))
+        return rgba, cnt
+
     def trace_volume(self, params: L.VolumeParams, view: L.View, W: int, H: int, x: int, y: int,
                      max_recs: int = 8192) -> np.ndarray:
         recs = (L.TraceRec * max_recs)()

@@ -216,6 +216,26 @@ int32_t hyd_fetch_pass(hyd_ctx *ctx, const hyd_volume_pixel *film_dev, int32_t w
  * sRGB + saturate (volume.cc:267-292).  rgba_dev / rgb_dev may be NULL. */
 int32_t hyd_composite(hyd_ctx *ctx, int32_t n, const hyd_fetch_pixel *partials_dev, const int32_t *z_order,
                       int32_t width, int32_t height, float *rgba_dev, uint8_t *rgb_dev);
+/* Sort-last rendering over residency batches.  The reference renders a volume
+ * larger than the pool in batches by pausing rays at non-resident bricks and
+ * resuming them (volume.cc:95-134), which needs the bricks in ray order; here
+ * each ray's per-brick partial (premultiplied RGBA of the samples inside one
+ * brick, LATTICE schedule, no early termination) is written to slot k of the
+ * ray for its k-th visited brick, in any brick order, and the partials are
+ * composited front to back on the GPU (the sort-last "over" of volume.cc:267-292
+ * applied per ray).  slots_dev holds max_slots * width * height float4
+ * (slot-major); max_slots >= sum of the brick grid dims.  nslots_dev holds one
+ * int32 per pixel. */
+int32_t hyd_segments_begin(hyd_ctx *ctx, int32_t width, int32_t height, int32_t max_slots, float *slots_dev);
+/* One pass with the context's current pool and page table: fills the slots of
+ * every visited brick resident now.  Repeat for every residency batch. */
+int32_t hyd_segments_pass(hyd_ctx *ctx, const hyd_volume_params *p, const hyd_view *view, int32_t width,
+                          int32_t height, int32_t max_slots, float *slots_dev, int32_t *nslots_dev,
+                          unsigned long long *samples_dev);
+/* Fold the slots in ray order into RGBA; a ray stops at its first slot that no
+ * pass filled (a brick never resident), as the reference stops at a page fault. */
+int32_t hyd_segments_composite(hyd_ctx *ctx, int32_t width, int32_t height, int32_t max_slots,
+                               const float *slots_dev, const int32_t *nslots_dev, float *rgba_dev);
 /* Per-sample trace of one pixel (debug; synchronous; FP32 reference schedule):
  * writes up to max_recs records to host memory, *n_out = count. */
 int32_t hyd_trace_volume(hyd_ctx *ctx, const hyd_volume_params *p, const hyd_view *view, int32_t width,

@@ -116,6 +116,43 @@ def test_resume_equals_single_pass(gpu_ctx):
     assert max_abs(resumed["rgba"].cpu().numpy(), r2["rgba"]) <= TOL
 
 
+def test_sort_last_segments_equal_single_march(gpu_ctx):
+    """In-GPU sort-last composite of per-brick partials over 3 residency batches equals the
+    single lattice march (all bricks resident, no early termination) to rounding, takes the
+    same samples, and matches the oracle's single march."""
+    from hydrant_amd import dataset as D
+
+    sc = W.paged_volume(n=256, W=128, H=128, bs=64, resident=1.0, schedule=L.SCHED_LATTICE)
+    sc.ert = 0.0
+    b = sc.bricked
+    vol = sc.extra["volume"]
+    ctx = gpu_ctx
+    ctx.load(b, sc.tf)
+    p, view = sc.volume_params(), sc.view()
+    ref = ctx.render_volume(p, view, sc.W, sc.H, film=False, count=True)
+    per = max(1, int(round(0.4 * b.n_nonempty)))
+    batches = D.residency_batches(vol, b, per)
+    assert len(batches) == 3
+    rgba, cnt = ctx.render_sort_last(p, view, sc.W, sc.H, batches, b.grid, count=True)
+    ctx.synchronize()
+    err = max_abs(rgba.cpu().numpy(), ref["rgba"].cpu().numpy())
+    print(f"sort-last vs single march: max-abs {err:.3g}, samples {int(cnt.item())} / {int(ref['samples'].item())}",
+          flush=True)
+    assert err <= 1e-5
+    assert int(cnt.item()) == int(ref["samples"].item())
+    o = oracle_volume(sc).render(oracle_view(sc), 32, film=False)
+    assert max_abs(rgba.cpu().numpy(), o["rgba"]) <= TOL
+    # only two of the three batches ever resident: rays stop at the first missing brick
+    part, _ = ctx.render_sort_last(p, view, sc.W, sc.H, batches[:2], b.grid)
+    vaddr2 = np.full(b.vaddr.shape, -1, np.int32)
+    vaddr2.reshape(-1)[b.order[:2 * per]] = 0   # any page: only residency matters for the stop
+    full_pool = b.pool_values()
+    vaddr_o = b.vaddr.copy()
+    vaddr_o[vaddr2 < 0] = -1
+    o2 = oracle_volume(sc, vaddr=vaddr_o).render(oracle_view(sc), 32, film=False)
+    assert max_abs(part.cpu().numpy(), o2["rgba"]) <= TOL
+
+
 def test_pixel_pass_matches_oracle(gpu_ctx):
     from oracle import oracle as O
 
