@@ -6,8 +6,11 @@ HBM.  Workloads:
   plain    one 512x512 view of a 256^3 Gaussian-blob volume stored as one FP32
            brick, reference step schedule (escalation), density 1e-2 (the plain
            single-view FP32 form).
-  batched  the batched FP16-brick form (see hydrant_amd/batch.py): many views of
-           several volumes in one persistent launch.
+  batched  the batched FP16-brick form (hydrant_amd/hip.py Batch over
+           k_batch.hip): many views of several volumes in one persistent launch.
+  diff     the differentiable form (hydrant_amd/diff.py): 32 views at 256^2 of a
+           128^3 FP32 volume, 512 samples per ray, transmittance checkpoints every
+           64; forward and forward + adjoint times, and a 200-step Adam fit.
 Unit of work: one composited sample (a march step that does a TF lookup and a
 front-to-back composite); the count comes from the kernel's counting variant
 and equals the oracle's.  Prints ONE JSON line.
@@ -148,6 +151,59 @@ def bench_batched(torch, args, plain_ms_per_sample=None):
     return res
 
 
+def bench_diff(torch, args):
+    """Differentiable form: forward (with checkpoints) and forward + adjoint over 32 views at
+    256^2 of a 128^3 FP32 volume, K = 512; then an Adam fit (lr 1e-2) of a constant volume
+    and a grey TF to the hidden pair's renders."""
+    from hydrant_amd import workloads as W
+    from hydrant_amd.diff import DiffRenderer
+    from hydrant_amd.hip import Context
+
+    wl = W.differentiable()
+    ctx = Context(0)
+    r = DiffRenderer(ctx, wl.views(), wl.W, wl.H, (wl.n,) * 3, wl.n_steps, wl.ckpt_every, wl.sigma)
+    vol = torch.from_numpy(wl.hidden_volume).cuda()
+    tf = torch.from_numpy(wl.hidden_tf).cuda()
+    target, ck, cnt = r.forward_raw(vol, tf, count=True)
+    torch.cuda.synchronize()
+    n_samples = int(cnt.item())
+    g = torch.randn(target.shape, generator=torch.Generator(device="cuda").manual_seed(7), device="cuda")
+
+    def fwd():
+        r.forward_raw(vol, tf)
+
+    def fwd_adj():
+        _, c, _ = r.forward_raw(vol, tf)
+        r.backward_raw(vol, tf, c, g)
+
+    wall_f, ev_f = time_kernel(torch, fwd, args.steps, args.warmup)
+    wall_b, ev_b = time_kernel(torch, fwd_adj, args.steps, args.warmup)
+    f_ms, fb_ms = wall_f * 1e3 / args.steps, wall_b * 1e3 / args.steps
+    res = {"views": len(wl.arms), "res": wl.W, "volume": wl.n, "n_steps": wl.n_steps, "ckpt_every": wl.ckpt_every,
+           "samples_per_forward": n_samples, "fwd_ms": round(f_ms, 4), "fwd_kernel_ms": round(float(np.mean(ev_f)), 4),
+           "fwd_adjoint_ms": round(fb_ms, 4), "fwd_adjoint_event_ms": round(float(np.mean(ev_b)), 4),
+           "ratio_fwd_adjoint_vs_fwd": round(fb_ms / f_ms, 3),
+           "fwd_msamples_per_s": round(n_samples / (f_ms * 1e-3) / 1e6, 3)}
+    if args.diff_fit_steps > 0:
+        v = torch.from_numpy(wl.init_volume).cuda().requires_grad_(True)
+        t = torch.from_numpy(wl.init_tf).cuda().requires_grad_(True)
+        opt = torch.optim.Adam([v, t], lr=1e-2)
+        losses = []
+        t0 = time.perf_counter()
+        for _ in range(args.diff_fit_steps):
+            opt.zero_grad()
+            loss = torch.mean((r.render(v, t) - target) ** 2)
+            loss.backward()
+            opt.step()
+            losses.append(loss.detach())
+        torch.cuda.synchronize()
+        ls = [float(x.item()) for x in losses]
+        res["adam_fit"] = {"steps": args.diff_fit_steps, "lr": 1e-2, "loss_first": ls[0], "loss_last": ls[-1],
+                           "loss_min": min(ls), "s": round(time.perf_counter() - t0, 3)}
+    ctx.close()
+    return res
+
+
 def batched_errors(res, n_views: int = 6):
     """Per-view max-abs error of the batched FP16 output against the FP64 oracle (exact u8
     source values) on the first n_views views: full frame up to 512^2, else a 64x64 crop
@@ -223,8 +279,10 @@ def main():
                     help="headline workload (the other one is reported alongside)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg and the FP64 error check")
     ap.add_argument("--err-views", type=int, default=6, help="batched views checked against the FP64 oracle")
-    ap.add_argument("--only", choices=["plain", "batched"],
+    ap.add_argument("--only", choices=["plain", "batched", "diff"],
                     help="profiling aid: run only this workload's timed launches and print its line")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-form leg")
+    ap.add_argument("--diff-fit-steps", type=int, default=200, help="Adam steps of the differentiable fit")
     ap.add_argument("--config", help="batched launch shape tile_w,unroll,waves_per_cu (skips the autotuner)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="profiling aid: copy a MIB-sized buffer 3 times (scales FETCH_SIZE / WRITE_SIZE)")
@@ -249,6 +307,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n * 4}), flush=True)
         return
+    if args.only == "diff":
+        print(json.dumps(bench_diff(torch, args)), flush=True)
+        return
     if args.only:
         res = bench_plain(torch, args) if args.only == "plain" else bench_batched(torch, args)
         print(json.dumps({k: v for k, v in res.items() if isinstance(v, (int, float, str, dict))}), flush=True)
@@ -260,6 +321,11 @@ def main():
     batched = bench_batched(torch, args)
     print(f"# batched fp16: {batched['msamples_per_s']:.1f} Msamples/s, {batched['ms_per_step']:.3f} ms/batch, "
           f"config {batched['config']}", file=sys.stderr, flush=True)
+
+    diff = None if args.no_diff else bench_diff(torch, args)
+    if diff is not None:
+        print(f"# diff: fwd {diff['fwd_ms']:.3f} ms, fwd+adjoint {diff['fwd_adjoint_ms']:.3f} ms "
+              f"(x{diff['ratio_fwd_adjoint_vs_fwd']:.2f})", file=sys.stderr, flush=True)
 
     head = batched if args.workload == "batched" else plain
     plain_info = {"msamples_per_s": round(plain["msamples_per_s"], 3), "ms_per_frame": round(plain["ms_per_step"], 4),
@@ -298,6 +364,7 @@ def main():
         "config": {"workload": workload, "samples_per_step": head["n_samples"]},
         "plain_fp32": plain_info,
         "batched_fp16": batched_info,
+        "diff": diff,
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": kern, "kernel_ms": round(head["kernel_ms"], 4),

@@ -86,6 +86,10 @@ class BatchView(C.Structure):
                 ("out_offset", C.c_int64)]
 
 
+class DiffParams(C.Structure):
+    _fields_ = [("dims", i3), ("n_steps", I32), ("ckpt_every", I32), ("sigma", C.c_float)]
+
+
 class BatchConfig(C.Structure):
     _fields_ = [("tile_w", I32), ("unroll", I32), ("waves_per_cu", I32)]
 
@@ -123,6 +127,9 @@ SIGNATURES = {
                                C.POINTER(TraceRec), I32, PI32]),
     "hyd_render_blocks": (I32, [P, C.POINTER(BlocksParams), C.POINTER(View), I32, I32, P]),
     "hyd_render_iso": (I32, [P, C.POINTER(IsoParams), C.POINTER(View), I32, I32, P, P]),
+    "hyd_diff_ckpt_floats": (I32, [C.POINTER(DiffParams), PI32]),
+    "hyd_diff_forward": (I32, [P, C.POINTER(DiffParams), C.POINTER(View), I32, I32, I32, P, P, I32, P, P, P]),
+    "hyd_diff_backward": (I32, [P, C.POINTER(DiffParams), C.POINTER(View), I32, I32, I32, P, P, I32, P, P, P, P]),
     "hyd_set_volumes": (I32, [P, I32, C.POINTER(VolumeDesc)]),
     "hyd_batch_create": (I32, [P, C.POINTER(BatchView), I32, C.POINTER(P)]),
     "hyd_batch_destroy": (I32, [P]),

@@ -85,6 +85,9 @@ struct Ctx {
   int32_t n_vols = 0;
   int32_t vol_pad = 0;            // padded brick edge shared by every volume
   int32_t n_cus = 0;
+  // differentiable renderer: device copy of the views
+  void *own_diff_views = nullptr;
+  int32_t diff_view_cap = 0;
 };
 
 // Flattened kernel arguments for the volume march (passed by value).

@@ -221,6 +221,49 @@ def batched(sizes=(256, 512, 768), n_views: int = 96, res_choices=(256, 384, 512
     return BatchedWorkload(tuple(sizes), fmt, vols, pool, views, tf, density, ert, bs, apron)
 
 
+SEED_DIFF = 5005
+
+
+@dataclass
+class DiffWorkload:
+    """Differentiable form: fit a volume and a TF to target views of hidden ones."""
+    n: int
+    W: int
+    H: int
+    arms: list
+    hidden_volume: np.ndarray
+    hidden_tf: np.ndarray
+    init_volume: np.ndarray
+    init_tf: np.ndarray
+    n_steps: int = 512
+    ckpt_every: int = 64
+    sigma: float = 8.0
+
+    def views(self):
+        from .hip import make_view, orbit_camera
+        c = np.full(3, 0.5, np.float32)
+        s = np.ones(3, np.float32)
+        return [make_view(orbit_camera(a), c, s) for a in self.arms]
+
+
+def differentiable(n: int = 128, n_views: int = 32, W: int = 256, H: int = 256, seed: int = SEED_DIFF,
+                   n_steps: int = 512, ckpt_every: int = 64, sigma: float = 8.0) -> DiffWorkload:
+    """Hidden n^3 FP32 blob volume (8 blobs, sigma 6-20 voxels at n = 128, scaled with n) and a
+    hidden 256-entry TF of seeded smooth bumps; n_views seeded orbit views (arm.x ~ U[0, 360),
+    arm.y ~ U[-60, 60], distance 2.2).  The fit starts from a constant volume (0.25) and a grey
+    translucent TF."""
+    from .diff import smooth_tf
+
+    vol = D.gen_blobs(n, 8, seed, sig=(6.0, 20.0), sigma_scale=n / 128.0, amp=(0.3, 1.0))
+    hidden = vol.astype(np.float32) / np.float32(255.0)
+    tf = smooth_tf(256, seed + 1)
+    rng = np.random.default_rng(seed + 2)
+    arms = [(float(rng.uniform(0.0, 360.0)), float(rng.uniform(-60.0, 60.0)), 2.2) for _ in range(n_views)]
+    init_v = np.full((n, n, n), 0.25, np.float32)
+    init_tf = np.tile(np.array([0.5, 0.5, 0.5, 0.2], np.float32), (256, 1))
+    return DiffWorkload(n, W, H, arms, hidden, tf, init_v, init_tf, n_steps, ckpt_every, sigma)
+
+
 def blocks_params(sc: Scene, mode: int = L.BLOCKS_VOLUME):
     p = L.BlocksParams()
     p.bbox_min = L.f3(0, 0, 0)

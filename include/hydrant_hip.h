@@ -308,6 +308,36 @@ int32_t hyd_batch_replay(hyd_ctx *ctx, hyd_batch *batch);
 int32_t hyd_batch_autotune(hyd_ctx *ctx, hyd_batch *batch, float density, float ert, float *rgba_dev, int32_t reps,
                            hyd_batch_config *best, float *best_ms);
 
+/* -------------------------------------------- differentiable form ----- */
+/* A differentiable variant of the Volume march (no counterpart in the
+ * reference): one FP32 brick of dims voxels spanning the unit box, n_steps
+ * samples per ray at the midpoints of equal sub-intervals of the clipped
+ * chord, the reference's trilinear and TF lookups, opacity
+ * alpha = 1 - exp(-sigma * max(a, 0) * dt), front-to-back compositing.  The
+ * forward keeps transmittance checkpoints every ckpt_every samples; the adjoint
+ * recomputes each segment from them and returns dL/dTF and dL/dvolume. */
+typedef struct {
+  int32_t dims[3];        /* voxels (x, y, z) */
+  int32_t n_steps;        /* samples per ray (512) */
+  int32_t ckpt_every;     /* checkpoint interval (64) */
+  float sigma;            /* extinction scale */
+} hyd_diff_params;
+
+/* Floats per ray of the checkpoint buffer (ceil(n_steps / ckpt_every) + 1). */
+int32_t hyd_diff_ckpt_floats(const hyd_diff_params *p, int32_t *per_ray);
+/* Forward: rgba_dev gets n_views * height * width float4, ckpt_dev
+ * n_views * height * width * hyd_diff_ckpt_floats floats.  volume_dev and
+ * tf_dev (tf_n RGBA entries, <= 256) are device FP32 arrays. */
+int32_t hyd_diff_forward(hyd_ctx *ctx, const hyd_diff_params *p, const hyd_view *views, int32_t n_views,
+                         int32_t width, int32_t height, const float *volume_dev, const float *tf_dev, int32_t tf_n,
+                         float *rgba_dev, float *ckpt_dev, unsigned long long *samples_dev);
+/* Adjoint: adds dL/dvolume into grad_volume_dev and dL/dTF into grad_tf_dev
+ * (both must be zeroed by the caller first) given dL/dRGBA (grad_rgba_dev). */
+int32_t hyd_diff_backward(hyd_ctx *ctx, const hyd_diff_params *p, const hyd_view *views, int32_t n_views,
+                          int32_t width, int32_t height, const float *volume_dev, const float *tf_dev, int32_t tf_n,
+                          const float *ckpt_dev, const float *grad_rgba_dev, float *grad_volume_dev,
+                          float *grad_tf_dev);
+
 #ifdef __cplusplus
 }
 #endif

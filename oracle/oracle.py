@@ -239,6 +239,46 @@ class VolumeScene:
         return np.frombuffer(recs, dtype=np.int32).reshape(max_recs, 10)[:n].copy()
 
 
+class ODiffArgs(C.Structure):
+    _fields_ = [("W", C.c_int), ("H", C.c_int), ("K", C.c_int), ("sigma", C.c_double), ("dims", C.c_int * 3),
+                ("vol", C.POINTER(C.c_float)), ("tf", C.POINTER(C.c_float)), ("tf_n", C.c_int)]
+
+
+class DiffScene:
+    """Differentiable-render oracle: forward RGBA and exact adjoint on listed pixels."""
+
+    def __init__(self, W, H, volume: np.ndarray, tf: np.ndarray, K=512, sigma=8.0):
+        self.vol = np.ascontiguousarray(volume, np.float32)
+        self.tf = np.ascontiguousarray(tf, np.float32).reshape(-1)
+        a = ODiffArgs()
+        a.W, a.H, a.K, a.sigma = W, H, int(K), float(sigma)
+        a.dims[:] = list(self.vol.shape[::-1])
+        a.vol, a.tf, a.tf_n = _pf(self.vol), _pf(self.tf), self.tf.size // 4
+        self.args = a
+        L = lib()
+        if not getattr(L, "_diff_typed", False):
+            L.orc_diff_render.argtypes = [C.c_int, C.c_int, C.POINTER(OView), C.POINTER(ODiffArgs), C.c_int,
+                                          C.POINTER(C.c_int), C.POINTER(C.c_double)]
+            L.orc_diff_grad.argtypes = [C.c_int, C.POINTER(OView), C.POINTER(ODiffArgs), C.c_int, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+            L._diff_typed = True
+
+    def render(self, view: OView, xy: np.ndarray, prec=64, nthreads=0) -> np.ndarray:
+        xy = np.ascontiguousarray(xy, np.int32)
+        out = np.zeros((xy.shape[0], 4))
+        lib().orc_diff_render(prec, nthreads, C.byref(view), C.byref(self.args), xy.shape[0], _pi(xy), _pd(out))
+        return out
+
+    def grad(self, view: OView, xy: np.ndarray, grad_rgba: np.ndarray, prec=64):
+        """(dL/dvolume, dL/dTF) for the listed pixels given dL/dRGBA (n x 4)."""
+        xy = np.ascontiguousarray(xy, np.int32)
+        g = np.ascontiguousarray(grad_rgba, np.float64)
+        gv = np.zeros(self.vol.size)
+        gt = np.zeros(self.tf.size)
+        lib().orc_diff_grad(prec, C.byref(view), C.byref(self.args), xy.shape[0], _pi(xy), _pd(g), _pd(gv), _pd(gt))
+        return gv.reshape(self.vol.shape), gt.reshape(-1, 4)
+
+
 def pixel_pass(rgba: np.ndarray, clear=(0, 0, 0)) -> np.ndarray:
     rgba = np.ascontiguousarray(rgba, np.float32)
     H, W = rgba.shape[:2]
