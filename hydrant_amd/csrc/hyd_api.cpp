@@ -144,6 +144,7 @@ int32_t hyd_destroy(hyd_ctx *c) {
   free_dev(c->own_vols);
   free_dev(c->own_cells);
   free_dev(c->own_diff_views);
+  free_dev(c->own_diff_scratch);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return HYD_OK;

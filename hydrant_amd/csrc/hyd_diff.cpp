@@ -1,5 +1,7 @@
 // C ABI of the differentiable renderer (forward with checkpoints + adjoint).
+#include <cmath>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "hyd_diff.hpp"
@@ -7,6 +9,35 @@
 using namespace hyd;
 
 namespace {
+
+// Inverse of a column-major 4x4 (m[4*col + row]) in double; rows 0-2 of the
+// result, row-major, into out[12].  Returns false for a singular matrix.
+bool inverse_rows(const float *m, float *out) {
+  double a[4][8];
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      a[r][c] = m[4 * c + r];
+      a[r][4 + c] = (r == c) ? 1.0 : 0.0;
+    }
+  for (int c = 0; c < 4; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < 4; ++r)
+      if (std::fabs(a[r][c]) > std::fabs(a[piv][c])) piv = r;
+    if (std::fabs(a[piv][c]) < 1e-300) return false;
+    if (piv != c)
+      for (int k = 0; k < 8; ++k) std::swap(a[c][k], a[piv][k]);
+    const double inv = 1.0 / a[c][c];
+    for (int k = 0; k < 8; ++k) a[c][k] *= inv;
+    for (int r = 0; r < 4; ++r)
+      if (r != c) {
+        const double f = a[r][c];
+        for (int k = 0; k < 8; ++k) a[r][k] -= f * a[c][k];
+      }
+  }
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 4; ++c) out[4 * r + c] = (float)a[r][4 + c];
+  return true;
+}
 
 int32_t nck_of(const hyd_diff_params *p) { return (p->n_steps + p->ckpt_every - 1) / p->ckpt_every + 1; }
 
@@ -40,6 +71,7 @@ int32_t fill(Ctx *c, const hyd_diff_params *p, const hyd_view *views, int32_t n_
     std::memcpy(hv[i].trans, views[i].trans, sizeof(hv[i].trans));
     hv[i].ctg = views[i].ctg_fovy_2;
     std::memcpy(hv[i].ray_o, views[i].ray_o, sizeof(hv[i].ray_o));
+    HYD_REQUIRE(inverse_rows(views[i].trans, hv[i].inv), HYD_ERR_ARG, "view %d has a singular camera matrix", i);
   }
   HYD_CHECK_HIP(hipMemcpyAsync(dviews, hv.data(), hv.size() * sizeof(DiffView), hipMemcpyHostToDevice, c->stream));
   HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
@@ -99,6 +131,32 @@ int32_t hyd_diff_backward(hyd_ctx *c, const hyd_diff_params *p, const hyd_view *
   a.grad_rgba = reinterpret_cast<const float4 *>(grad_rgba_dev);
   a.grad_vol = grad_volume_dev;
   a.grad_tf = grad_tf_dev;
+  // adjoint scratch: per-sample dL/ds (padded 8x8 tiles), per-ray maxima, bound partials, counter
+  const int64_t n_rays = (int64_t)n_views * width * height;
+  HYD_REQUIRE(n_rays < (1ll << 31), HYD_ERR_RANGE, "adjoint launch of %lld rays; the limit is 2^31", (long long)n_rays);
+  const size_t ds_floats = diff_ds_floats(n_views, width, height, a.K);
+  const size_t rays_off = (ds_floats * sizeof(float) + 255) & ~(size_t)255;
+  const size_t blocks_floats = diff_block_floats(a.nx, a.ny, a.nz);
+  const size_t need =
+      rays_off + sizeof(float) * (8 * (size_t)n_rays + blocks_floats + (size_t)n_rays + kDiffBoundParts) + 64;
+  if (c->diff_scratch_cap < need) {
+    if (c->own_diff_scratch) (void)hipFree(c->own_diff_scratch);
+    c->own_diff_scratch = nullptr;
+    c->diff_scratch_cap = 0;
+    HYD_CHECK_HIP(hipMalloc(&c->own_diff_scratch, need));
+    c->diff_scratch_cap = need;
+  }
+  char *base = static_cast<char *>(c->own_diff_scratch);
+  a.ds = reinterpret_cast<float *>(base);
+  a.rays = reinterpret_cast<float4 *>(base + rays_off);
+  a.blocks = reinterpret_cast<float *>(a.rays + 2 * n_rays);
+  a.dsmax = a.blocks + blocks_floats;
+  a.bound = a.dsmax + n_rays;
+  a.tile_counter = reinterpret_cast<int32_t *>(a.bound + kDiffBoundParts);
+  // fixed-point headroom: the int64 accumulators hold 2^33 terms of up to 2^30 each
+  const double terms = 2.0 * (double)n_rays * (double)a.K;
+  a.headroom = terms > 8589934592.0 ? (int32_t)std::ceil(std::log2(terms / 8589934592.0)) : 0;
+  a.n_workers = c->n_cus > 0 ? c->n_cus * 5 : 1280;   // 4-wave workgroups at 5 waves per SIMD
   HYD_CHECK_HIP(launch_diff_backward(a, c->stream));
   return HYD_OK;
 }

@@ -9,6 +9,7 @@ struct DiffView {
   float trans[16];
   float ctg;
   float ray_o[3];
+  float inv[12];               // rows 0-2 of trans^-1 (world -> camera), row-major
 };
 
 struct DiffArgs {
@@ -29,8 +30,28 @@ struct DiffArgs {
   float *grad_vol;
   float *grad_tf;
   unsigned long long *samples;
+  // adjoint scratch
+  float *ds;                   // [view][y][x][K] dL/ds per sample
+  float *dsmax;                // [view][y][x] max |dL/ds| of the ray (0: nothing to scatter)
+  float4 *rays;                // [view][y][x][2] setup of active rays: (dx, dy, dz, tnear), (dt, 0, 0, 0)
+  float *blocks;               // [brick][9^3] per-brick voxel-gradient blocks (brick + high-side halo)
+  float *bound;                // [1] bound of one sample's TF-gradient term
+  int32_t *tile_counter;       // [1] work queue of the ray-order pass
+  int32_t headroom;            // extra right shift of the fixed-point scales for very large launches
+  int32_t n_workers;           // workgroups of the ray-order pass
 };
 
+constexpr int kDiffBoundParts = 256;   // workgroups (partial maxima) of the adjoint's bound prepass
+
+// floats of the per-brick gradient blocks (8^3 bricks with a one-voxel high-side halo)
+inline size_t diff_block_floats(int32_t nx, int32_t ny, int32_t nz) {
+  return (size_t)((nx + 7) / 8) * (size_t)((ny + 7) / 8) * (size_t)((nz + 7) / 8) * 729u;
+}
+
+// floats of the dL/ds buffer ([view][y][x][k])
+inline size_t diff_ds_floats(int32_t n_views, int32_t W, int32_t H, int32_t K) {
+  return (size_t)n_views * (size_t)W * (size_t)H * (size_t)K;
+}
 hipError_t launch_diff_forward(const DiffArgs &a, hipStream_t s);
 hipError_t launch_diff_backward(const DiffArgs &a, hipStream_t s);
 

@@ -88,6 +88,8 @@ struct Ctx {
   // differentiable renderer: device copy of the views
   void *own_diff_views = nullptr;
   int32_t diff_view_cap = 0;
+  void *own_diff_scratch = nullptr;   // adjoint scratch (per-sample dL/ds), grown on demand
+  size_t diff_scratch_cap = 0;
 };
 
 // Flattened kernel arguments for the volume march (passed by value).

@@ -162,68 +162,138 @@ __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs a) {
   }
 }
 
-__device__ __forceinline__ void accumulate(const DiffArgs &a, float *s_gtf, const float4 *s_tf, const RaySetup &r,
-                                           const Smp &sm, const Shade &sh, float Tk, const float g[3], float gA,
-                                           float S[3], float &P) {
-  const float dalpha = Tk * (g[0] * (sh.c[0] - S[0]) + g[1] * (sh.c[1] - S[1]) + g[2] * (sh.c[2] - S[2]) + gA * P);
-  float d[4];
-  const float ta = Tk * sh.alpha;
-  d[0] = ta * g[0];
-  d[1] = ta * g[1];
-  d[2] = ta * g[2];
-  d[3] = sh.a > 0.f ? dalpha * a.sigma * r.dt * (1.f - sh.alpha) : 0.f;
-  if (sh.inside) {
-    const float h = 1.f - sh.g;
-    const float4 p = s_tf[sh.j0], q = s_tf[sh.j1];
-    const float ds = float(a.tf_n) * ((q.x - p.x) * d[0] + (q.y - p.y) * d[1] + (q.z - p.z) * d[2] + (q.w - p.w) * d[3]);
+constexpr int kBoundParts = kDiffBoundParts;
+constexpr int kBrick = 8;          // gradient bricks of the scatter pass
+
+__device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
-    for (int ch = 0; ch < 4; ++ch) {
-      atomicAdd(&s_gtf[4 * sh.j0 + ch], h * d[ch]);
-      atomicAdd(&s_gtf[4 * sh.j1 + ch], sh.g * d[ch]);
-    }
-    if (ds != 0.f) {
-#pragma unroll
-      for (int q2 = 0; q2 < 8; ++q2) unsafeAtomicAdd(&a.grad_vol[sm.idx[q2]], sm.w[q2] * ds);
-    }
-  }
-  S[0] = sh.alpha * sh.c[0] + (1.f - sh.alpha) * S[0];
-  S[1] = sh.alpha * sh.c[1] + (1.f - sh.alpha) * S[1];
-  S[2] = sh.alpha * sh.c[2] + (1.f - sh.alpha) * S[2];
-  P = (1.f - sh.alpha) * P;
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
 }
 
-__global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs a) {
-  __shared__ float4 s_tf[kMaxTf];
-  __shared__ float s_gtf[4 * kMaxTf];
-  for (int i = threadIdx.x; i < a.tf_n; i += 64) s_tf[i] = reinterpret_cast<const float4 *>(a.tf)[i];
-  for (int i = threadIdx.x; i < 4 * a.tf_n; i += 64) s_gtf[i] = 0.f;
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// One fixed-point term: v * 2^e rounded to an int32 (|v| 2^e < 2^30 by the
+// choice of e), sign-extended for the int64 accumulators.
+__device__ __forceinline__ unsigned long long fixed_term(float v, float scale) {
+  return (unsigned long long)(long long)(int)rintf(v * scale);
+}
+
+// Scale 2^e with b 2^e < 2^30 (frexp: b < 2^E), less the headroom, in [2^-100, 2^100].
+__device__ __forceinline__ int fixed_exponent(float b, int headroom) {
+  int E = 0;
+  (void)frexpf(b, &E);
+  return min(100, max(-100, 30 - E - headroom));
+}
+
+// Prepass: partial maxima of the bound b >= |h d_ch| of one TF-gradient term,
+//   |d_ch| = T alpha |g_ch| <= |g_ch|              (colour channels),
+//   |d_3| <= |dalpha| sigma dt, |dalpha| <= 2 max|tf_rgb| |g|_1 + |g_A|
+// (|c|, |S| <= max|tf_rgb|, T, P <= 1, dt <= sqrt(3)/K for the unit box).  Workgroup 0
+// also resets the work queue and clears grad_tf (both outputs are overwritten).
+__global__ __launch_bounds__(256) void diff_bound_kernel(DiffArgs a) {
+  __shared__ float s_red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (blockIdx.x == 0) {
+    if (tid == 0) *a.tile_counter = 0;
+    for (int i = tid; i < 4 * a.tf_n; i += 256) a.grad_tf[i] = 0.f;
+  }
+  float mc = 0.f;
+  for (int i = tid; i < a.tf_n; i += 256) {
+    const float4 t = reinterpret_cast<const float4 *>(a.tf)[i];
+    mc = fmaxf(mc, fmaxf(fabsf(t.x), fmaxf(fabsf(t.y), fabsf(t.z))));
+  }
+  mc = wave_max(mc);
+  if (lane == 0) s_red[wave] = mc;
   __syncthreads();
-  const int view = blockIdx.z;
-  const int x = blockIdx.x * kTile + (threadIdx.x % kTile);
-  const int y = blockIdx.y * kTile + (threadIdx.x / kTile);
-  if (x < a.W && y < a.H) {
-    const size_t ray = ((size_t)view * a.H + y) * a.W + x;
-    const float *ck = a.ckpts + ray * a.nck;
+  mc = fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
+  __syncthreads();
+  const float dtmax = 1.7322f / float(a.K);
+  const long long n_rays = (long long)a.n_views * a.W * a.H;
+  float b = 0.f;
+  for (long long r = (long long)blockIdx.x * 256 + tid; r < n_rays; r += (long long)gridDim.x * 256) {
+    const float4 g = a.grad_rgba[r];
+    const float ginf = fmaxf(fabsf(g.x), fmaxf(fabsf(g.y), fabsf(g.z)));
+    const float g1 = fabsf(g.x) + fabsf(g.y) + fabsf(g.z);
+    b = fmaxf(b, fmaxf(ginf, (2.f * mc * g1 + fabsf(g.w)) * a.sigma * dtmax));
+  }
+  b = wave_max(b);
+  if (lane == 0) s_red[wave] = b;
+  __syncthreads();
+  if (tid == 0) a.bound[blockIdx.x] = fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3])) * 1.0625f;
+}
+
+constexpr int kRun = 8;      // samples per register run (TF terms, scatter corners): 3 bits of headroom
+
+__device__ __forceinline__ void flush_tf(unsigned long long *s_gtf, int j0, int j1, const float *T0, const float *T1,
+                                         float scale) {
+#pragma unroll
+  for (int ch = 0; ch < 4; ++ch) {
+    atomicAdd(&s_gtf[ch * kMaxTf + j0], fixed_term(T0[ch], scale));
+    atomicAdd(&s_gtf[ch * kMaxTf + j1], fixed_term(T1[ch], scale));
+  }
+}
+
+constexpr int kStage = 16;   // samples per ray staged in LDS before a store
+constexpr int kStagePitch = kStage + 1;
+
+// Back-to-front adjoint of one 8x8 ray tile; the wave runs the sample loop in
+// lockstep (inactive rays masked).  The TF table is read from global memory
+// (L1-resident) rather than LDS, so the TF lookups of a sample do not queue
+// behind the previous sample's LDS atomics.  Writes dL/ds of every sample of the
+// active rays (dL/ds layout [view][y][x][k]: staged 16 samples at a time in
+// LDS so each ray's run leaves as one 64 B store) and every ray's max |dL/ds|
+// (0 for inactive rays); adds the TF-gradient terms to the fixed-point LDS
+// accumulators.
+__device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long long *s_gtf, float *stage,
+                             float scale, int view, int tx, int ty, int lane) {
+  const int x = tx * kTile + (lane % kTile), y = ty * kTile + (lane / kTile);
+  const bool in_image = x < a.W && y < a.H;
+  const size_t ray = in_image ? ((size_t)view * a.H + y) * a.W + x : 0;
+  float g[3] = {0.f, 0.f, 0.f}, gA = 0.f;
+  if (in_image) {
     const float4 gr = a.grad_rgba[ray];
-    const float g[3] = {gr.x, gr.y, gr.z};
-    const float gA = gr.w;
-    RaySetup r;
-    if ((g[0] != 0.f || g[1] != 0.f || g[2] != 0.f || gA != 0.f) && setup(a, a.views[view], x, y, r)) {
-      float S[3] = {0.f, 0.f, 0.f}, P = 1.f;
-      const int nseg = a.nck - 1;
-      float Tnext = fabsf(ck[nseg]);
-      for (int seg = nseg - 1; seg >= 0; --seg) {
-        const int k0 = seg * a.ckpt, k1 = min(a.K, k0 + a.ckpt);
-        const float cks = ck[seg];
-        const float Tstart = fabsf(cks);
-        const bool exact = signbit(cks);
-        for (int k = k1 - 1; k >= k0; --k) {
+    g[0] = gr.x; g[1] = gr.y; g[2] = gr.z; gA = gr.w;
+  }
+  RaySetup r{};
+  const bool active =
+      in_image && (g[0] != 0.f || g[1] != 0.f || g[2] != 0.f || gA != 0.f) && setup(a, a.views[view], x, y, r);
+  const unsigned long long amask = __ballot(active);
+  if (active) {   // the scatter pass reuses the ray setup
+    a.rays[2 * ray] = make_float4(r.dx, r.dy, r.dz, r.tnear);
+    a.rays[2 * ray + 1] = make_float4(r.dt, 0.f, 0.f, 0.f);
+  }
+  float dmax = 0.f;
+  // TF-gradient run: terms of consecutive samples on the same pair of TF entries
+  // are summed in registers (at most kRun of them) before they reach LDS
+  int tj0 = -1, tj1 = -1, tn = 0;
+  float T0[4] = {0.f, 0.f, 0.f, 0.f}, T1[4] = {0.f, 0.f, 0.f, 0.f};
+  if (amask != 0ull) {
+    const float *ck = a.ckpts + ray * a.nck;
+    float S[3] = {0.f, 0.f, 0.f}, P = 1.f;
+    const int nseg = a.nck - 1;
+    float Tnext = active ? fabsf(ck[nseg]) : 1.f;
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+      const int k0 = seg * a.ckpt, k1 = min(a.K, k0 + a.ckpt);
+      const float cks = active ? ck[seg] : 1.f;
+      const float Tstart = fabsf(cks);
+      const bool exact = signbit(cks);
+      for (int k = k1 - 1; k >= k0; --k) {
+        float ds = 0.f;
+        if (active) {
           float px, py, pz;
           position(r, k, px, py, pz);
           Smp sm;
-          sample(a, px, py, pz, sm, true);
+          sample(a, px, py, pz, sm, false);
           Shade sh;
-          shade(s_tf, a.tf_n, a.sigma, r.dt, sm.s, sh);
+          shade(tf, a.tf_n, a.sigma, r.dt, sm.s, sh);
           float Tk;
           if (!exact) {
             Tk = Tnext / (1.f - sh.alpha);
@@ -235,20 +305,365 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs a) {
               Smp sj;
               sample(a, qx, qy, qz, sj, false);
               Shade hj;
-              shade(s_tf, a.tf_n, a.sigma, r.dt, sj.s, hj);
+              shade(tf, a.tf_n, a.sigma, r.dt, sj.s, hj);
               Tk = Tk * (1.f - hj.alpha);
             }
           }
-          accumulate(a, s_gtf, s_tf, r, sm, sh, Tk, g, gA, S, P);
+          // dL/dalpha_k = T_k (g . (c_k - S_{k+1}) + gA P_{k+1}),  dL/dc_k = T_k alpha_k g
+          const float dalpha =
+              Tk * (g[0] * (sh.c[0] - S[0]) + g[1] * (sh.c[1] - S[1]) + g[2] * (sh.c[2] - S[2]) + gA * P);
+          const float ta = Tk * sh.alpha;
+          float d[4];
+          d[0] = ta * g[0];
+          d[1] = ta * g[1];
+          d[2] = ta * g[2];
+          d[3] = sh.a > 0.f ? dalpha * a.sigma * r.dt * (1.f - sh.alpha) : 0.f;
+          if (sh.inside) {
+            const float h = 1.f - sh.g;
+            const float4 p = tf[sh.j0], q = tf[sh.j1];
+            ds = float(a.tf_n) * ((q.x - p.x) * d[0] + (q.y - p.y) * d[1] + (q.z - p.z) * d[2] + (q.w - p.w) * d[3]);
+            if (sh.j0 != tj0 || sh.j1 != tj1 || tn == kRun) {
+              if (tn > 0) flush_tf(s_gtf, tj0, tj1, T0, T1, scale);
+              tj0 = sh.j0;
+              tj1 = sh.j1;
+              tn = 0;
+#pragma unroll
+              for (int ch = 0; ch < 4; ++ch) T0[ch] = T1[ch] = 0.f;
+            }
+#pragma unroll
+            for (int ch = 0; ch < 4; ++ch) {
+              T0[ch] += h * d[ch];
+              T1[ch] += sh.g * d[ch];
+            }
+            ++tn;
+          }
+          dmax = fmaxf(dmax, fabsf(ds));
+          S[0] = sh.alpha * sh.c[0] + (1.f - sh.alpha) * S[0];
+          S[1] = sh.alpha * sh.c[1] + (1.f - sh.alpha) * S[1];
+          S[2] = sh.alpha * sh.c[2] + (1.f - sh.alpha) * S[2];
+          P = (1.f - sh.alpha) * P;
           Tnext = Tk;
         }
-        Tnext = Tstart;   // re-anchor at the checkpoint
+        stage[lane * kStagePitch + (k % kStage)] = ds;
+        if (k % kStage == 0) {
+          // samples k .. k+15 of all 64 rays are staged: store 16 rays x 4 at a time,
+          // 16 lanes per ray, one 64 B run each
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          const int kk = lane % kStage, kg = k + kk;
+#pragma unroll 1
+          for (int i = 0; i < 64 / (64 / kStage); ++i) {
+            const int rr = i * (64 / kStage) + lane / kStage;
+            if (kg < a.K && ((amask >> rr) & 1ull)) {
+              const int xr = tx * kTile + (rr % kTile), yr = ty * kTile + (rr / kTile);
+              a.ds[(((size_t)view * a.H + yr) * a.W + xr) * a.K + kg] = stage[rr * kStagePitch + kk];
+            }
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      if (active) Tnext = Tstart;   // re-anchor at the checkpoint
+    }
+    if (tn > 0) flush_tf(s_gtf, tj0, tj1, T0, T1, scale);
+  }
+  if (in_image) a.dsmax[ray] = dmax;
+}
+
+// Ray-order adjoint: persistent workgroups of four waves; each wave claims
+// 8x8 ray tiles from the work queue.  TF gradients: register runs, then int64
+// LDS accumulators with one scale per launch (from the prepass bound), folded
+// once per workgroup into FP32 global atomics.
+__global__ __launch_bounds__(256) void diff_adjoint_kernel(DiffArgs a) {
+  __shared__ unsigned long long s_gtf[4 * kMaxTf];   // [channel][entry]
+  __shared__ float s_b[kBoundParts];
+  __shared__ float s_stage[4][64 * kStagePitch];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int i = tid; i < 4 * kMaxTf; i += 256) s_gtf[i] = 0ull;
+  for (int i = tid; i < kBoundParts; i += 256) s_b[i] = a.bound[i];
+  __syncthreads();
+  float b = 0.f;
+  for (int i = lane; i < kBoundParts; i += 64) b = fmaxf(b, s_b[i]);
+  b = wave_max(b);
+  const int e = fixed_exponent(b, a.headroom + 3);   // a run sums <= kRun = 8 terms
+  const float scale = ldexpf(1.f, e);
+  const int tiles_x = (a.W + kTile - 1) / kTile, tiles_y = (a.H + kTile - 1) / kTile;
+  const int per_view = tiles_x * tiles_y, total = per_view * a.n_views;
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(a.tile_counter, 1);
+    t = __shfl(t, 0);
+    if (t >= total) break;
+    const int view = t / per_view, rem = t - view * per_view;
+    const int ty = rem / tiles_x, tx = rem - ty * tiles_x;
+    adjoint_tile(a, reinterpret_cast<const float4 *>(a.tf), s_gtf, s_stage[wave], scale, view, tx, ty, lane);
+  }
+  __syncthreads();
+  const double unscale = ldexp(1.0, -e);
+  for (int i = tid; i < 4 * kMaxTf; i += 256) {
+    const int ch = i / kMaxTf, j = i - ch * kMaxTf;
+    const long long v = (long long)s_gtf[i];
+    if (j < a.tf_n && v != 0) unsafeAtomicAdd(&a.grad_tf[4 * j + ch], (float)((double)v * unscale));
+  }
+}
+
+// Conservative slab test (the hardware reciprocal's error is far inside the
+// support box's margin).
+__device__ __forceinline__ void slab(float o, float d, float lo, float hi, float &ta, float &tb, bool &hit) {
+  if (d != 0.f) {
+    const float rd = __builtin_amdgcn_rcpf(d);
+    const float t1 = (lo - o) * rd, t2 = (hi - o) * rd;
+    ta = fmaxf(ta, fminf(t1, t2));
+    tb = fminf(tb, fmaxf(t1, t2));
+  } else if (o < lo || o > hi) {
+    hit = false;
+  }
+}
+
+// Scatter of dL/ds into the voxel gradient, one wave per 8^3 brick.  Every
+// sample belongs to the brick that holds its trilinear cell floor(n p - 0.5)
+// (cell -1 of the clamped low border goes to the first brick); its corners lie
+// in the brick or one voxel above it on each axis, so a brick accumulates a
+// kOwn^3 = 9^3 block: its voxels plus a high-side halo.  Per view: the pixels
+// in the projection of the box holding the brick's cells (plus a margin); the
+// rays that reach the box go to an LDS list with their sample range inside it
+// (conservative); the list's samples are split into 64 contiguous runs, one
+// per lane; each candidate sample's position and cell are recomputed with the
+// forward's operations and samples of other bricks' cells dropped.  Each
+// corner term goes to an int64 LDS accumulator (one scale per (brick, view)
+// from the rays' max |dL/ds|), folded per view into the brick's FP32 block in
+// global scratch; diff_combine_kernel then adds each voxel's block entries
+// (own brick and the halos of up to seven lower neighbours) in a fixed order.
+// No float atomics: the voxel gradient is deterministic.
+constexpr int kList = 64;    // rays per scatter list
+constexpr int kOwn = kBrick + 1;
+constexpr int kOwn3 = kOwn * kOwn * kOwn;
+
+struct ScatterList {
+  float dx[kList], dy[kList], dz[kList], tn[kList], dt[kList];
+  int k0[kList], off[kList], ray[kList];
+};
+
+struct BrickBox {
+  int X0, Y0, Z0;        // first voxel
+  int cx0, cy0, cz0;     // first owned cell (-1 on the low border)
+  int cx1, cy1, cz1;     // last owned cell
+};
+
+// Scatter the samples of the first n rays of the list (total samples `total`);
+// lane l takes the contiguous run [l per, (l+1) per) and prefetches the next
+// sample's dL/ds while it works on the current one.
+__device__ void scatter_list(const DiffArgs &a, const DiffView &v, const ScatterList &L, int n, int total,
+                             float scale, unsigned long long *s_acc, int lane, const BrickBox &bb) {
+  const int per = (total + 63) / 64;
+  const int start = lane * per, end = min(total, start + per);
+  int j = 0;   // the last ray whose first sample is <= start (every listed ray has samples)
+  for (int step = kList / 2; step > 0; step >>= 1)
+    if (j + step < n && L.off[j + step] <= start) j += step;
+  RaySetup q;
+  q.ox = v.ray_o[0]; q.oy = v.ray_o[1]; q.oz = v.ray_o[2];
+  q.dx = L.dx[j]; q.dy = L.dy[j]; q.dz = L.dz[j]; q.tnear = L.tn[j]; q.dt = L.dt[j];
+  int next = (j + 1 < n) ? L.off[j + 1] : total;
+  long long base = (long long)L.ray[j] * a.K + L.k0[j] - L.off[j];   // ds index of list sample `it`: base + it
+  int kb = L.k0[j] - L.off[j];                                         // its sample index: kb + it
+  float dsv = start < end ? a.ds[base + start] : 0.f;
+  for (int i = 0; i < per; ++i) {
+    const int it = start + i;
+    if (it < end) {
+      if (it == next) {   // next ray of the list
+        ++j;
+        q.dx = L.dx[j]; q.dy = L.dy[j]; q.dz = L.dz[j]; q.tnear = L.tn[j]; q.dt = L.dt[j];
+        next = (j + 1 < n) ? L.off[j + 1] : total;
+        base = (long long)L.ray[j] * a.K + L.k0[j] - L.off[j];
+        kb = L.k0[j] - L.off[j];
+      }
+      const float cur = dsv;
+      const int k = kb + it;
+      if (it + 1 < end) {
+        const long long nb = (it + 1 == next) ? (long long)L.ray[j + 1] * a.K + L.k0[j + 1] - L.off[j + 1] : base;
+        dsv = a.ds[nb + it + 1];
+      }
+      if (cur != 0.f) {
+        float px, py, pz;
+        position(q, k, px, py, pz);
+        const float fx = float(a.nx) * px - 0.5f, fy = float(a.ny) * py - 0.5f, fz = float(a.nz) * pz - 0.5f;
+        const float flx = floorf(fx), fly = floorf(fy), flz = floorf(fz);
+        const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+        if (ix >= bb.cx0 && ix <= bb.cx1 && iy >= bb.cy0 && iy <= bb.cy1 && iz >= bb.cz0 && iz <= bb.cz1) {
+          const float gx = fx - flx, gy = fy - fly, gz = fz - flz;
+          const float vs = cur * scale;
+          const float wy[2] = {1.f - gy, gy}, wz[2] = {1.f - gz, gz};
+          const int xs[2] = {iclamp(ix, 0, a.nx - 1) - bb.X0, iclamp(ix + 1, 0, a.nx - 1) - bb.X0};
+          const int ys[2] = {(iclamp(iy, 0, a.ny - 1) - bb.Y0) * kOwn, (iclamp(iy + 1, 0, a.ny - 1) - bb.Y0) * kOwn};
+          const int zs[2] = {(iclamp(iz, 0, a.nz - 1) - bb.Z0) * kOwn * kOwn,
+                             (iclamp(iz + 1, 0, a.nz - 1) - bb.Z0) * kOwn * kOwn};
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            // w * (dL/ds * 2^e) as in the forward's weight order: (wx wy) wz, times the scaled dL/ds
+            const float w = (((c & 1) ? gx : 1.f - gx) * wy[(c >> 1) & 1]) * wz[c >> 2];
+            atomicAdd(&s_acc[zs[c >> 2] + ys[(c >> 1) & 1] + xs[c & 1]], fixed_term(w, vs));
+          }
+        }
       }
     }
   }
+}
+
+__global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
+  __shared__ unsigned long long s_acc[kOwn3];
+  __shared__ ScatterList L;
+  const int lane = threadIdx.x;
+  const int nbx = (a.nx + kBrick - 1) / kBrick, nby = (a.ny + kBrick - 1) / kBrick;
+  const int bi = blockIdx.x, bx = bi % nbx, by = (bi / nbx) % nby, bz = bi / (nbx * nby);
+  BrickBox bb;
+  bb.X0 = bx * kBrick; bb.Y0 = by * kBrick; bb.Z0 = bz * kBrick;
+  bb.cx0 = bb.X0 == 0 ? -1 : bb.X0; bb.cy0 = bb.Y0 == 0 ? -1 : bb.Y0; bb.cz0 = bb.Z0 == 0 ? -1 : bb.Z0;
+  bb.cx1 = min(bb.X0 + kBrick - 1, a.nx - 1); bb.cy1 = min(bb.Y0 + kBrick - 1, a.ny - 1);
+  bb.cz1 = min(bb.Z0 + kBrick - 1, a.nz - 1);
+  float *blk = a.blocks + (size_t)bi * kOwn3;
+  for (int i = lane; i < kOwn3; i += 64) {
+    s_acc[i] = 0ull;
+    blk[i] = 0.f;
+  }
+  // the box of the owned cells: floor(n p - 0.5) in [c0, c1]  <=>  p in [(c0 + 0.5) / n, (c1 + 1.5) / n)
+  const float delta = 1e-4f;
+  const float lo[3] = {(float(bb.cx0) + 0.5f) / float(a.nx) - delta, (float(bb.cy0) + 0.5f) / float(a.ny) - delta,
+                       (float(bb.cz0) + 0.5f) / float(a.nz) - delta};
+  const float hi[3] = {(float(bb.cx1) + 1.5f) / float(a.nx) + delta, (float(bb.cy1) + 1.5f) / float(a.ny) + delta,
+                       (float(bb.cz1) + 1.5f) / float(a.nz) + delta};
   __syncthreads();
-  for (int i = threadIdx.x; i < 4 * a.tf_n; i += 64)
-    if (s_gtf[i] != 0.f) unsafeAtomicAdd(&a.grad_tf[i], s_gtf[i]);
+  for (int view = 0; view < a.n_views; ++view) {
+    const DiffView &v = a.views[view];
+    // pixel bounding box of the projected box (whole image if a corner is not in front)
+    int px0 = 0, px1 = a.W - 1, py0 = 0, py1 = a.H - 1;
+    {
+      bool front = true;
+      float umin = 3e38f, umax = -3e38f, vmin = 3e38f, vmax = -3e38f;
+      for (int c = 0; c < 8 && front; ++c) {
+        const float X = (c & 1) ? hi[0] : lo[0], Y = (c & 2) ? hi[1] : lo[1], Z = (c & 4) ? hi[2] : lo[2];
+        const float qx = v.inv[0] * X + v.inv[1] * Y + v.inv[2] * Z + v.inv[3];
+        const float qy = v.inv[4] * X + v.inv[5] * Y + v.inv[6] * Z + v.inv[7];
+        const float qz = v.inv[8] * X + v.inv[9] * Y + v.inv[10] * Z + v.inv[11];
+        if (!(qz < -1e-4f)) {
+          front = false;
+        } else {
+          const float sc = v.ctg / (-qz) * float(a.H) * 0.5f;
+          const float u = float(a.W) * 0.5f + qx * sc, w = float(a.H) * 0.5f - qy * sc;
+          umin = fminf(umin, u); umax = fmaxf(umax, u);
+          vmin = fminf(vmin, w); vmax = fmaxf(vmax, w);
+        }
+      }
+      if (front) {
+        px0 = max(px0, (int)floorf(fmaxf(umin, -4.f)) - 2);
+        px1 = min(px1, (int)ceilf(fminf(umax, float(a.W) + 4.f)) + 2);
+        py0 = max(py0, (int)floorf(fmaxf(vmin, -4.f)) - 2);
+        py1 = min(py1, (int)ceilf(fminf(vmax, float(a.H) + 4.f)) + 2);
+      }
+    }
+    const int bw = px1 - px0 + 1, bh = py1 - py0 + 1;
+    if (bw <= 0 || bh <= 0) continue;
+    const int nR = bw * bh;
+    float B = 0.f;
+    for (int r = lane; r < nR; r += 64) {
+      const int y = py0 + r / bw, x = px0 + r % bw;
+      B = fmaxf(B, a.dsmax[((size_t)view * a.H + y) * a.W + x]);
+    }
+    B = wave_max(B);
+    if (!(B > 0.f)) continue;
+    // terms below 2^30 each; terms per accumulator in this view <= 8 nR K, so the
+    // int64 sums hold while nR K <= 2^29
+    const double items = (double)nR * (double)a.K;
+    const int hv = items > 536870912.0 ? (int)ceil(log2(items / 536870912.0)) : 0;
+    const int e = fixed_exponent(B, hv);
+    const float scale = ldexpf(1.f, e);
+    int n = 0, total = 0;   // rays and samples in the list (wave-uniform)
+    for (int base = 0; base < nR; base += 64) {
+      const int rr = base + lane;
+      int cnt = 0, k0 = 0, ray = 0;
+      RaySetup rs{};
+      if (rr < nR) {
+        const int y = py0 + rr / bw, x = px0 + rr % bw;
+        ray = (view * a.H + y) * a.W + x;
+        if (a.dsmax[ray] > 0.f) {   // an active ray: its setup was stored by the ray-order pass
+          const float4 r0 = a.rays[2 * (size_t)ray], r1 = a.rays[2 * (size_t)ray + 1];
+          rs.ox = v.ray_o[0]; rs.oy = v.ray_o[1]; rs.oz = v.ray_o[2];
+          rs.dx = r0.x; rs.dy = r0.y; rs.dz = r0.z; rs.tnear = r0.w; rs.dt = r1.x;
+          float ta = -3e38f, tb = 3e38f;
+          bool hit = true;
+          slab(rs.ox, rs.dx, lo[0], hi[0], ta, tb, hit);
+          slab(rs.oy, rs.dy, lo[1], hi[1], ta, tb, hit);
+          slab(rs.oz, rs.dz, lo[2], hi[2], ta, tb, hit);
+          if (hit && ta <= tb) {
+            if (rs.dt > 0.f) {
+              const float rdt = __builtin_amdgcn_rcpf(rs.dt);
+              const float fa = (ta - rs.tnear) * rdt - 0.5f, fb = (tb - rs.tnear) * rdt - 0.5f;
+              const float ka = fminf(fmaxf(ceilf(fa) - 1.f, 0.f), float(a.K));
+              const float kb = fminf(fmaxf(floorf(fb) + 1.f, -1.f), float(a.K - 1));
+              k0 = (int)ka;
+              cnt = max(0, (int)kb - k0 + 1);
+            } else {
+              cnt = a.K;
+            }
+          }
+        }
+      }
+      const unsigned long long m = __ballot(cnt > 0);
+      const int n_new = __popcll(m);
+      if (n_new == 0) continue;
+      if (n + n_new > kList) {   // list full: scatter it first
+        __syncthreads();
+        scatter_list(a, v, L, n, total, scale, s_acc, lane, bb);
+        __syncthreads();
+        n = 0;
+        total = 0;
+      }
+      const int incl = wave_incl_scan(cnt, lane);
+      if (cnt > 0) {
+        const int pos = n + __popcll(m & ((1ull << lane) - 1ull));
+        L.dx[pos] = rs.dx; L.dy[pos] = rs.dy; L.dz[pos] = rs.dz; L.tn[pos] = rs.tnear; L.dt[pos] = rs.dt;
+        L.k0[pos] = k0; L.off[pos] = total + incl - cnt; L.ray[pos] = ray;
+      }
+      n += n_new;
+      total += __shfl(incl, 63);
+    }
+    if (n > 0) {
+      __syncthreads();
+      scatter_list(a, v, L, n, total, scale, s_acc, lane, bb);
+    }
+    __syncthreads();
+    const double unscale = ldexp(1.0, -e);
+    for (int i = lane; i < kOwn3; i += 64) {
+      const long long val = (long long)s_acc[i];
+      if (val != 0) {
+        blk[i] += (float)((double)val * unscale);
+        s_acc[i] = 0ull;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// grad_vol[v] = the owning brick's block entry plus the high-side halo entries of
+// the (up to seven) bricks below it on each axis, summed in a fixed order.
+__global__ __launch_bounds__(256) void diff_combine_kernel(DiffArgs a) {
+  const size_t nvox = (size_t)a.nx * a.ny * a.nz;
+  const int nbx = (a.nx + kBrick - 1) / kBrick, nby = (a.ny + kBrick - 1) / kBrick;
+  for (size_t vi = (size_t)blockIdx.x * 256 + threadIdx.x; vi < nvox; vi += (size_t)gridDim.x * 256) {
+    const int X = (int)(vi % a.nx), Y = (int)((vi / a.nx) % a.ny), Z = (int)(vi / ((size_t)a.nx * a.ny));
+    const int bx = X / kBrick, by = Y / kBrick, bz = Z / kBrick;
+    const int lx = X - bx * kBrick, ly = Y - by * kBrick, lz = Z - bz * kBrick;
+    float g = 0.f;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int sx = s & 1, sy = (s >> 1) & 1, sz = s >> 2;
+      // a lower neighbour's halo covers only its first layer above it: local index kBrick
+      if ((sx && (lx != 0 || bx == 0)) || (sy && (ly != 0 || by == 0)) || (sz && (lz != 0 || bz == 0))) continue;
+      const int b = ((bz - sz) * nby + (by - sy)) * nbx + (bx - sx);
+      const int ox = lx + sx * kBrick, oy = ly + sy * kBrick, oz = lz + sz * kBrick;
+      g += a.blocks[(size_t)b * kOwn3 + (oz * kOwn + oy) * kOwn + ox];
+    }
+    a.grad_vol[vi] = g;
+  }
 }
 
 }  // namespace
@@ -261,8 +676,17 @@ hipError_t launch_diff_forward(const DiffArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_diff_backward(const DiffArgs &a, hipStream_t s) {
-  dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile, a.n_views);
-  hipLaunchKernelGGL(diff_backward_kernel, grid, dim3(64), 0, s, a);
+  hipLaunchKernelGGL(diff_bound_kernel, dim3(kBoundParts), dim3(256), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(diff_adjoint_kernel, dim3(a.n_workers), dim3(256), 0, s, a);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const int nb = ((a.nx + kBrick - 1) / kBrick) * ((a.ny + kBrick - 1) / kBrick) * ((a.nz + kBrick - 1) / kBrick);
+  hipLaunchKernelGGL(diff_scatter_kernel, dim3(nb), dim3(64), 0, s, a);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const size_t nvox = (size_t)a.nx * a.ny * a.nz;
+  const int cg = (int)std::min<size_t>((nvox + 255) / 256, 4096);
+  hipLaunchKernelGGL(diff_combine_kernel, dim3(cg), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -46,8 +46,8 @@ class DiffRenderer:
     def backward_raw(self, volume, tf, ck, grad_rgba):
         t = self.ctx.torch
         self.ctx.sync_stream()
-        gv = t.zeros_like(volume)
-        gt = t.zeros_like(tf)
+        gv = t.empty_like(volume)   # both outputs are overwritten by the adjoint
+        gt = t.empty_like(tf)
         grad_rgba = grad_rgba.contiguous()
         L.call("hyd_diff_backward", self.ctx.h, C.byref(self.params), self.views, self.n_views, self.W, self.H,
                C.c_void_p(volume.data_ptr()), C.c_void_p(tf.data_ptr()), tf.shape[0], C.c_void_p(ck.data_ptr()),

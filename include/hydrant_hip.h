@@ -331,8 +331,11 @@ int32_t hyd_diff_ckpt_floats(const hyd_diff_params *p, int32_t *per_ray);
 int32_t hyd_diff_forward(hyd_ctx *ctx, const hyd_diff_params *p, const hyd_view *views, int32_t n_views,
                          int32_t width, int32_t height, const float *volume_dev, const float *tf_dev, int32_t tf_n,
                          float *rgba_dev, float *ckpt_dev, unsigned long long *samples_dev);
-/* Adjoint: adds dL/dvolume into grad_volume_dev and dL/dTF into grad_tf_dev
- * (both must be zeroed by the caller first) given dL/dRGBA (grad_rgba_dev). */
+/* Adjoint: writes dL/dvolume to grad_volume_dev and dL/dTF to grad_tf_dev
+ * (both overwritten) given dL/dRGBA (grad_rgba_dev).  The context keeps a
+ * scratch buffer of n_views * height * width * (n_steps + 1) floats (dL/ds of
+ * every sample), grown on demand; n_views * height * width must be < 2^31.
+ * Voxel gradients are deterministic (integer accumulation per brick). */
 int32_t hyd_diff_backward(hyd_ctx *ctx, const hyd_diff_params *p, const hyd_view *views, int32_t n_views,
                           int32_t width, int32_t height, const float *volume_dev, const float *tf_dev, int32_t tf_n,
                           const float *ckpt_dev, const float *grad_rgba_dev, float *grad_volume_dev,

@@ -77,6 +77,39 @@ def test_adjoint_matches_fp64_gradient(gpu_ctx):
     assert ev <= 1e-3 and et <= 1e-3
 
 
+def test_adjoint_ragged_sizes(gpu_ctx):
+    """Image and volume sizes that are not multiples of the 8x8 ray tiles and 8^3
+    gradient bricks (partial tiles and bricks), gradient on every pixel, two calls in a
+    row (the outputs are overwritten, not accumulated)."""
+    import torch
+
+    wl = W.differentiable(n=44, n_views=2, W=60, H=37)
+    vol_np = wl.hidden_volume[:, :, :41].copy()          # dims (x, y, z) = (41, 44, 44)
+    r = DiffRenderer(gpu_ctx, wl.views()[:2], wl.W, wl.H, vol_np.shape[::-1], wl.n_steps, wl.ckpt_every, wl.sigma)
+    vol = torch.from_numpy(vol_np).cuda()
+    tf = torch.from_numpy(wl.hidden_tf).cuda()
+    rng = np.random.default_rng(3)
+    grad = rng.normal(size=(2, wl.H, wl.W, 4)).astype(np.float32)
+    _, ck, _ = r.forward_raw(vol, tf)
+    gv1, gt1 = r.backward_raw(vol, tf, ck, torch.from_numpy(grad).cuda())
+    gv, gt = r.backward_raw(vol, tf, ck, torch.from_numpy(grad).cuda())
+    torch.cuda.synchronize()
+    assert torch.equal(gv, gv1), "voxel gradients are deterministic"
+    sc = O.DiffScene(wl.W, wl.H, vol_np, wl.hidden_tf, K=wl.n_steps, sigma=wl.sigma)
+    ys, xs = np.mgrid[0:wl.H, 0:wl.W]
+    xy = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.int32)
+    ref_v = np.zeros(vol_np.shape)
+    ref_t = np.zeros((256, 4))
+    for i, v in enumerate(wl.views()[:2]):
+        a, b = sc.grad(_oview(v), xy, grad[i].reshape(-1, 4), prec=64)
+        ref_v += a
+        ref_t += b
+    ev = float(np.max(np.abs(gv.cpu().numpy() - ref_v)) / np.max(np.abs(ref_v)))
+    et = float(np.max(np.abs(gt.cpu().numpy() - ref_t)) / np.max(np.abs(ref_t)))
+    print(f"ragged adjoint vs FP64 gradient: volume rel {ev:.3g}, tf rel {et:.3g}", flush=True)
+    assert ev <= 1e-3 and et <= 1e-3
+
+
 def test_finite_differences_fp64_reductions(gpu_ctx):
     """GPU adjoint of L = sum(w * render) over a 24x24 crop of 2 views against central
     differences (h = 1e-4) of the FP64 forward with FP64 reductions, on 32 TF entries and 32
