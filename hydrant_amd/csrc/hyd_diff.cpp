@@ -46,6 +46,8 @@ int32_t check_params(const hyd_diff_params *p) {
   HYD_REQUIRE(p->dims[0] > 0 && p->dims[1] > 0 && p->dims[2] > 0, HYD_ERR_ARG, "volume dims must be > 0");
   HYD_REQUIRE(p->n_steps > 0 && p->ckpt_every > 0, HYD_ERR_ARG, "n_steps and ckpt_every must be > 0");
   HYD_REQUIRE(p->sigma >= 0.f, HYD_ERR_ARG, "sigma must be >= 0");
+  HYD_REQUIRE((int64_t)p->dims[0] * p->dims[1] * p->dims[2] * 4 <= 2147483647ll, HYD_ERR_RANGE,
+              "volume of %d x %d x %d voxels exceeds 2^31 - 1 bytes", p->dims[0], p->dims[1], p->dims[2]);
   return HYD_OK;
 }
 
@@ -136,7 +138,8 @@ int32_t hyd_diff_backward(hyd_ctx *c, const hyd_diff_params *p, const hyd_view *
   HYD_REQUIRE(n_rays < (1ll << 31), HYD_ERR_RANGE, "adjoint launch of %lld rays; the limit is 2^31", (long long)n_rays);
   const size_t ds_floats = diff_ds_floats(n_views, width, height, a.K);
   const size_t rays_off = (ds_floats * sizeof(float) + 255) & ~(size_t)255;
-  const size_t blocks_floats = diff_block_floats(a.nx, a.ny, a.nz);
+  a.view_groups = diff_view_groups(a.nx, a.ny, a.nz, n_views);
+  const size_t blocks_floats = (size_t)a.view_groups * diff_bricks(a.nx, a.ny, a.nz) * 729u;
   const size_t need =
       rays_off + sizeof(float) * (8 * (size_t)n_rays + blocks_floats + (size_t)n_rays + kDiffBoundParts) + 64;
   if (c->diff_scratch_cap < need) {

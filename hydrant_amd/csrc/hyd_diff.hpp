@@ -34,7 +34,8 @@ struct DiffArgs {
   float *ds;                   // [view][y][x][K] dL/ds per sample
   float *dsmax;                // [view][y][x] max |dL/ds| of the ray (0: nothing to scatter)
   float4 *rays;                // [view][y][x][2] setup of active rays: (dx, dy, dz, tnear), (dt, 0, 0, 0)
-  float *blocks;               // [brick][9^3] per-brick voxel-gradient blocks (brick + high-side halo)
+  float *blocks;               // [view group][brick][9^3] voxel-gradient blocks (brick + high-side halo)
+  int32_t view_groups;         // scatter workgroups per brick (views split into contiguous groups)
   float *bound;                // [1] bound of one sample's TF-gradient term
   int32_t *tile_counter;       // [1] work queue of the ray-order pass
   int32_t headroom;            // extra right shift of the fixed-point scales for very large launches
@@ -43,9 +44,15 @@ struct DiffArgs {
 
 constexpr int kDiffBoundParts = 256;   // workgroups (partial maxima) of the adjoint's bound prepass
 
-// floats of the per-brick gradient blocks (8^3 bricks with a one-voxel high-side halo)
-inline size_t diff_block_floats(int32_t nx, int32_t ny, int32_t nz) {
-  return (size_t)((nx + 7) / 8) * (size_t)((ny + 7) / 8) * (size_t)((nz + 7) / 8) * 729u;
+inline size_t diff_bricks(int32_t nx, int32_t ny, int32_t nz) {
+  return (size_t)((nx + 7) / 8) * (size_t)((ny + 7) / 8) * (size_t)((nz + 7) / 8);
+}
+
+// view groups of the scatter: about 16384 workgroups in all, at most one group per view
+inline int32_t diff_view_groups(int32_t nx, int32_t ny, int32_t nz, int32_t n_views) {
+  const size_t nb = diff_bricks(nx, ny, nz);
+  const size_t g = (16384 + nb - 1) / nb;
+  return (int32_t)(g < 1 ? 1 : (g > (size_t)n_views ? (size_t)n_views : g));
 }
 
 // floats of the dL/ds buffer ([view][y][x][k])

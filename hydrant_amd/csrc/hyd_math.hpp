@@ -20,7 +20,8 @@ struct f4v { float x, y, z, w; };
 HD float gmin(float x, float y) { return (y < x) ? y : x; }
 HD float gmax(float x, float y) { return (x < y) ? y : x; }
 HD float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
-HD int iclamp(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+// Integer clamp for lo <= hi (one v_med3_i32 on the device).
+HD int iclamp(int x, int lo, int hi) { return min(max(x, lo), hi); }
 
 HD f3 add(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 HD f3 sub(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }

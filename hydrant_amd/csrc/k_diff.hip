@@ -30,32 +30,42 @@ constexpr int kMaxTf = 256;
 
 struct Smp {
   float s;
-  int idx[8];
-  float w[8];
 };
 
-__device__ __forceinline__ void sample(const DiffArgs &a, float px, float py, float pz, Smp &sm, bool want_taps) {
+// Volume taps use unsigned 32-bit byte offsets from the (wave-uniform) base, so
+// each load is one global_load with a scalar base and no 64-bit address
+// arithmetic (the volume is limited to 2^31 - 1 bytes).
+struct VolumeRef {
+  const char *base;
+};
+
+__device__ __forceinline__ VolumeRef volume_ref(const DiffArgs &a) {
+  return VolumeRef{reinterpret_cast<const char *>(a.vol)};
+}
+
+__device__ __forceinline__ float vol_at(const VolumeRef &v, int byte_offset) {
+  return *reinterpret_cast<const float *>(v.base + (unsigned)byte_offset);
+}
+
+__device__ __forceinline__ void sample(const DiffArgs &a, const VolumeRef &vr, float px, float py, float pz,
+                                       Smp &sm) {
   const float fx = float(a.nx) * px - 0.5f, fy = float(a.ny) * py - 0.5f, fz = float(a.nz) * pz - 0.5f;
   const float flx = floorf(fx), fly = floorf(fy), flz = floorf(fz);
   const float gx = fx - flx, gy = fy - fly, gz = fz - flz;
   const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-  const int x0 = iclamp(ix, 0, a.nx - 1), x1 = iclamp(ix + 1, 0, a.nx - 1);
-  const int y0 = iclamp(iy, 0, a.ny - 1), y1 = iclamp(iy + 1, 0, a.ny - 1);
-  const int z0 = iclamp(iz, 0, a.nz - 1), z1 = iclamp(iz + 1, 0, a.nz - 1);
-  const int r00 = (z0 * a.ny + y0) * a.nx, r01 = (z0 * a.ny + y1) * a.nx;
-  const int r10 = (z1 * a.ny + y0) * a.nx, r11 = (z1 * a.ny + y1) * a.nx;
+  const int sx = a.nx * 4, sxy = a.nx * a.ny * 4;   // byte strides
+  const int x0 = iclamp(ix, 0, a.nx - 1) * 4, x1 = iclamp(ix + 1, 0, a.nx - 1) * 4;
+  const int y0 = iclamp(iy, 0, a.ny - 1) * sx, y1 = iclamp(iy + 1, 0, a.ny - 1) * sx;
+  const int z0 = iclamp(iz, 0, a.nz - 1) * sxy, z1 = iclamp(iz + 1, 0, a.nz - 1) * sxy;
+  const int r00 = z0 + y0, r01 = z0 + y1, r10 = z1 + y0, r11 = z1 + y1;
   const int id[8] = {r00 + x0, r00 + x1, r01 + x0, r01 + x1, r10 + x0, r10 + x1, r11 + x0, r11 + x1};
   const float hx = 1.f - gx, hy = 1.f - gy, hz = 1.f - gz;
   const float w[8] = {hx * hy * hz, gx * hy * hz, hx * gy * hz, gx * gy * hz,
                       hx * hy * gz, gx * hy * gz, hx * gy * gz, gx * gy * gz};
   float s = 0.f;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) s = fmaf(w[q], a.vol[id[q]], s);
+  for (int q = 0; q < 8; ++q) s = fmaf(w[q], vol_at(vr, id[q]), s);
   sm.s = s;
-  if (want_taps) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) { sm.idx[q] = id[q]; sm.w[q] = w[q]; }
-  }
 }
 
 struct Shade {
@@ -125,6 +135,7 @@ __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs a) {
     const size_t ray = ((size_t)view * a.H + y) * a.W + x;
     float *ck = a.ckpts + ray * a.nck;
     RaySetup r;
+    const VolumeRef vr = volume_ref(a);
     float C0 = 0.f, C1 = 0.f, C2 = 0.f, T = 1.f;
     const int nseg = a.nck - 1;
     if (setup(a, a.views[view], x, y, r)) {
@@ -136,7 +147,7 @@ __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs a) {
           float px, py, pz;
           position(r, k, px, py, pz);
           Smp sm;
-          sample(a, px, py, pz, sm, false);
+          sample(a, vr, px, py, pz, sm);
           Shade sh;
           shade(s_tf, a.tf_n, a.sigma, r.dt, sm.s, sh);
           const float ta = T * sh.alpha;
@@ -263,6 +274,7 @@ __device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long 
     g[0] = gr.x; g[1] = gr.y; g[2] = gr.z; gA = gr.w;
   }
   RaySetup r{};
+  const VolumeRef vr = volume_ref(a);
   const bool active =
       in_image && (g[0] != 0.f || g[1] != 0.f || g[2] != 0.f || gA != 0.f) && setup(a, a.views[view], x, y, r);
   const unsigned long long amask = __ballot(active);
@@ -291,19 +303,19 @@ __device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long 
           float px, py, pz;
           position(r, k, px, py, pz);
           Smp sm;
-          sample(a, px, py, pz, sm, false);
+          sample(a, vr, px, py, pz, sm);
           Shade sh;
           shade(tf, a.tf_n, a.sigma, r.dt, sm.s, sh);
           float Tk;
           if (!exact) {
-            Tk = Tnext / (1.f - sh.alpha);
+            Tk = Tnext * __builtin_amdgcn_rcpf(1.f - sh.alpha);   // alpha <= 0.5 here
           } else {
             Tk = Tstart;   // recompute T_k forward from the checkpoint
             for (int j = k0; j < k; ++j) {
               float qx, qy, qz;
               position(r, j, qx, qy, qz);
               Smp sj;
-              sample(a, qx, qy, qz, sj, false);
+              sample(a, vr, qx, qy, qz, sj);
               Shade hj;
               shade(tf, a.tf_n, a.sigma, r.dt, sj.s, hj);
               Tk = Tk * (1.f - hj.alpha);
@@ -434,6 +446,8 @@ __device__ __forceinline__ void slab(float o, float d, float lo, float hi, float
 // from the rays' max |dL/ds|), folded per view into the brick's FP32 block in
 // global scratch; diff_combine_kernel then adds each voxel's block entries
 // (own brick and the halos of up to seven lower neighbours) in a fixed order.
+// The views are split into groups (one workgroup per brick and group, each
+// with its own block) so that the launch has several waves per SIMD slot.
 // No float atomics: the voxel gradient is deterministic.
 constexpr int kList = 64;    // rays per scatter list
 constexpr int kOwn = kBrick + 1;
@@ -513,14 +527,19 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
   __shared__ unsigned long long s_acc[kOwn3];
   __shared__ ScatterList L;
   const int lane = threadIdx.x;
-  const int nbx = (a.nx + kBrick - 1) / kBrick, nby = (a.ny + kBrick - 1) / kBrick;
-  const int bi = blockIdx.x, bx = bi % nbx, by = (bi / nbx) % nby, bz = bi / (nbx * nby);
+  const int nbx = (a.nx + kBrick - 1) / kBrick, nby = (a.ny + kBrick - 1) / kBrick,
+            nbz = (a.nz + kBrick - 1) / kBrick;
+  const int bi = blockIdx.x % (nbx * nby * nbz), grp = blockIdx.x / (nbx * nby * nbz);
+  const int bx = bi % nbx, by = (bi / nbx) % nby, bz = bi / (nbx * nby);
+  // views of this workgroup: a contiguous share of the launch's views
+  const int v_begin = (int)((long long)a.n_views * grp / a.view_groups);
+  const int v_end = (int)((long long)a.n_views * (grp + 1) / a.view_groups);
   BrickBox bb;
   bb.X0 = bx * kBrick; bb.Y0 = by * kBrick; bb.Z0 = bz * kBrick;
   bb.cx0 = bb.X0 == 0 ? -1 : bb.X0; bb.cy0 = bb.Y0 == 0 ? -1 : bb.Y0; bb.cz0 = bb.Z0 == 0 ? -1 : bb.Z0;
   bb.cx1 = min(bb.X0 + kBrick - 1, a.nx - 1); bb.cy1 = min(bb.Y0 + kBrick - 1, a.ny - 1);
   bb.cz1 = min(bb.Z0 + kBrick - 1, a.nz - 1);
-  float *blk = a.blocks + (size_t)bi * kOwn3;
+  float *blk = a.blocks + ((size_t)grp * nbx * nby * nbz + bi) * kOwn3;
   for (int i = lane; i < kOwn3; i += 64) {
     s_acc[i] = 0ull;
     blk[i] = 0.f;
@@ -532,7 +551,7 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
   const float hi[3] = {(float(bb.cx1) + 1.5f) / float(a.nx) + delta, (float(bb.cy1) + 1.5f) / float(a.ny) + delta,
                        (float(bb.cz1) + 1.5f) / float(a.nz) + delta};
   __syncthreads();
-  for (int view = 0; view < a.n_views; ++view) {
+  for (int view = v_begin; view < v_end; ++view) {
     const DiffView &v = a.views[view];
     // pixel bounding box of the projected box (whole image if a corner is not in front)
     int px0 = 0, px1 = a.W - 1, py0 = 0, py1 = a.H - 1;
@@ -643,24 +662,28 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
   }
 }
 
-// grad_vol[v] = the owning brick's block entry plus the high-side halo entries of
-// the (up to seven) bricks below it on each axis, summed in a fixed order.
+// grad_vol[v] = over the view groups, the owning brick's block entry plus the
+// high-side halo entries of the (up to seven) bricks below it on each axis,
+// summed in a fixed order.
 __global__ __launch_bounds__(256) void diff_combine_kernel(DiffArgs a) {
   const size_t nvox = (size_t)a.nx * a.ny * a.nz;
   const int nbx = (a.nx + kBrick - 1) / kBrick, nby = (a.ny + kBrick - 1) / kBrick;
+  const size_t nb = (size_t)nbx * nby * ((a.nz + kBrick - 1) / kBrick);
   for (size_t vi = (size_t)blockIdx.x * 256 + threadIdx.x; vi < nvox; vi += (size_t)gridDim.x * 256) {
     const int X = (int)(vi % a.nx), Y = (int)((vi / a.nx) % a.ny), Z = (int)(vi / ((size_t)a.nx * a.ny));
     const int bx = X / kBrick, by = Y / kBrick, bz = Z / kBrick;
     const int lx = X - bx * kBrick, ly = Y - by * kBrick, lz = Z - bz * kBrick;
     float g = 0.f;
+    for (int grp = 0; grp < a.view_groups; ++grp) {
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const int sx = s & 1, sy = (s >> 1) & 1, sz = s >> 2;
-      // a lower neighbour's halo covers only its first layer above it: local index kBrick
-      if ((sx && (lx != 0 || bx == 0)) || (sy && (ly != 0 || by == 0)) || (sz && (lz != 0 || bz == 0))) continue;
-      const int b = ((bz - sz) * nby + (by - sy)) * nbx + (bx - sx);
-      const int ox = lx + sx * kBrick, oy = ly + sy * kBrick, oz = lz + sz * kBrick;
-      g += a.blocks[(size_t)b * kOwn3 + (oz * kOwn + oy) * kOwn + ox];
+      for (int s = 0; s < 8; ++s) {
+        const int sx = s & 1, sy = (s >> 1) & 1, sz = s >> 2;
+        // a lower neighbour's halo covers only its first layer above it: local index kBrick
+        if ((sx && (lx != 0 || bx == 0)) || (sy && (ly != 0 || by == 0)) || (sz && (lz != 0 || bz == 0))) continue;
+        const size_t b = grp * nb + ((size_t)(bz - sz) * nby + (by - sy)) * nbx + (bx - sx);
+        const int ox = lx + sx * kBrick, oy = ly + sy * kBrick, oz = lz + sz * kBrick;
+        g += a.blocks[b * kOwn3 + (oz * kOwn + oy) * kOwn + ox];
+      }
     }
     a.grad_vol[vi] = g;
   }
@@ -682,7 +705,7 @@ hipError_t launch_diff_backward(const DiffArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(diff_adjoint_kernel, dim3(a.n_workers), dim3(256), 0, s, a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int nb = ((a.nx + kBrick - 1) / kBrick) * ((a.ny + kBrick - 1) / kBrick) * ((a.nz + kBrick - 1) / kBrick);
-  hipLaunchKernelGGL(diff_scatter_kernel, dim3(nb), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(diff_scatter_kernel, dim3(nb * a.view_groups), dim3(64), 0, s, a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const size_t nvox = (size_t)a.nx * a.ny * a.nz;
   const int cg = (int)std::min<size_t>((nvox + 255) / 256, 4096);
