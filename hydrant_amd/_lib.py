@@ -1,4 +1,4 @@
-"""ctypes binding of libhydrant_hip.so (include/hydrant_hip.h, include/hydrant_dataset.h).
+"""ctypes binding of libhydrant_hip.so (include/hydrant_hip.h, hydrant_dataset.h, hydrant_renderer.h).
 
 The library is built in-tree by ``hydrant_amd.build.build_native`` (or
 ``__graft_entry__.build()``).  Loading fails loudly when it is missing: there is
@@ -90,6 +90,12 @@ class DiffParams(C.Structure):
     _fields_ = [("dims", i3), ("n_steps", I32), ("ckpt_every", I32), ("sigma", C.c_float)]
 
 
+class Dataset(C.Structure):
+    """hyd_dataset (include/hydrant_renderer.h)."""
+    _fields_ = [("dims", C.c_int32 * 3), ("fmt", C.c_int32), ("voxels", C.c_void_p), ("block_size", C.c_int32),
+                ("apron", C.c_int32)]
+
+
 class BatchConfig(C.Structure):
     _fields_ = [("tile_w", I32), ("unroll", I32), ("waves_per_cu", I32)]
 
@@ -151,6 +157,15 @@ SIGNATURES = {
     "hyd_tf_alpha_zero_on": (I32, [PF, I32, C.c_float, C.c_float]),
     "hyd_chebyshev": (I32, [PI32, C.POINTER(C.c_uint8), PI32]),
     "hyd_seeded_page_table": (I32, [PI32, C.POINTER(C.c_uint8), I32, C.c_uint64, PI32, PI32, PI32]),
+    # hydrant_renderer.h
+    "hyd_renderer_candidates": (I32, [C.c_char_p, I32, C.POINTER(I32)]),
+    "hyd_renderer_create": (I32, [P, C.POINTER(Dataset), C.c_char_p, C.POINTER(P)]),
+    "hyd_renderer_update": (I32, [P, C.c_char_p]),
+    "hyd_renderer_resolution": (I32, [P, C.POINTER(I32), C.POINTER(I32)]),
+    "hyd_renderer_offline_render": (I32, [P, C.c_char_p, P]),
+    "hyd_renderer_destroy": (I32, [P]),
+    "hyd_renderer_parse_config": (I32, [C.c_char_p, C.c_char_p, I32, C.POINTER(I32), C.POINTER(I32)]),
+    "hyd_renderer_parse_camera": (I32, [C.c_char_p, C.POINTER(Camera)]),
 }
 
 _lib = None

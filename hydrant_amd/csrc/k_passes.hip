@@ -25,6 +25,25 @@ __global__ __launch_bounds__(kBlock) void pixel_kernel(const float4 *rgba, int n
   rgb[3 * i] = r; rgb[3 * i + 1] = g; rgb[3 * i + 2] = b;
 }
 
+// The same write_to over pixel records of stride_floats floats whose first four
+// are the RGBA value (IsosurfacePixel derives from StdVec4Pixel,
+// isosurface_shader.hpp:11-15).
+__global__ __launch_bounds__(kBlock) void pixel_strided_kernel(const float *v, int stride, int n, uchar3 clear,
+                                                               uint8_t *rgb) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float *p = v + (size_t)i * stride;
+  uint8_t r, g, b;
+  if (p[3] != 0.f) {
+    r = (uint8_t)(int)gclamp(p[0] * 255.f, 0.f, 255.f);
+    g = (uint8_t)(int)gclamp(p[1] * 255.f, 0.f, 255.f);
+    b = (uint8_t)(int)gclamp(p[2] * 255.f, 0.f, 255.f);
+  } else {
+    r = clear.x; g = clear.y; b = clear.z;
+  }
+  rgb[3 * i] = r; rgb[3 * i + 1] = g; rgb[3 * i + 2] = b;
+}
+
 // VolumeShaderKernel::fetch (volume_shader.cu:23-30).
 __global__ __launch_bounds__(kBlock) void fetch_kernel(const hyd_volume_pixel *film, int n, hyd_fetch_pixel *out) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -73,6 +92,14 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(int n, const hyd_fetc
 hipError_t launch_pixel_pass(const float4 *rgba, int W, int H, uchar3 clear, uint8_t *rgb, hipStream_t s) {
   const int n = W * H;
   hipLaunchKernelGGL(pixel_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rgba, n, clear, rgb);
+  return hipGetLastError();
+}
+
+hipError_t launch_pixel_pass_strided(const float *v, int stride_floats, int W, int H, uchar3 clear, uint8_t *rgb,
+                                     hipStream_t s) {
+  const int n = W * H;
+  hipLaunchKernelGGL(pixel_strided_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, v, stride_floats, n,
+                     clear, rgb);
   return hipGetLastError();
 }
 

@@ -1,0 +1,88 @@
+/*
+ * hydrant_renderer.h - the JSON-configured renderer interface over the C ABI.
+ *
+ * Mirrors the reference's renderer plugin API (include/hydrant/core/renderer.hpp:30-89):
+ *   RendererFactory(dataset).create(RendererConfig)  -> hyd_renderer_create
+ *   IRenderer::update(json params)                    -> hyd_renderer_update
+ *   IRenderer::offline_render(Camera) -> RGB8 image    -> hyd_renderer_offline_render
+ *   RendererFactory::list_candidates()                -> hyd_renderer_candidates
+ * with the renderers "Volume", "Blocks" and "Isosurface" registered by name as
+ * REGISTER_RENDERER does (volume.cc:315, blocks.cc:74, isosurface.cc:313).
+ *
+ * Configuration JSON follows the reference schemas:
+ *   RendererConfig  {"resolution": [w, h], "renderer": "Volume", "params": {...}}
+ *                   (core/renderer.schema.hpp:8-13); a whole config file
+ *                   {"camera": ..., "render": RendererConfig} is accepted too.
+ *   params          BasicRendererParams (basic_renderer.schema.hpp:10-16):
+ *                   device, sample_rate (1), max_steps (4000000), clear_color;
+ *                   VolumeRendererParams (ui/volume.schema.hpp:16-21): mode
+ *                   Default|Partition|Paging ("Volume" = Default, as the README
+ *                   writes it), transfer_fn {values: flat or nested RGBA},
+ *                   mem_limit_mb (2048), and density (opacity scale, 1 = the
+ *                   reference, which has none);
+ *                   BlocksRendererParams (ui/blocks.schema.hpp:14-18): mode
+ *                   Volume|Solid, density (1e-2);
+ *                   IsosurfaceRendererParams (ui/isosurface.schema.hpp:16-22): mode
+ *                   Color|Normal|Position, surface_color, isovalue (0.5),
+ *                   mem_limit_mb; plus "corrected" (0 = the reference's crossing
+ *                   test, 1 = bracket + "bisect_steps" bisections).
+ *   camera          CameraConfig (core/scene.schema.hpp:10-35): "orbit" {arm,
+ *                   center} or "ptu" {position, target, up}, "perspective" {fovy}.
+ * device: "Cuda" or "Hip" render on this context's GPU; "Cpu" (the reference's
+ * multi-thread backend) is refused - this library has no CPU path.
+ *
+ * Errors: every call returns HYD_OK (0) or an HYD_ERR_* code and sets
+ * hyd_last_error() (the reference aborts with LOG(FATAL) instead).
+ */
+#ifndef HYDRANT_RENDERER_H
+#define HYDRANT_RENDERER_H
+
+#include <stdint.h>
+
+#include "hydrant_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hyd_renderer hyd_renderer;
+
+/* The dataset: raw voxels in host memory, x fastest (the reference reads a
+ * varch package of decoded bricks, renderer.hpp:17-21 and
+ * lossless_block_paging.cc:137-174; decode is outside this library).  The
+ * renderer copies what it needs during create; the pointer need not outlive
+ * the call. */
+typedef struct {
+  int32_t dims[3];       /* voxels (x, y, z) */
+  int32_t fmt;           /* HYD_FMT_U8 (uint8), HYD_FMT_F16 (half bits), HYD_FMT_F32 */
+  const void *voxels;
+  int32_t block_size;    /* brick edge (the package's block size, e.g. 64) */
+  int32_t apron;         /* padding voxels per side (e.g. 1) */
+} hyd_dataset;
+
+/* Registered renderer names, comma separated, into buf (cap bytes, NUL
+ * terminated); *len receives the full length. */
+int32_t hyd_renderer_candidates(char *buf, int32_t cap, int32_t *len);
+/* RendererFactory::create + IRenderer::init. */
+int32_t hyd_renderer_create(hyd_ctx *ctx, const hyd_dataset *dataset, const char *config_json, hyd_renderer **out);
+/* IRenderer::update with a params object. */
+int32_t hyd_renderer_update(hyd_renderer *r, const char *params_json);
+int32_t hyd_renderer_resolution(const hyd_renderer *r, int32_t *width, int32_t *height);
+/* IRenderer::offline_render: RGB8, width*height*3 bytes into host memory (row 0 first). */
+int32_t hyd_renderer_offline_render(hyd_renderer *r, const char *camera_json, uint8_t *rgb_host);
+int32_t hyd_renderer_destroy(hyd_renderer *r);
+
+/* Parse a renderer configuration without creating a renderer (no GPU use): the
+ * renderer name into name (cap bytes, NUL terminated) and the resolution;
+ * fails on malformed JSON or a missing / unregistered renderer name. */
+int32_t hyd_renderer_parse_config(const char *config_json, char *name, int32_t cap, int32_t *width,
+                                  int32_t *height);
+/* Parse a CameraConfig (or a whole config holding "camera") into a hyd_camera
+ * (no GPU use): orbit via Camera::update_params (scene.hpp:68-81) or ptu. */
+int32_t hyd_renderer_parse_camera(const char *camera_json, hyd_camera *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYDRANT_RENDERER_H */
