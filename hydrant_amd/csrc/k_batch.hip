@@ -31,19 +31,27 @@ namespace {
 constexpr int kMaxTf = 256;
 
 // Raw stored values; the interpolated value is multiplied once by the
-// volume's value_scale (1/255 for bricks that hold 8-bit counts).
+// volume's value_scale (1/255 for bricks that hold 8-bit counts).  Voxels are
+// addressed by a 32-bit byte offset from the (uniform) pool base, so a load is
+// one global_load with a scalar base, and the neighbour offsets of a brick
+// edge known at compile time fold into the load's immediate offset.
 template <int FMT> struct BVox;
 template <> struct BVox<HYD_FMT_F32> {
-  static __device__ __forceinline__ float at(const void *p, uint32_t i) { return static_cast<const float *>(p)[i]; }
+  using T = float;
+  static __device__ __forceinline__ float at(const char *base, uint32_t byte_off, uint32_t d) {
+    return reinterpret_cast<const float *>(base + byte_off)[d];
+  }
 };
 template <> struct BVox<HYD_FMT_F16> {
-  static __device__ __forceinline__ float at(const void *p, uint32_t i) {
-    return __half2float(static_cast<const __half *>(p)[i]);
+  using T = __half;
+  static __device__ __forceinline__ float at(const char *base, uint32_t byte_off, uint32_t d) {
+    return __half2float(reinterpret_cast<const __half *>(base + byte_off)[d]);
   }
 };
 template <> struct BVox<HYD_FMT_U8> {
-  static __device__ __forceinline__ float at(const void *p, uint32_t i) {
-    return float(static_cast<const uint8_t *>(p)[i]);
+  using T = uint8_t;
+  static __device__ __forceinline__ float at(const char *base, uint32_t byte_off, uint32_t d) {
+    return float(reinterpret_cast<const uint8_t *>(base + byte_off)[d]);
   }
 };
 
@@ -62,7 +70,9 @@ struct Tap {
   float ax, ay, az;
 };
 
-__device__ __forceinline__ Tap local_tap(float qx, float qy, float qz, float hi, uint32_t pad, uint32_t base) {
+// The tap base index (flz * pad + fly) * pad + flx is formed in float: exact
+// while pad^3 < 2^24, one conversion instead of three plus two integer products.
+__device__ __forceinline__ Tap local_tap(float qx, float qy, float qz, float hi, float fpad, uint32_t base) {
   const float fx = __builtin_amdgcn_fmed3f(qx, 0.f, hi);
   const float fy = __builtin_amdgcn_fmed3f(qy, 0.f, hi);
   const float fz = __builtin_amdgcn_fmed3f(qz, 0.f, hi);
@@ -71,23 +81,29 @@ __device__ __forceinline__ Tap local_tap(float qx, float qy, float qz, float hi,
   t.ax = fx - flx;
   t.ay = fy - fly;
   t.az = fz - flz;
-  t.off = base + ((uint32_t)flz * pad + (uint32_t)fly) * pad + (uint32_t)flx;
+  t.off = base + (uint32_t)fmaf(fmaf(flz, fpad, fly), fpad, flx);
   return t;
 }
 
 struct Taps8 { float v[8]; };
 
-template <int FMT>
-__device__ __forceinline__ Taps8 fetch(const void *pool, uint32_t off, uint32_t p1, uint32_t p2) {
+// The 8 taps: two z planes, each addressed once; PAD > 0 is the padded brick
+// edge as a compile-time constant (0: runtime pad).
+template <int FMT, int PAD>
+__device__ __forceinline__ Taps8 fetch(const char *pool, uint32_t off, uint32_t pad_rt) {
+  using V = BVox<FMT>;
+  const uint32_t pad = PAD > 0 ? (uint32_t)PAD : pad_rt;
+  const uint32_t b0 = off * (uint32_t)sizeof(typename V::T);
+  const uint32_t b1 = (off + pad * pad) * (uint32_t)sizeof(typename V::T);
   Taps8 r;
-  r.v[0] = BVox<FMT>::at(pool, off);
-  r.v[1] = BVox<FMT>::at(pool, off + 1);
-  r.v[2] = BVox<FMT>::at(pool, off + p1);
-  r.v[3] = BVox<FMT>::at(pool, off + p1 + 1);
-  r.v[4] = BVox<FMT>::at(pool, off + p2);
-  r.v[5] = BVox<FMT>::at(pool, off + p2 + 1);
-  r.v[6] = BVox<FMT>::at(pool, off + p2 + p1);
-  r.v[7] = BVox<FMT>::at(pool, off + p2 + p1 + 1);
+  r.v[0] = V::at(pool, b0, 0);
+  r.v[1] = V::at(pool, b0, 1);
+  r.v[2] = V::at(pool, b0, pad);
+  r.v[3] = V::at(pool, b0, pad + 1);
+  r.v[4] = V::at(pool, b1, 0);
+  r.v[5] = V::at(pool, b1, 1);
+  r.v[6] = V::at(pool, b1, pad);
+  r.v[7] = V::at(pool, b1, pad + 1);
   return r;
 }
 
@@ -140,13 +156,15 @@ __device__ __forceinline__ float box_exit(const RayState &r, float ipx, float ip
   return fminf(fminf(tx, ty), tz);
 }
 
-template <int FMT, int UNROLL, bool COUNT>
+template <int FMT, int UNROLL, bool COUNT, int PAD>
 __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const float4 *ext, RayState &r,
                       unsigned &count) {
   const float step = vol.step, inv_step = 1.f / vol.step;
   const float fn = float(a.tf_n);
-  const uint32_t pad = (uint32_t)a.pad, p2 = pad * pad;
-  const float hi = float(a.pad) - 1.0001f;
+  const uint32_t pad = PAD > 0 ? (uint32_t)PAD : (uint32_t)a.pad;
+  const float fpad = float(pad);
+  const float hi = fpad - 1.0001f;
+  const char *pool = static_cast<const char *>(a.pool);
   const float fbs = float(vol.bs), off = float(vol.apron) - 0.5f;
   while (r.k < r.kend) {
     const float t0 = fmaf(0.5f, fminf(1.f, r.lf - float(r.k)), float(r.k)) * step;
@@ -180,8 +198,8 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const float4 *
         for (int u = 0; u < UNROLL; ++u) {
           wv[u] = fminf(1.f, r.lf - float(k + u));
           const float kf = fmaf(0.5f, wv[u], float(k + u));
-          tp[u] = local_tap(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), hi, pad, base);
-          tv[u] = fetch<FMT>(a.pool, tp[u].off, pad, p2);
+          tp[u] = local_tap(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), hi, fpad, base);
+          tv[u] = fetch<FMT, PAD>(pool, tp[u].off, pad);
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
@@ -195,8 +213,8 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const float4 *
       } else {
         const float w = fminf(1.f, r.lf - float(k));
         const float kf = fmaf(0.5f, w, float(k));
-        const Tap tp = local_tap(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), hi, pad, base);
-        const Taps8 tv = fetch<FMT>(a.pool, tp.off, pad, p2);
+        const Tap tp = local_tap(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), hi, fpad, base);
+        const Taps8 tv = fetch<FMT, PAD>(pool, tp.off, pad);
         const float s = lerp8(tv, tp) * vol.value_scale;
         composite(ext, fn, a.density, s, w, r.v);
         if (COUNT) ++count;
@@ -214,7 +232,7 @@ __device__ __forceinline__ unsigned claim(unsigned *counter) {
   return __shfl(t, 0);
 }
 
-template <int FMT, int TW, int UNROLL, bool COUNT>
+template <int FMT, int TW, int UNROLL, bool COUNT, int PAD>
 __global__ __launch_bounds__(64) void batch_kernel(BatchArgs a) {
   __shared__ float4 s_ext[kMaxTf + 2];
   __shared__ unsigned long long s_count;
@@ -259,7 +277,7 @@ __global__ __launch_bounds__(64) void batch_kernel(BatchArgs a) {
           r.k = 0;
           r.kend = (int)ceilf(r.lf);
           r.cdu = 1.f / comp_max(abs3(d));
-          march<FMT, UNROLL, COUNT>(a, vol, s_ext, r, count);
+          march<FMT, UNROLL, COUNT, PAD>(a, vol, s_ext, r, count);
         }
         a.out[vw.out_offset + (int64_t)y * vw.W + x] = r.v;
       }
@@ -276,12 +294,16 @@ __global__ void reset_counters(unsigned *c, int n) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
 }
 
+// 64^3 bricks with a one-voxel apron (pad 66) get the compile-time brick edge.
 template <int FMT, int TW, int UNROLL>
 hipError_t launch3(const BatchArgs &a, int grid, hipStream_t s) {
-  if (a.samples)
-    hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, true>), dim3(grid), dim3(64), 0, s, a);
-  else
-    hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, false>), dim3(grid), dim3(64), 0, s, a);
+  if (a.pad == 66) {
+    if (a.samples) hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, true, 66>), dim3(grid), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, false, 66>), dim3(grid), dim3(64), 0, s, a);
+  } else {
+    if (a.samples) hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, true, 0>), dim3(grid), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, false, 0>), dim3(grid), dim3(64), 0, s, a);
+  }
   return hipGetLastError();
 }
 
