@@ -29,6 +29,8 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
+
 METRIC = "Msamples/s + ms/frame at 512², plain fp32 vs batched fp16; fwd+adjoint ms"
 HBM_PEAK_GBS = 8000.0
 
@@ -49,6 +51,17 @@ def cpu_model() -> str:
     except OSError:
         pass
     return "unknown"
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter runs
+    (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE in KiB per dispatch, FETCH
+    scaled by the plain-copy calibration factor), or None."""
+    try:
+        rec = json.loads(TRAFFIC_FILE.read_text())[kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return int(rec["fetch_kib"] * 1024 * rec["fetch_calibration"] + rec["write_kib"] * 1024)
 
 
 def time_kernel(torch, fn, steps: int, warmup: int):
@@ -127,8 +140,8 @@ def bench_batched(torch, args, plain_ms_per_sample=None):
     if args.config:
         from hydrant_amd.hip import batch_config
 
-        tw, un, wpc = (int(x) for x in args.config.split(","))
-        best, tune_ms = batch_config(tw, un, wpc), None
+        parts = [int(x) for x in args.config.split(",")]
+        best, tune_ms = batch_config(*parts), None
     else:
         best, tune_ms = b.autotune(wl.density, wl.ert, out, reps=2)
     _, cnt = b.render(best, wl.density, wl.ert, out=out, count=True)
@@ -283,7 +296,7 @@ def main():
                     help="profiling aid: run only this workload's timed launches and print its line")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-form leg")
     ap.add_argument("--diff-fit-steps", type=int, default=200, help="Adam steps of the differentiable fit")
-    ap.add_argument("--config", help="batched launch shape tile_w,unroll,waves_per_cu (skips the autotuner)")
+    ap.add_argument("--config", help="batched launch shape tile_w,unroll,waves_per_cu[,group] (skips the autotuner)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="profiling aid: copy a MIB-sized buffer 3 times (scales FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -366,7 +379,8 @@ def main():
         "batched_fp16": batched_info,
         "diff": diff,
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(kern),
                      "kernel": kern, "kernel_ms": round(head["kernel_ms"], 4),
                      "alg_bytes_per_launch": head["alg_bytes"],
                      "alg_bytes_rule": f"samples x 8 taps x {bytes_per_voxel} B + pixels x 16 B"},

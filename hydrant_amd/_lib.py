@@ -97,10 +97,12 @@ class Dataset(C.Structure):
 
 
 class BatchConfig(C.Structure):
-    _fields_ = [("tile_w", I32), ("unroll", I32), ("waves_per_cu", I32)]
+    _fields_ = [("tile_w", I32), ("unroll", I32), ("waves_per_cu", I32), ("group", I32)]
 
     def as_dict(self):
-        return {"tile_w": self.tile_w, "tile_h": 8, "unroll": self.unroll, "waves_per_cu": self.waves_per_cu}
+        g = max(1, self.group)
+        return {"tile_w": self.tile_w * (2 if g == 4 else 1), "tile_h": 16 if g == 4 else 8, "unroll": self.unroll,
+                "waves_per_cu": self.waves_per_cu, "waves_per_group": g}
 
 # name -> (restype, argtypes); every symbol the public headers declare.
 SIGNATURES = {

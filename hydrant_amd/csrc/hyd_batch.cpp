@@ -18,12 +18,17 @@ int32_t dev_upload(void **dst, const void *src, size_t bytes, hipStream_t s) {
   return HYD_OK;
 }
 
+int group_of(const hyd_batch_config *c) { return c->group == 4 ? 4 : 1; }
+
 bool valid_config(const hyd_batch_config *c) {
   return c && (c->tile_w == 8 || c->tile_w == 16 || c->tile_w == 32) &&
-         (c->unroll == 1 || c->unroll == 2 || c->unroll == 4) && c->waves_per_cu >= 1 && c->waves_per_cu <= 32;
+         (c->unroll == 1 || c->unroll == 2 || c->unroll == 4) && c->waves_per_cu >= 1 && c->waves_per_cu <= 32 &&
+         (c->group == 0 || c->group == 1 || (c->group == 4 && c->tile_w == 8));
 }
 
-int tw_slot(int tw) { return tw == 8 ? 0 : tw == 16 ? 1 : 2; }
+// tile plans: tile_w 8 / 16 / 32 (TW x 8 tiles) and 16 x 16 blocks of four 8x8 tiles
+constexpr int kPlans = 4;
+int plan_slot(int tw, int group) { return group == 4 ? 3 : (tw == 8 ? 0 : tw == 16 ? 1 : 2); }
 
 }  // namespace
 }  // namespace hyd
@@ -33,8 +38,8 @@ using namespace hyd;
 struct hyd_batch {
   std::vector<hyd_batch_view> views;
   BatchViewDev *d_views = nullptr;
-  uint32_t *d_tiles[3] = {nullptr, nullptr, nullptr};   // per tile width 8 / 16 / 32
-  std::vector<int32_t> chunks[3];
+  uint32_t *d_tiles[kPlans] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<int32_t> chunks[kPlans];
   unsigned *d_counters = nullptr;
   int64_t total_px = 0;
   // captured graph
@@ -45,12 +50,13 @@ struct hyd_batch {
 
 namespace {
 
-int32_t ensure_tiles(hyd_batch *b, int tw, hipStream_t s) {
-  const int slot = tw_slot(tw);
+int32_t ensure_tiles(hyd_batch *b, int tw, int group, hipStream_t s) {
+  const int slot = plan_slot(tw, group);
   if (b->d_tiles[slot]) return HYD_OK;
+  const int bw = group == 4 ? 16 : tw, bh = group == 4 ? 16 : 8;
   std::vector<uint32_t> tiles;
   for (size_t v = 0; v < b->views.size(); ++v) {
-    const int nx = (b->views[v].width + tw - 1) / tw, ny = (b->views[v].height + 7) / 8;
+    const int nx = (b->views[v].width + bw - 1) / bw, ny = (b->views[v].height + bh - 1) / bh;
     HYD_REQUIRE(nx <= 256 && ny <= 256, HYD_ERR_RANGE, "view %zu: %d x %d tiles exceeds 256 per axis", v, nx, ny);
     for (int ty = 0; ty < ny; ++ty)
       for (int tx = 0; tx < nx; ++tx) tiles.push_back((uint32_t)v << 16 | (uint32_t)ty << 8 | (uint32_t)tx);
@@ -67,15 +73,17 @@ int32_t ensure_tiles(hyd_batch *b, int tw, hipStream_t s) {
 
 int32_t fill_args(Ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float density, float ert, float *rgba_dev,
                   unsigned long long *samples_dev, BatchArgs &a, int &grid) {
-  HYD_REQUIRE(valid_config(cfg), HYD_ERR_ARG, "invalid batch config (tile_w 8/16/32, unroll 1/2/4, waves_per_cu 1..32)");
+  HYD_REQUIRE(valid_config(cfg), HYD_ERR_ARG,
+              "invalid batch config (tile_w 8/16/32, unroll 1/2/4, waves_per_cu 1..32, group 1 or 4 with tile_w 8)");
   HYD_REQUIRE(rgba_dev, HYD_ERR_ARG, "null output");
   HYD_REQUIRE(c->own_cells && c->n_vols > 0, HYD_ERR_STATE, "no volume set: call hyd_set_volumes first");
   HYD_REQUIRE(c->pool.data && c->tf, HYD_ERR_STATE, "batched render needs a pool and a transfer function");
   HYD_REQUIRE(c->pool.pad == c->vol_pad, HYD_ERR_STATE, "pool pad edge %d differs from the volume set's %d",
               c->pool.pad, c->vol_pad);
-  int32_t st = ensure_tiles(b, cfg->tile_w, c->stream);
+  const int group = group_of(cfg);
+  int32_t st = ensure_tiles(b, cfg->tile_w, group, c->stream);
   if (st) return st;
-  const int slot = tw_slot(cfg->tile_w);
+  const int slot = plan_slot(cfg->tile_w, group);
   std::memset(&a, 0, sizeof(a));
   a.tiles = b->d_tiles[slot];
   for (int i = 0; i <= kXcds; ++i) a.chunk_start[i] = b->chunks[slot][i];
@@ -93,7 +101,7 @@ int32_t fill_args(Ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float densi
   a.max_steps = 4000000;
   a.out = reinterpret_cast<float4 *>(rgba_dev);
   a.samples = samples_dev;
-  grid = c->n_cus * cfg->waves_per_cu;
+  grid = (c->n_cus * cfg->waves_per_cu + group - 1) / group;
   return HYD_OK;
 }
 
@@ -214,7 +222,7 @@ int32_t hyd_batch_render(hyd_ctx *c, hyd_batch *b, const hyd_batch_config *cfg, 
   BatchArgs a;
   int grid = 0;
   if ((st = fill_args(c, b, cfg, density, ert, rgba_dev, samples_dev, a, grid))) return st;
-  HYD_CHECK_HIP(launch_batch(a, c->pool.fmt, cfg->tile_w, cfg->unroll, grid, c->stream));
+  HYD_CHECK_HIP(launch_batch(a, c->pool.fmt, cfg->tile_w, cfg->unroll, group_of(cfg), grid, c->stream));
   return HYD_OK;
 }
 
@@ -231,7 +239,7 @@ int32_t hyd_batch_capture(hyd_ctx *c, hyd_batch *b, const hyd_batch_config *cfg,
   if (!b->capture_stream) HYD_CHECK_HIP(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
   HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
   HYD_CHECK_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = launch_batch(a, c->pool.fmt, cfg->tile_w, cfg->unroll, grid, b->capture_stream);
+  hipError_t e = launch_batch(a, c->pool.fmt, cfg->tile_w, cfg->unroll, group_of(cfg), grid, b->capture_stream);
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(b->capture_stream, &g);
   HYD_CHECK_HIP(e);
@@ -258,28 +266,34 @@ int32_t hyd_batch_autotune(hyd_ctx *c, hyd_batch *b, float density, float ert, f
   HYD_CHECK_HIP(hipEventCreate(&e0));
   HYD_CHECK_HIP(hipEventCreate(&e1));
   float best_t = 1e30f;
-  hyd_batch_config best_c{8, 1, 8};
-  const int tws[3] = {8, 16, 32};
-  const int unrolls[3] = {1, 2, 4};
-  const int wpcs[3] = {8, 16, 32};
-  for (int tw : tws)
-    for (int un : unrolls)
-      for (int wpc : wpcs) {
-        hyd_batch_config cfg{tw, un, wpc};
-        BatchArgs a;
-        int grid = 0;
-        if ((st = fill_args(c, b, &cfg, density, ert, rgba_dev, nullptr, a, grid))) goto done;
-        // one untimed launch (first-use costs), then reps timed launches
-        if (launch_batch(a, c->pool.fmt, tw, un, grid, c->stream) != hipSuccess) { st = HYD_ERR_HIP; goto done; }
-        (void)hipEventRecord(e0, c->stream);
-        for (int r = 0; r < reps; ++r) (void)launch_batch(a, c->pool.fmt, tw, un, grid, c->stream);
-        (void)hipEventRecord(e1, c->stream);
-        if (hipEventSynchronize(e1) != hipSuccess) { st = HYD_ERR_HIP; set_error("autotune launch failed"); goto done; }
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        ms /= (float)reps;
-        if (ms < best_t) { best_t = ms; best_c = cfg; }
-      }
+  hyd_batch_config best_c{8, 4, 8, 1};
+  // the search space: measured optima sit at 2 waves per SIMD and unroll 4;
+  // neighbouring shapes and the 4-wave shared-tile form are timed too
+  std::vector<hyd_batch_config> space;
+  for (int tw : {8, 16, 32})
+    for (int un : {2, 4})
+      for (int wpc : {6, 8, 12}) space.push_back(hyd_batch_config{tw, un, wpc, 1});
+  for (int un : {2, 4})
+    for (int wpc : {8, 12, 16}) space.push_back(hyd_batch_config{8, un, wpc, 4});
+  for (const hyd_batch_config &cfg : space) {
+    BatchArgs a;
+    int grid = 0;
+    if ((st = fill_args(c, b, &cfg, density, ert, rgba_dev, nullptr, a, grid))) goto done;
+    const int g = group_of(&cfg);
+    // one untimed launch (first-use costs), then reps timed launches
+    if (launch_batch(a, c->pool.fmt, cfg.tile_w, cfg.unroll, g, grid, c->stream) != hipSuccess) {
+      st = HYD_ERR_HIP;
+      goto done;
+    }
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; ++r) (void)launch_batch(a, c->pool.fmt, cfg.tile_w, cfg.unroll, g, grid, c->stream);
+    (void)hipEventRecord(e1, c->stream);
+    if (hipEventSynchronize(e1) != hipSuccess) { st = HYD_ERR_HIP; set_error("autotune launch failed"); goto done; }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= (float)reps;
+    if (ms < best_t) { best_t = ms; best_c = cfg; }
+  }
 done:
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

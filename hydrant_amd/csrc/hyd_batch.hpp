@@ -45,6 +45,6 @@ struct BatchArgs {
   unsigned long long *samples;
 };
 
-hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int grid, hipStream_t s);
+hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int group, int grid, hipStream_t s);
 
 }  // namespace hyd

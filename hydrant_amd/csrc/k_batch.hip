@@ -85,29 +85,46 @@ __device__ __forceinline__ Tap local_tap(float qx, float qy, float qz, float hi,
   return t;
 }
 
-struct Taps8 { float v[8]; };
+// Raw tap values: FP16 taps stay half so the x-lerps can consume them in
+// mixed-precision FMAs; u8 and FP32 taps are floats.
+template <int FMT> struct TapVals { float v[8]; };
+template <> struct TapVals<HYD_FMT_F16> { __half v[8]; };
+
+template <int FMT> struct Raw {
+  static __device__ __forceinline__ float at(const char *base, uint32_t byte_off, uint32_t d) {
+    return BVox<FMT>::at(base, byte_off, d);
+  }
+};
+template <> struct Raw<HYD_FMT_F16> {
+  static __device__ __forceinline__ __half at(const char *base, uint32_t byte_off, uint32_t d) {
+    return reinterpret_cast<const __half *>(base + byte_off)[d];
+  }
+};
 
 // The 8 taps: two z planes, each addressed once; PAD > 0 is the padded brick
 // edge as a compile-time constant (0: runtime pad).
 template <int FMT, int PAD>
-__device__ __forceinline__ Taps8 fetch(const char *pool, uint32_t off, uint32_t pad_rt) {
+__device__ __forceinline__ TapVals<FMT> fetch(const char *pool, uint32_t off, uint32_t pad_rt) {
   using V = BVox<FMT>;
   const uint32_t pad = PAD > 0 ? (uint32_t)PAD : pad_rt;
   const uint32_t b0 = off * (uint32_t)sizeof(typename V::T);
   const uint32_t b1 = (off + pad * pad) * (uint32_t)sizeof(typename V::T);
-  Taps8 r;
-  r.v[0] = V::at(pool, b0, 0);
-  r.v[1] = V::at(pool, b0, 1);
-  r.v[2] = V::at(pool, b0, pad);
-  r.v[3] = V::at(pool, b0, pad + 1);
-  r.v[4] = V::at(pool, b1, 0);
-  r.v[5] = V::at(pool, b1, 1);
-  r.v[6] = V::at(pool, b1, pad);
-  r.v[7] = V::at(pool, b1, pad + 1);
+  TapVals<FMT> r;
+  r.v[0] = Raw<FMT>::at(pool, b0, 0);
+  r.v[1] = Raw<FMT>::at(pool, b0, 1);
+  r.v[2] = Raw<FMT>::at(pool, b0, pad);
+  r.v[3] = Raw<FMT>::at(pool, b0, pad + 1);
+  r.v[4] = Raw<FMT>::at(pool, b1, 0);
+  r.v[5] = Raw<FMT>::at(pool, b1, 1);
+  r.v[6] = Raw<FMT>::at(pool, b1, pad);
+  r.v[7] = Raw<FMT>::at(pool, b1, pad + 1);
   return r;
 }
 
-__device__ __forceinline__ float lerp8(const Taps8 &v, const Tap &t) {
+// Trilinear x -> y -> z.  FP16 x-lerps as (v0 - a v0) + a v1: both FMAs take
+// the half taps directly (v_fma_mix_f32), no separate conversions.
+template <int FMT>
+__device__ __forceinline__ float lerp8(const TapVals<FMT> &v, const Tap &t) {
   const float x00 = fmaf(t.ax, v.v[1] - v.v[0], v.v[0]);
   const float x10 = fmaf(t.ax, v.v[3] - v.v[2], v.v[2]);
   const float x01 = fmaf(t.ax, v.v[5] - v.v[4], v.v[4]);
@@ -117,24 +134,47 @@ __device__ __forceinline__ float lerp8(const Taps8 &v, const Tap &t) {
   return fmaf(t.az, y1 - y0, y0);
 }
 
-// TF lookup on the edge-extended table ext[0] = tf[0], ext[1..n] = tf[0..n-1],
-// ext[n+1] = tf[n-1]: fx = n*s - 0.5 in [-0.5, n - 0.5] for s in [0, 1], so
-// floor(fx) + 1 indexes ext without clamping and reproduces the reference's
-// clamp-to-edge; s outside [0, 1] is the Border zero.
-__device__ __forceinline__ void composite(const float4 *ext, float fn, float density, float s, float w,
+template <>
+__device__ __forceinline__ float lerp8<HYD_FMT_F16>(const TapVals<HYD_FMT_F16> &v, const Tap &t) {
+  float x[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const __half2 h = __halves2half2(v.v[2 * i], v.v[2 * i + 1]);
+    const float h0 = __low2float(h), h1 = __high2float(h);
+    x[i] = fmaf(t.ax, h1, fmaf(-t.ax, h0, h0));
+  }
+  const float y0 = fmaf(t.ay, x[1] - x[0], x[0]);
+  const float y1 = fmaf(t.ay, x[3] - x[2], x[2]);
+  return fmaf(t.az, y1 - y0, y0);
+}
+
+// TF lookup on the edge-extended table in slope form: entry i holds
+// p = ext[i] and dq = ext[i+1] - ext[i] with ext[0] = tf[0], ext[1..n] =
+// tf[0..n-1], ext[n+1] = tf[n-1].  fx = n*s - 0.5 lies in [-0.5, n - 0.5] for
+// s in [0, 1], so floor(fx) + 1 indexes the table without clamping and
+// reproduces the reference's clamp-to-edge; p + t*dq is the lerp
+// fma(t, q - p, p).  s outside [0, 1] is the Border zero.  A sample is
+// composited only while the ray is live (alpha 0 leaves v unchanged), so the
+// early-termination test needs no branch per sample.
+struct TfSlope {
+  float4 p, dq;
+};
+
+__device__ __forceinline__ void composite(const TfSlope *ext, float fsc, float smax, float raw, float w, bool live,
                                           float4 &v) {
-  const float fx = fmaf(fn, s, -0.5f);
-  const bool inside = (0.f <= s) & (s <= 1.f);
+  // s = raw * value_scale; fx = n s - 0.5 = (n value_scale) raw - 0.5; inside <=> 0 <= raw <= 1 / value_scale
+  const float fx = fmaf(fsc, raw, -0.5f);
+  const bool inside = (0.f <= raw) & (raw <= smax);
   const float flr = floorf(fx);
   const float t = fx - flr;
   const int i = inside ? (int)flr + 1 : 0;
-  const float4 p = ext[i], q = ext[i + 1];
-  const float a = inside ? fmaf(t, q.w - p.w, p.w) : 0.f;
-  const float alpha = a * density * w;
+  const TfSlope e = ext[i];
+  // the alpha column of the table is premultiplied by the density
+  const float alpha = ((inside & live) ? w : 0.f) * fmaf(t, e.dq.w, e.p.w);
   const float tr = (1.f - v.w) * alpha;
-  v.x = fmaf(fmaf(t, q.x - p.x, p.x), tr, v.x);
-  v.y = fmaf(fmaf(t, q.y - p.y, p.y), tr, v.y);
-  v.z = fmaf(fmaf(t, q.z - p.z, p.z), tr, v.z);
+  v.x = fmaf(fmaf(t, e.dq.x, e.p.x), tr, v.x);
+  v.y = fmaf(fmaf(t, e.dq.y, e.p.y), tr, v.y);
+  v.z = fmaf(fmaf(t, e.dq.z, e.p.z), tr, v.z);
   v.w = fmaf(1.f - v.w, alpha, v.w);
 }
 
@@ -157,10 +197,11 @@ __device__ __forceinline__ float box_exit(const RayState &r, float ipx, float ip
 }
 
 template <int FMT, int UNROLL, bool COUNT, int PAD>
-__device__ void march(const BatchArgs &a, const BatchVolDev &vol, const float4 *ext, RayState &r,
+__device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope *ext, RayState &r,
                       unsigned &count) {
   const float step = vol.step, inv_step = 1.f / vol.step;
-  const float fn = float(a.tf_n);
+  const float fsc = float(a.tf_n) * vol.value_scale, smax = 1.f / vol.value_scale;
+  const float ert = a.ert > 0.f ? a.ert : 3.f;   // opacity never exceeds 1: 3 disables termination
   const uint32_t pad = PAD > 0 ? (uint32_t)PAD : (uint32_t)a.pad;
   const float fpad = float(pad);
   const float hi = fpad - 1.0001f;
@@ -189,38 +230,31 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const float4 *
     const float qdx = r.dx * fbs * step, qdy = r.dy * fbs * step, qdz = r.dz * fbs * step;
     int k = r.k;
     bool done = false;
+    // groups of UNROLL samples with all tap loads in flight together; samples
+    // past kb get weight 0 (they composite nothing, and their taps stay inside
+    // the brick through the clamped local frame)
     while (k < kb) {
-      if (UNROLL > 1 && k + UNROLL <= kb) {
-        Tap tp[UNROLL];
-        Taps8 tv[UNROLL];
-        float wv[UNROLL];
+      const int nvalid = min(UNROLL, kb - k);
+      Tap tp[UNROLL];
+      TapVals<FMT> tv[UNROLL];
+      float wv[UNROLL];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-          wv[u] = fminf(1.f, r.lf - float(k + u));
-          const float kf = fmaf(0.5f, wv[u], float(k + u));
-          tp[u] = local_tap(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), hi, fpad, base);
-          tv[u] = fetch<FMT, PAD>(pool, tp[u].off, pad);
-        }
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-          const float s = lerp8(tv[u], tp[u]) * vol.value_scale;
-          composite(ext, fn, a.density, s, wv[u], r.v);
-          if (COUNT) ++count;
-          ++k;
-          if (a.ert > 0.f && r.v.w > a.ert) { done = true; break; }
-        }
-        if (done) break;
-      } else {
-        const float w = fminf(1.f, r.lf - float(k));
-        const float kf = fmaf(0.5f, w, float(k));
-        const Tap tp = local_tap(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), hi, fpad, base);
-        const Taps8 tv = fetch<FMT, PAD>(pool, tp.off, pad);
-        const float s = lerp8(tv, tp) * vol.value_scale;
-        composite(ext, fn, a.density, s, w, r.v);
-        if (COUNT) ++count;
-        ++k;
-        if (a.ert > 0.f && r.v.w > a.ert) { done = true; break; }
+      for (int u = 0; u < UNROLL; ++u) {
+        const float kf0 = float(k + u);
+        const float w = fminf(1.f, r.lf - kf0);
+        wv[u] = u < nvalid ? w : 0.f;
+        const float kf = fmaf(0.5f, w, kf0);
+        tp[u] = local_tap(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), hi, fpad, base);
+        tv[u] = fetch<FMT, PAD>(pool, tp[u].off, pad);
       }
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        const bool live = r.v.w <= ert;
+        composite(ext, fsc, smax, lerp8<FMT>(tv[u], tp[u]), wv[u], live, r.v);
+        if (COUNT) count += (live && u < nvalid) ? 1u : 0u;
+      }
+      k += nvalid;
+      if (r.v.w > ert) { done = true; break; }
     }
     r.k = done ? r.kend : k;
   }
@@ -232,35 +266,53 @@ __device__ __forceinline__ unsigned claim(unsigned *counter) {
   return __shfl(t, 0);
 }
 
-template <int FMT, int TW, int UNROLL, bool COUNT, int PAD>
-__global__ __launch_bounds__(64) void batch_kernel(BatchArgs a) {
-  __shared__ float4 s_ext[kMaxTf + 2];
+// GROUP 1: single-wave workgroups, each claiming TW x 8 tiles.  GROUP 4: four
+// waves per workgroup claim a 16 x 16 block and take one 8 x 8 tile each, so
+// the four rays packets that touch the same bricks share one CU's L1.
+template <int FMT, int TW, int UNROLL, bool COUNT, int PAD, int GROUP>
+__global__ __launch_bounds__(64 * GROUP) void batch_kernel(BatchArgs a) {
+  extern __shared__ TfSlope s_ext[];   // tf_n + 1 entries (dynamic LDS)
   __shared__ unsigned long long s_count;
+  __shared__ unsigned s_claim;
   const float4 *tf4 = reinterpret_cast<const float4 *>(a.tf);
-  for (int i = threadIdx.x; i < a.tf_n + 2; i += 64) {
-    const int j = i == 0 ? 0 : (i > a.tf_n ? a.tf_n - 1 : i - 1);
-    s_ext[i] = tf4[j];
+  for (int i = threadIdx.x; i < a.tf_n + 1; i += 64 * GROUP) {
+    const int j0 = i == 0 ? 0 : i - 1, j1 = i > a.tf_n - 1 ? a.tf_n - 1 : i;
+    const float4 p = tf4[j0], q = tf4[j1];
+    const float pw = p.w * a.density, qw = q.w * a.density;
+    s_ext[i].p = make_float4(p.x, p.y, p.z, pw);
+    s_ext[i].dq = make_float4(q.x - p.x, q.y - p.y, q.z - p.z, qw - pw);
   }
   if (COUNT && threadIdx.x == 0) s_count = 0;
   __syncthreads();
   unsigned count = 0;
+  const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
   const int home = blockIdx.x % kXcds;
   for (int c = 0; c < kXcds; ++c) {
     const int chunk = (home + c) % kXcds;
     const unsigned base = (unsigned)a.chunk_start[chunk];
     const unsigned len = (unsigned)(a.chunk_start[chunk + 1] - a.chunk_start[chunk]);
     while (true) {
-      const unsigned t = claim(a.counters + chunk);
+      unsigned t;
+      if (GROUP == 1) {
+        t = claim(a.counters + chunk);
+      } else {
+        if (threadIdx.x == 0) s_claim = atomicAdd(a.counters + chunk, 1u);
+        __syncthreads();
+        t = s_claim;
+        __syncthreads();   // every wave has read the claim before the next one is written
+      }
       if (t >= len) break;
       const uint32_t desc = a.tiles[base + t];
       const int view = (int)(desc >> 16);
       const int tx = (int)(desc & 0xffu), ty = (int)((desc >> 8) & 0xffu);
       const BatchViewDev &vw = a.views[view];
       const BatchVolDev &vol = a.vols[vw.volume];
+      const int x0 = GROUP == 1 ? tx * TW : tx * 16 + (wave & 1) * 8;
+      const int y0 = GROUP == 1 ? ty * 8 : ty * 16 + (wave >> 1) * 8;
 #pragma unroll 1
-      for (int sub = 0; sub < TW / 8; ++sub) {
-        const int x = tx * TW + sub * 8 + (int)(threadIdx.x & 7);
-        const int y = ty * 8 + (int)(threadIdx.x >> 3);
+      for (int sub = 0; sub < (GROUP == 1 ? TW / 8 : 1); ++sub) {
+        const int x = x0 + sub * 8 + (lane & 7);
+        const int y = y0 + (lane >> 3);
         if (x >= vw.W || y >= vw.H) continue;
         const f3 ro = f3{vw.ray_o[0], vw.ray_o[1], vw.ray_o[2]};
         const f3 d = pixel_ray_dir(vw.trans, vw.ctg, ro, x, y, vw.W, vw.H);
@@ -295,46 +347,54 @@ __global__ void reset_counters(unsigned *c, int n) {
 }
 
 // 64^3 bricks with a one-voxel apron (pad 66) get the compile-time brick edge.
-template <int FMT, int TW, int UNROLL>
-hipError_t launch3(const BatchArgs &a, int grid, hipStream_t s) {
+template <int FMT, int TW, int UNROLL, int GROUP>
+hipError_t launch4(const BatchArgs &a, int grid, hipStream_t s) {
+  const size_t shmem = (size_t)(a.tf_n + 1) * sizeof(TfSlope);
+  const dim3 blk(64 * GROUP);
   if (a.pad == 66) {
-    if (a.samples) hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, true, 66>), dim3(grid), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, false, 66>), dim3(grid), dim3(64), 0, s, a);
+    if (a.samples) hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, true, 66, GROUP>), dim3(grid), blk, shmem, s, a);
+    else hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, false, 66, GROUP>), dim3(grid), blk, shmem, s, a);
   } else {
-    if (a.samples) hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, true, 0>), dim3(grid), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, false, 0>), dim3(grid), dim3(64), 0, s, a);
+    if (a.samples) hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, true, 0, GROUP>), dim3(grid), blk, shmem, s, a);
+    else hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, false, 0, GROUP>), dim3(grid), blk, shmem, s, a);
   }
   return hipGetLastError();
 }
 
+template <int FMT, int TW, int UNROLL>
+hipError_t launch3(const BatchArgs &a, int group, int grid, hipStream_t s) {
+  if (TW == 8 && group == 4) return launch4<FMT, 8, UNROLL, 4>(a, grid, s);
+  return launch4<FMT, TW, UNROLL, 1>(a, grid, s);
+}
+
 template <int FMT, int TW>
-hipError_t launch2(const BatchArgs &a, int unroll, int grid, hipStream_t s) {
+hipError_t launch2(const BatchArgs &a, int unroll, int group, int grid, hipStream_t s) {
   switch (unroll) {
-    case 4: return launch3<FMT, TW, 4>(a, grid, s);
-    case 2: return launch3<FMT, TW, 2>(a, grid, s);
-    default: return launch3<FMT, TW, 1>(a, grid, s);
+    case 4: return launch3<FMT, TW, 4>(a, group, grid, s);
+    case 2: return launch3<FMT, TW, 2>(a, group, grid, s);
+    default: return launch3<FMT, TW, 1>(a, group, grid, s);
   }
 }
 
 template <int FMT>
-hipError_t launch1(const BatchArgs &a, int tw, int unroll, int grid, hipStream_t s) {
+hipError_t launch1(const BatchArgs &a, int tw, int unroll, int group, int grid, hipStream_t s) {
   switch (tw) {
-    case 32: return launch2<FMT, 32>(a, unroll, grid, s);
-    case 16: return launch2<FMT, 16>(a, unroll, grid, s);
-    default: return launch2<FMT, 8>(a, unroll, grid, s);
+    case 32: return launch2<FMT, 32>(a, unroll, group, grid, s);
+    case 16: return launch2<FMT, 16>(a, unroll, group, grid, s);
+    default: return launch2<FMT, 8>(a, unroll, group, grid, s);
   }
 }
 
 }  // namespace
 
-hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int grid, hipStream_t s) {
+hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int group, int grid, hipStream_t s) {
   hipLaunchKernelGGL(reset_counters, dim3(1), dim3(kXcds), 0, s, a.counters, kXcds);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   switch (fmt) {
-    case HYD_FMT_U8: return launch1<HYD_FMT_U8>(a, tile_w, unroll, grid, s);
-    case HYD_FMT_F16: return launch1<HYD_FMT_F16>(a, tile_w, unroll, grid, s);
-    default: return launch1<HYD_FMT_F32>(a, tile_w, unroll, grid, s);
+    case HYD_FMT_U8: return launch1<HYD_FMT_U8>(a, tile_w, unroll, group, grid, s);
+    case HYD_FMT_F16: return launch1<HYD_FMT_F16>(a, tile_w, unroll, group, grid, s);
+    default: return launch1<HYD_FMT_F32>(a, tile_w, unroll, group, grid, s);
   }
 }
 

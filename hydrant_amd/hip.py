@@ -324,7 +324,7 @@ class Batch:
         return out[off:off + w * h].view(h, w, 4)
 
 
-def batch_config(tile_w=8, unroll=1, waves_per_cu=8) -> L.BatchConfig:
+def batch_config(tile_w=8, unroll=1, waves_per_cu=8, group=1) -> L.BatchConfig:
     c = L.BatchConfig()
-    c.tile_w, c.unroll, c.waves_per_cu = tile_w, unroll, waves_per_cu
+    c.tile_w, c.unroll, c.waves_per_cu, c.group = tile_w, unroll, waves_per_cu, group
     return c

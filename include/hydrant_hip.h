@@ -282,7 +282,9 @@ typedef struct {
 typedef struct {
   int32_t tile_w;            /* ray-packet super-tile: 8, 16 or 32 pixels wide, 8 high */
   int32_t unroll;            /* samples resolved per inner iteration: 1, 2 or 4 */
-  int32_t waves_per_cu;      /* persistent single-wave workgroups per CU: 1..32 */
+  int32_t waves_per_cu;      /* persistent waves per CU: 1..32 */
+  int32_t group;             /* waves per workgroup: 0 or 1, or 4 sharing a 16x16 block of
+                              * 8x8 tiles on one CU (tile_w 8 only) */
 } hyd_batch_config;
 
 typedef struct hyd_batch hyd_batch;

@@ -76,10 +76,11 @@ def test_launch_shape_does_not_change_output(gpu_ctx):
     wl = _small(L.FMT_F16, n_views=4)
     b = _setup(gpu_ctx, wl)
     ref, c0 = b.render(batch_config(8, 1, 8), wl.density, wl.ert, count=True)
-    for tw, un, wpc in [(16, 2, 4), (32, 1, 16), (32, 2, 2), (8, 4, 32), (16, 4, 8)]:
-        out, c1 = b.render(batch_config(tw, un, wpc), wl.density, wl.ert, count=True)
+    for tw, un, wpc, g in [(16, 2, 4, 1), (32, 1, 16, 1), (32, 2, 2, 1), (8, 4, 32, 1), (16, 4, 8, 1),
+                           (8, 4, 8, 4), (8, 2, 12, 4), (8, 1, 3, 4)]:
+        out, c1 = b.render(batch_config(tw, un, wpc, g), wl.density, wl.ert, count=True)
         torch.cuda.synchronize()
-        assert torch.equal(out, ref), (tw, un, wpc)
+        assert torch.equal(out, ref), (tw, un, wpc, g)
         assert int(c1.item()) == int(c0.item())
     best, ms = b.autotune(wl.density, wl.ert, b.new_output(), reps=1)
     assert best.tile_w in (8, 16, 32) and best.unroll in (1, 2, 4) and ms > 0
@@ -93,7 +94,7 @@ def test_graph_replay_matches_eager(gpu_ctx):
 
     wl = _small(L.FMT_F16, n_views=5)
     b = _setup(gpu_ctx, wl)
-    cfg = batch_config(16, 2, 8)
+    cfg = batch_config(8, 4, 8, 4)
     ref, _ = b.render(cfg, wl.density, wl.ert)
     out = b.new_output()
     b.capture(cfg, wl.density, wl.ert, out)
@@ -113,7 +114,8 @@ def test_batch_rejects_bad_config(gpu_ctx):
     wl = _small(L.FMT_F16, n_views=2)
     b = _setup(gpu_ctx, wl)
     out = b.new_output()
-    bad = batch_config(12, 1, 8)
-    code = L.lib().hyd_batch_render(gpu_ctx.h, b.h, C.byref(bad), 0.01, 0.9995, C.c_void_p(out.data_ptr()), None)
-    assert code == L.HYD_ERR_ARG
+    for bad in (batch_config(12, 1, 8), batch_config(16, 1, 8, 4), batch_config(8, 1, 8, 2)):
+        code = L.lib().hyd_batch_render(gpu_ctx.h, b.h, C.byref(bad), 0.01, 0.9995, C.c_void_p(out.data_ptr()),
+                                        None)
+        assert code == L.HYD_ERR_ARG
     b.close()
