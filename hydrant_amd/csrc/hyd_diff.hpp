@@ -48,10 +48,14 @@ inline size_t diff_bricks(int32_t nx, int32_t ny, int32_t nz) {
   return (size_t)((nx + 7) / 8) * (size_t)((ny + 7) / 8) * (size_t)((nz + 7) / 8);
 }
 
-// view groups of the scatter: about 16384 workgroups in all, at most one group per view
+// view groups of the scatter: about kDiffScatterWgs workgroups in all (128 per CU
+// of MI355X, so that the slow bricks' workgroups are spread over a long launch),
+// at most one group per view
+constexpr size_t kDiffScatterWgs = 32768;
+
 inline int32_t diff_view_groups(int32_t nx, int32_t ny, int32_t nz, int32_t n_views) {
   const size_t nb = diff_bricks(nx, ny, nz);
-  const size_t g = (16384 + nb - 1) / nb;
+  const size_t g = (kDiffScatterWgs + nb - 1) / nb;
   return (int32_t)(g < 1 ? 1 : (g > (size_t)n_views ? (size_t)n_views : g));
 }
 

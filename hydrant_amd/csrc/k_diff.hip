@@ -443,8 +443,9 @@ __device__ __forceinline__ void slab(float o, float d, float lo, float hi, float
 // per lane; each candidate sample's position and cell are recomputed with the
 // forward's operations and samples of other bricks' cells dropped.  Each
 // corner term goes to an int64 LDS accumulator (one scale per (brick, view)
-// from the rays' max |dL/ds|), folded per view into the brick's FP32 block in
-// global scratch; diff_combine_kernel then adds each voxel's block entries
+// from the rays' max |dL/ds|), folded per view into FP32 sums held in
+// registers and stored once as the brick's block in global scratch;
+// diff_combine_kernel then adds each voxel's block entries
 // (own brick and the halos of up to seven lower neighbours) in a fixed order.
 // The views are split into groups (one workgroup per brick and group, each
 // with its own block) so that the launch has several waves per SIMD slot.
@@ -452,6 +453,7 @@ __device__ __forceinline__ void slab(float o, float d, float lo, float hi, float
 constexpr int kList = 64;    // rays per scatter list
 constexpr int kOwn = kBrick + 1;
 constexpr int kOwn3 = kOwn * kOwn * kOwn;
+constexpr int kOwnPerLane = (kOwn3 + 63) / 64;
 
 struct ScatterList {
   float dx[kList], dy[kList], dz[kList], tn[kList], dt[kList];
@@ -540,10 +542,10 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
   bb.cx1 = min(bb.X0 + kBrick - 1, a.nx - 1); bb.cy1 = min(bb.Y0 + kBrick - 1, a.ny - 1);
   bb.cz1 = min(bb.Z0 + kBrick - 1, a.nz - 1);
   float *blk = a.blocks + ((size_t)grp * nbx * nby * nbz + bi) * kOwn3;
-  for (int i = lane; i < kOwn3; i += 64) {
-    s_acc[i] = 0ull;
-    blk[i] = 0.f;
-  }
+  for (int i = lane; i < kOwn3; i += 64) s_acc[i] = 0ull;
+  float acc[kOwnPerLane];   // the block's FP32 sums over this group's views; entry lane + 64 i
+#pragma unroll
+  for (int i = 0; i < kOwnPerLane; ++i) acc[i] = 0.f;
   // the box of the owned cells: floor(n p - 0.5) in [c0, c1]  <=>  p in [(c0 + 0.5) / n, (c1 + 1.5) / n)
   const float delta = 1e-4f;
   const float lo[3] = {(float(bb.cx0) + 0.5f) / float(a.nx) - delta, (float(bb.cy0) + 0.5f) / float(a.ny) - delta,
@@ -651,14 +653,23 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
     }
     __syncthreads();
     const double unscale = ldexp(1.0, -e);
-    for (int i = lane; i < kOwn3; i += 64) {
-      const long long val = (long long)s_acc[i];
-      if (val != 0) {
-        blk[i] += (float)((double)val * unscale);
-        s_acc[i] = 0ull;
+#pragma unroll
+    for (int i = 0; i < kOwnPerLane; ++i) {
+      const int idx = lane + 64 * i;
+      if (idx < kOwn3) {
+        const long long val = (long long)s_acc[idx];
+        if (val != 0) {
+          acc[i] += (float)((double)val * unscale);
+          s_acc[idx] = 0ull;
+        }
       }
     }
     __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < kOwnPerLane; ++i) {
+    const int idx = lane + 64 * i;
+    if (idx < kOwn3) blk[idx] = acc[i];
   }
 }
 
