@@ -135,6 +135,7 @@ int32_t hyd_destroy(hyd_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   free_dev(c->own_pool);
+  free_dev(c->own_oriented);
   free_dev(c->own_vaddr);
   free_dev(c->own_page);
   free_dev(c->own_cheb);
@@ -173,6 +174,7 @@ int32_t hyd_upload_pool(hyd_ctx *c, int32_t fmt, int32_t pad_edge, int32_t n_pag
   st = upload(c->own_pool, host, bytes, c->stream);
   if (st) return st;
   c->pool = Pool{c->own_pool, fmt, pad_edge, n_pages, true};
+  ++c->pool_gen;
   return HYD_OK;
 }
 
@@ -183,6 +185,7 @@ int32_t hyd_bind_pool_device(hyd_ctx *c, int32_t fmt, int32_t pad_edge, int32_t 
   HYD_REQUIRE(pad_edge > 0 && pad_edge <= 1024 && n_pages > 0 && pool_dev, HYD_ERR_ARG, "invalid pool");
   free_dev(c->own_pool);
   c->pool = Pool{pool_dev, fmt, pad_edge, n_pages, false};
+  ++c->pool_gen;
   return HYD_OK;
 }
 

@@ -71,6 +71,32 @@ int32_t ensure_tiles(hyd_batch *b, int tw, int group, hipStream_t s) {
   return HYD_OK;
 }
 
+// The pool's y- and z-fastest layouts, made once per pool upload / bind (the
+// stream is synchronised so that graphs captured on another stream see them).
+size_t layout_bytes(const Ctx *c) {
+  const size_t elem = c->pool.fmt == HYD_FMT_U8 ? 1 : c->pool.fmt == HYD_FMT_F16 ? 2 : 4;
+  return (size_t)c->pool.pad * c->pool.pad * c->pool.pad * (size_t)c->pool.n_pages * elem;
+}
+
+int32_t ensure_oriented(Ctx *c) {
+  if (c->own_oriented && c->oriented_gen == c->pool_gen) return HYD_OK;
+  const size_t bytes = layout_bytes(c);
+  if (c->oriented_cap < 2 * bytes) {
+    if (c->own_oriented) (void)hipFree(c->own_oriented);
+    c->own_oriented = nullptr;
+    c->oriented_cap = 0;
+    HYD_REQUIRE(hipMalloc(&c->own_oriented, 2 * bytes) == hipSuccess, HYD_ERR_HIP,
+                "hipMalloc of the oriented pool layouts (%zu bytes) failed", 2 * bytes);
+    c->oriented_cap = 2 * bytes;
+  }
+  HYD_CHECK_HIP(launch_orient_pool(c->pool.fmt, c->pool.data, c->own_oriented,
+                                   static_cast<char *>(c->own_oriented) + bytes, c->pool.pad, c->pool.n_pages,
+                                   c->n_cus, c->stream));
+  HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
+  c->oriented_gen = c->pool_gen;
+  return HYD_OK;
+}
+
 int32_t fill_args(Ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float density, float ert, float *rgba_dev,
                   unsigned long long *samples_dev, BatchArgs &a, int &grid) {
   HYD_REQUIRE(valid_config(cfg), HYD_ERR_ARG,
@@ -83,6 +109,7 @@ int32_t fill_args(Ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float densi
   const int group = group_of(cfg);
   int32_t st = ensure_tiles(b, cfg->tile_w, group, c->stream);
   if (st) return st;
+  if ((st = ensure_oriented(c))) return st;
   const int slot = plan_slot(cfg->tile_w, group);
   std::memset(&a, 0, sizeof(a));
   a.tiles = b->d_tiles[slot];
@@ -91,7 +118,9 @@ int32_t fill_args(Ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float densi
   a.views = b->d_views;
   a.vols = static_cast<const BatchVolDev *>(c->own_vols);
   a.cells = c->own_cells;
-  a.pool = c->pool.data;
+  a.pools[0] = c->pool.data;
+  a.pools[1] = c->own_oriented;
+  a.pools[2] = static_cast<const char *>(c->own_oriented) + layout_bytes(c);
   a.pad = c->pool.pad;
   a.page_elems = (uint32_t)c->pool.pad * c->pool.pad * c->pool.pad;
   a.tf = c->tf;

@@ -33,7 +33,7 @@ struct BatchArgs {
   const BatchViewDev *views;
   const BatchVolDev *vols;
   const int32_t *cells;        // concatenated cell codes of every volume
-  const void *pool;
+  const void *pools[3];         // the pool in its three layouts (x, y, z fastest)
   int32_t pad;
   uint32_t page_elems;         // pad^3
   const float *tf;
@@ -46,5 +46,9 @@ struct BatchArgs {
 };
 
 hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int group, int grid, hipStream_t s);
+
+// Writes the y-fastest (dst1) and z-fastest (dst2) layouts of an x-fastest pool.
+hipError_t launch_orient_pool(int fmt, const void *src, void *dst1, void *dst2, int pad, int n_pages, int n_cus,
+                              hipStream_t s);
 
 }  // namespace hyd

@@ -70,8 +70,12 @@ struct Ctx {
   GridF mean;
   const float *tf = nullptr;
   int32_t tf_n = 0;
+  uint64_t pool_gen = 0;          // bumped by every pool upload / bind
   // owned device allocations (freed at destroy)
   void *own_pool = nullptr;
+  void *own_oriented = nullptr;   // the pool's y- and z-fastest layouts (batched renderer)
+  size_t oriented_cap = 0;
+  uint64_t oriented_gen = ~0ull;  // pool_gen the oriented layouts were made from
   int32_t *own_vaddr = nullptr;
   float4 *own_page = nullptr;
   int32_t *own_cheb = nullptr;

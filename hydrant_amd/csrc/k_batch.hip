@@ -20,7 +20,21 @@
 // from either neighbour: the apron holds the neighbour's voxels, so the value
 // is the same up to rounding.  UNROLL samples are resolved per inner
 // iteration so their tap loads are in flight together; compositing stays
-// strictly in order.  FP16 / u8 / FP32 bricks, FP32 accumulation.
+// strictly in order.  A lane whose sample falls in the same cell as its
+// previous sample (about half of them at half-voxel steps) issues no tap loads
+// and reuses that sample's taps.  FP16 / u8 / FP32 bricks, FP32 accumulation.
+//
+// Pool orientation: the pool is held in three layouts (x, y or z the fastest
+// voxel axis, the other two following cyclically).  The vector-memory unit,
+// not HBM, bounds this kernel, and it serves a gather at full rate only for
+// quads (4 consecutive lanes) whose addresses share a cache line; a quad that
+// spans lines is served one lane at a time.  The four rays of a quad sample
+// points about a pixel apart along one screen direction, so each tile picks
+// the lane order (quads along screen x or along screen y) and the layout whose
+// fastest axis is best aligned with that direction at the tile: the quad's
+// taps then lie along one voxel row.  The eight taps are put back in x-y-z
+// order before the interpolation and a ray's work does not depend on its lane,
+// so every choice gives bit-identical images.
 #include <hip/hip_fp16.h>
 
 #include "hyd_batch.hpp"
@@ -70,8 +84,28 @@ struct Tap {
   float ax, ay, az;
 };
 
-// The tap base index (flz * pad + fly) * pad + flx is formed in float: exact
-// while pad^3 < 2^24, one conversion instead of three plus two integer products.
+// Pool layouts: PERM 0 stores (x, y, z) with x fastest, PERM 1 (y, z, x) with
+// y fastest, PERM 2 (z, x, y) with z fastest; a voxel's index in a page is
+// (a2 * pad + a1) * pad + a0 for the layout's axis order (a0, a1, a2).
+template <int PERM>
+__device__ __forceinline__ float layout_index(float flx, float fly, float flz, float fpad) {
+  if (PERM == 1) return fmaf(fmaf(flx, fpad, flz), fpad, fly);
+  if (PERM == 2) return fmaf(fmaf(fly, fpad, flx), fpad, flz);
+  return fmaf(fmaf(flz, fpad, fly), fpad, flx);
+}
+
+// Position in the fetched taps (bits along a0, a1, a2) of the tap with x-y-z
+// bits c = bx + 2 by + 4 bz.
+template <int PERM>
+__host__ __device__ constexpr int tap_slot(int c) {
+  return PERM == 1 ? ((c >> 1) & 1) + 2 * (c >> 2) + 4 * (c & 1)
+       : PERM == 2 ? (c >> 2) + 2 * (c & 1) + 4 * ((c >> 1) & 1)
+                   : c;
+}
+
+// The tap base index is formed in float: exact while pad^3 < 2^24, one
+// conversion instead of three plus two integer products.
+template <int PERM>
 __device__ __forceinline__ Tap local_tap(float qx, float qy, float qz, float hi, float fpad, uint32_t base) {
   const float fx = __builtin_amdgcn_fmed3f(qx, 0.f, hi);
   const float fy = __builtin_amdgcn_fmed3f(qy, 0.f, hi);
@@ -81,28 +115,22 @@ __device__ __forceinline__ Tap local_tap(float qx, float qy, float qz, float hi,
   t.ax = fx - flx;
   t.ay = fy - fly;
   t.az = fz - flz;
-  t.off = base + (uint32_t)fmaf(fmaf(flz, fpad, fly), fpad, flx);
+  t.off = base + (uint32_t)layout_index<PERM>(flx, fly, flz, fpad);
   return t;
 }
 
-// Raw tap values: FP16 taps stay half so the x-lerps can consume them in
-// mixed-precision FMAs; u8 and FP32 taps are floats.
+// Fetched taps in the layout's order, as loaded: FP16 as four words each
+// holding a pair of neighbours along the fastest axis, u8 as the raw bytes,
+// FP32 as floats.  Values are unpacked / converted only in the interpolation,
+// so a sample that reuses its predecessor's taps selects whole registers.
 template <int FMT> struct TapVals { float v[8]; };
-template <> struct TapVals<HYD_FMT_F16> { __half v[8]; };
+template <> struct TapVals<HYD_FMT_U8> { uint32_t v[8]; };
+template <> struct TapVals<HYD_FMT_F16> { uint32_t v[4]; };
 
-template <int FMT> struct Raw {
-  static __device__ __forceinline__ float at(const char *base, uint32_t byte_off, uint32_t d) {
-    return BVox<FMT>::at(base, byte_off, d);
-  }
-};
-template <> struct Raw<HYD_FMT_F16> {
-  static __device__ __forceinline__ __half at(const char *base, uint32_t byte_off, uint32_t d) {
-    return reinterpret_cast<const __half *>(base + byte_off)[d];
-  }
-};
+template <int FMT> constexpr int tap_regs() { return FMT == HYD_FMT_F16 ? 4 : 8; }
 
-// The 8 taps: two z planes, each addressed once; PAD > 0 is the padded brick
-// edge as a compile-time constant (0: runtime pad).
+// The 8 taps: two planes along the slowest axis, each addressed once; PAD > 0
+// is the padded brick edge as a compile-time constant (0: runtime pad).
 template <int FMT, int PAD>
 __device__ __forceinline__ TapVals<FMT> fetch(const char *pool, uint32_t off, uint32_t pad_rt) {
   using V = BVox<FMT>;
@@ -110,42 +138,71 @@ __device__ __forceinline__ TapVals<FMT> fetch(const char *pool, uint32_t off, ui
   const uint32_t b0 = off * (uint32_t)sizeof(typename V::T);
   const uint32_t b1 = (off + pad * pad) * (uint32_t)sizeof(typename V::T);
   TapVals<FMT> r;
-  r.v[0] = Raw<FMT>::at(pool, b0, 0);
-  r.v[1] = Raw<FMT>::at(pool, b0, 1);
-  r.v[2] = Raw<FMT>::at(pool, b0, pad);
-  r.v[3] = Raw<FMT>::at(pool, b0, pad + 1);
-  r.v[4] = Raw<FMT>::at(pool, b1, 0);
-  r.v[5] = Raw<FMT>::at(pool, b1, 1);
-  r.v[6] = Raw<FMT>::at(pool, b1, pad);
-  r.v[7] = Raw<FMT>::at(pool, b1, pad + 1);
+  if constexpr (FMT == HYD_FMT_F16) {
+    // pairs (a0, a0 + 1) at 2-byte alignment: one unaligned dword each
+    __builtin_memcpy(&r.v[0], pool + b0, 4);
+    __builtin_memcpy(&r.v[1], pool + b0 + 2 * pad, 4);
+    __builtin_memcpy(&r.v[2], pool + b1, 4);
+    __builtin_memcpy(&r.v[3], pool + b1 + 2 * pad, 4);
+  } else if constexpr (FMT == HYD_FMT_U8) {
+    const uint8_t *q0 = reinterpret_cast<const uint8_t *>(pool + b0), *q1 = reinterpret_cast<const uint8_t *>(pool + b1);
+    r.v[0] = q0[0]; r.v[1] = q0[1]; r.v[2] = q0[pad]; r.v[3] = q0[pad + 1];
+    r.v[4] = q1[0]; r.v[5] = q1[1]; r.v[6] = q1[pad]; r.v[7] = q1[pad + 1];
+  } else {
+    const float *q0 = reinterpret_cast<const float *>(pool + b0), *q1 = reinterpret_cast<const float *>(pool + b1);
+    r.v[0] = q0[0]; r.v[1] = q0[1]; r.v[2] = q0[pad]; r.v[3] = q0[pad + 1];
+    r.v[4] = q1[0]; r.v[5] = q1[1]; r.v[6] = q1[pad]; r.v[7] = q1[pad + 1];
+  }
   return r;
 }
 
-// Trilinear x -> y -> z.  FP16 x-lerps as (v0 - a v0) + a v1: both FMAs take
-// the half taps directly (v_fma_mix_f32), no separate conversions.
-template <int FMT>
+// Tap with x-y-z bits c of layout PERM, as a float (u8 / FP32).
+template <int PERM, int FMT>
+__device__ __forceinline__ float tap_f(const TapVals<FMT> &t, int c) {
+  if constexpr (FMT == HYD_FMT_U8) return float(t.v[tap_slot<PERM>(c)]);
+  else return t.v[tap_slot<PERM>(c)];
+}
+
+// FP16 tap with x-y-z bits c of layout PERM (slot s: word s / 2, high half if s is odd).
+template <int PERM>
+__device__ __forceinline__ __half tap_h(const TapVals<HYD_FMT_F16> &t, int c) {
+  const int s = tap_slot<PERM>(c);
+  const uint32_t w = t.v[s >> 1];
+  return __ushort_as_half((unsigned short)((s & 1) ? (w >> 16) : (w & 0xffffu)));
+}
+
+// Trilinear x -> y -> z.
+template <int PERM, int FMT>
 __device__ __forceinline__ float lerp8(const TapVals<FMT> &v, const Tap &t) {
-  const float x00 = fmaf(t.ax, v.v[1] - v.v[0], v.v[0]);
-  const float x10 = fmaf(t.ax, v.v[3] - v.v[2], v.v[2]);
-  const float x01 = fmaf(t.ax, v.v[5] - v.v[4], v.v[4]);
-  const float x11 = fmaf(t.ax, v.v[7] - v.v[6], v.v[6]);
+  const float x00 = fmaf(t.ax, tap_f<PERM>(v, 1) - tap_f<PERM>(v, 0), tap_f<PERM>(v, 0));
+  const float x10 = fmaf(t.ax, tap_f<PERM>(v, 3) - tap_f<PERM>(v, 2), tap_f<PERM>(v, 2));
+  const float x01 = fmaf(t.ax, tap_f<PERM>(v, 5) - tap_f<PERM>(v, 4), tap_f<PERM>(v, 4));
+  const float x11 = fmaf(t.ax, tap_f<PERM>(v, 7) - tap_f<PERM>(v, 6), tap_f<PERM>(v, 6));
   const float y0 = fmaf(t.ay, x10 - x00, x00);
   const float y1 = fmaf(t.ay, x11 - x01, x01);
   return fmaf(t.az, y1 - y0, y0);
 }
 
-template <>
-__device__ __forceinline__ float lerp8<HYD_FMT_F16>(const TapVals<HYD_FMT_F16> &v, const Tap &t) {
+// FP16: x-lerps as (v0 - a v0) + a v1, both FMAs taking the half taps directly
+// (v_fma_mix_f32), no separate conversions.
+template <int PERM>
+__device__ __forceinline__ float lerp8_h(const TapVals<HYD_FMT_F16> &v, const Tap &t) {
   float x[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const __half2 h = __halves2half2(v.v[2 * i], v.v[2 * i + 1]);
+    const __half2 h = __halves2half2(tap_h<PERM>(v, 2 * i), tap_h<PERM>(v, 2 * i + 1));
     const float h0 = __low2float(h), h1 = __high2float(h);
     x[i] = fmaf(t.ax, h1, fmaf(-t.ax, h0, h0));
   }
   const float y0 = fmaf(t.ay, x[1] - x[0], x[0]);
   const float y1 = fmaf(t.ay, x[3] - x[2], x[2]);
   return fmaf(t.az, y1 - y0, y0);
+}
+
+template <int PERM, int FMT>
+__device__ __forceinline__ float interpolate(const TapVals<FMT> &v, const Tap &t) {
+  if constexpr (FMT == HYD_FMT_F16) return lerp8_h<PERM>(v, t);
+  else return lerp8<PERM, FMT>(v, t);
 }
 
 // TF lookup on the edge-extended table in slope form: entry i holds
@@ -196,7 +253,7 @@ __device__ __forceinline__ float box_exit(const RayState &r, float ipx, float ip
   return fminf(fminf(tx, ty), tz);
 }
 
-template <int FMT, int UNROLL, bool COUNT, int PAD>
+template <int FMT, int UNROLL, bool COUNT, int PAD, int PERM>
 __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope *ext, RayState &r,
                       unsigned &count) {
   const float step = vol.step, inv_step = 1.f / vol.step;
@@ -205,8 +262,10 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope 
   const uint32_t pad = PAD > 0 ? (uint32_t)PAD : (uint32_t)a.pad;
   const float fpad = float(pad);
   const float hi = fpad - 1.0001f;
-  const char *pool = static_cast<const char *>(a.pool);
+  const char *pool = static_cast<const char *>(a.pools[PERM]);
   const float fbs = float(vol.bs), off = float(vol.apron) - 0.5f;
+  uint32_t poff = 0xffffffffu;   // tap offset of the last resolved sample (none yet)
+  TapVals<FMT> pv{};             // its taps
   while (r.k < r.kend) {
     const float t0 = fmaf(0.5f, fminf(1.f, r.lf - float(r.k)), float(r.k)) * step;
     const float ipx = floorf(fmaf(r.dx, t0, r.ox));
@@ -236,7 +295,8 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope 
     while (k < kb) {
       const int nvalid = min(UNROLL, kb - k);
       Tap tp[UNROLL];
-      TapVals<FMT> tv[UNROLL];
+      TapVals<FMT> tv[UNROLL];   // fetched taps, layout order
+      bool fresh[UNROLL];        // the sample's cell differs from the previous sample's
       float wv[UNROLL];
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
@@ -244,15 +304,28 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope 
         const float w = fminf(1.f, r.lf - kf0);
         wv[u] = u < nvalid ? w : 0.f;
         const float kf = fmaf(0.5f, w, kf0);
-        tp[u] = local_tap(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), hi, fpad, base);
-        tv[u] = fetch<FMT, PAD>(pool, tp[u].off, pad);
+        tp[u] = local_tap<PERM>(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), hi, fpad, base);
+        fresh[u] = tp[u].off != (u == 0 ? poff : tp[u - 1].off);
+        TapVals<FMT> t{};
+        if (fresh[u]) t = fetch<FMT, PAD>(pool, tp[u].off, pad);
+        tv[u] = t;
       }
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
+        // a sample in the previous sample's cell reuses its taps (same offset, same voxels)
+        // the flag passes through an empty asm so that the compiler cannot fold
+        // this select into the fetch branch (as "copy the previous taps, then
+        // load": the copy would wait for the previous sample's loads)
+        int keep = fresh[u] ? 0 : 1;
+        asm("" : "+v"(keep));
+#pragma unroll
+        for (int c = 0; c < tap_regs<FMT>(); ++c) tv[u].v[c] = keep ? (u == 0 ? pv.v[c] : tv[u - 1].v[c]) : tv[u].v[c];
         const bool live = r.v.w <= ert;
-        composite(ext, fsc, smax, lerp8<FMT>(tv[u], tp[u]), wv[u], live, r.v);
+        composite(ext, fsc, smax, interpolate<PERM, FMT>(tv[u], tp[u]), wv[u], live, r.v);
         if (COUNT) count += (live && u < nvalid) ? 1u : 0u;
       }
+      poff = tp[UNROLL - 1].off;
+      pv = tv[UNROLL - 1];
       k += nvalid;
       if (r.v.w > ert) { done = true; break; }
     }
@@ -309,12 +382,29 @@ __global__ __launch_bounds__(64 * GROUP) void batch_kernel(BatchArgs a) {
       const BatchVolDev &vol = a.vols[vw.volume];
       const int x0 = GROUP == 1 ? tx * TW : tx * 16 + (wave & 1) * 8;
       const int y0 = GROUP == 1 ? ty * 8 : ty * 16 + (wave >> 1) * 8;
+      const f3 ro = f3{vw.ray_o[0], vw.ray_o[1], vw.ray_o[2]};
+      // lane order and layout: the screen direction (x or y) and volume axis of
+      // best alignment, from the ray directions one pixel apart at the tile centre
+      int perm, cols;
+      {
+        const int cx = x0 + (GROUP == 1 ? TW / 2 : 4), cy = y0 + 4;
+        const f3 dc = pixel_ray_dir(vw.trans, vw.ctg, ro, cx, cy, vw.W, vw.H);
+        const f3 dx = sub(pixel_ray_dir(vw.trans, vw.ctg, ro, cx + 1, cy, vw.W, vw.H), dc);
+        const f3 dy = sub(pixel_ray_dir(vw.trans, vw.ctg, ro, cx, cy + 1, vw.W, vw.H), dc);
+        const float nx = comp_max(abs3(dx)) * rsqrtf(dot3(dx, dx) + 1e-30f);
+        const float ny = comp_max(abs3(dy)) * rsqrtf(dot3(dy, dy) + 1e-30f);
+        cols = ny > nx ? 1 : 0;
+        const f3 m = abs3(cols ? dy : dx);
+        perm = (m.x >= m.y && m.x >= m.z) ? 0 : (m.y >= m.z ? 1 : 2);
+        perm = __builtin_amdgcn_readfirstlane(perm);
+        cols = __builtin_amdgcn_readfirstlane(cols);
+      }
+      const int lx = cols ? (lane >> 3) : (lane & 7), ly = cols ? (lane & 7) : (lane >> 3);
 #pragma unroll 1
       for (int sub = 0; sub < (GROUP == 1 ? TW / 8 : 1); ++sub) {
-        const int x = x0 + sub * 8 + (lane & 7);
-        const int y = y0 + (lane >> 3);
+        const int x = x0 + sub * 8 + lx;
+        const int y = y0 + ly;
         if (x >= vw.W || y >= vw.H) continue;
-        const f3 ro = f3{vw.ray_o[0], vw.ray_o[1], vw.ray_o[2]};
         const f3 d = pixel_ray_dir(vw.trans, vw.ctg, ro, x, y, vw.W, vw.H);
         RayState r;
         r.v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -329,7 +419,9 @@ __global__ __launch_bounds__(64 * GROUP) void batch_kernel(BatchArgs a) {
           r.k = 0;
           r.kend = (int)ceilf(r.lf);
           r.cdu = 1.f / comp_max(abs3(d));
-          march<FMT, UNROLL, COUNT, PAD>(a, vol, s_ext, r, count);
+          if (perm == 0) march<FMT, UNROLL, COUNT, PAD, 0>(a, vol, s_ext, r, count);
+          else if (perm == 1) march<FMT, UNROLL, COUNT, PAD, 1>(a, vol, s_ext, r, count);
+          else march<FMT, UNROLL, COUNT, PAD, 2>(a, vol, s_ext, r, count);
         }
         a.out[vw.out_offset + (int64_t)y * vw.W + x] = r.v;
       }
@@ -339,6 +431,20 @@ __global__ __launch_bounds__(64 * GROUP) void batch_kernel(BatchArgs a) {
     if (count) atomicAdd(&s_count, (unsigned long long)count);
     __syncthreads();
     if (threadIdx.x == 0 && s_count) atomicAdd(a.samples, s_count);
+  }
+}
+
+// The two other layouts of a pool (PERM 1 and 2) from the x-fastest one.
+template <typename T>
+__global__ __launch_bounds__(256) void orient_pool_kernel(const T *src, T *dst1, T *dst2, int pad, size_t n) {
+  const size_t p3 = (size_t)pad * pad * pad;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const size_t page = i / p3;
+    const int r = (int)(i - page * p3);
+    const int x = r % pad, y = (r / pad) % pad, z = r / (pad * pad);
+    const T v = src[i];
+    dst1[page * p3 + ((size_t)x * pad + z) * pad + y] = v;
+    dst2[page * p3 + ((size_t)y * pad + x) * pad + z] = v;
   }
 }
 
@@ -386,6 +492,27 @@ hipError_t launch1(const BatchArgs &a, int tw, int unroll, int group, int grid, 
 }
 
 }  // namespace
+
+hipError_t launch_orient_pool(int fmt, const void *src, void *dst1, void *dst2, int pad, int n_pages, int n_cus,
+                              hipStream_t s) {
+  const size_t n = (size_t)pad * pad * pad * (size_t)n_pages;
+  const dim3 grid(n_cus * 8), blk(256);
+  switch (fmt) {
+    case HYD_FMT_U8:
+      hipLaunchKernelGGL(orient_pool_kernel<uint8_t>, grid, blk, 0, s, static_cast<const uint8_t *>(src),
+                         static_cast<uint8_t *>(dst1), static_cast<uint8_t *>(dst2), pad, n);
+      break;
+    case HYD_FMT_F16:
+      hipLaunchKernelGGL(orient_pool_kernel<uint16_t>, grid, blk, 0, s, static_cast<const uint16_t *>(src),
+                         static_cast<uint16_t *>(dst1), static_cast<uint16_t *>(dst2), pad, n);
+      break;
+    default:
+      hipLaunchKernelGGL(orient_pool_kernel<uint32_t>, grid, blk, 0, s, static_cast<const uint32_t *>(src),
+                         static_cast<uint32_t *>(dst1), static_cast<uint32_t *>(dst2), pad, n);
+      break;
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int group, int grid, hipStream_t s) {
   hipLaunchKernelGGL(reset_counters, dim3(1), dim3(kXcds), 0, s, a.counters, kXcds);
