@@ -296,12 +296,13 @@ int32_t hyd_batch_autotune(hyd_ctx *c, hyd_batch *b, float density, float ert, f
   HYD_CHECK_HIP(hipEventCreate(&e1));
   float best_t = 1e30f;
   hyd_batch_config best_c{8, 4, 8, 1};
-  // the search space: measured optima sit at 2 waves per SIMD and unroll 4;
-  // neighbouring shapes and the 4-wave shared-tile form are timed too
+  // the search space: measured optima sit at 8x8 tiles, unroll 4 and 2-4 waves
+  // per SIMD; neighbouring shapes and the 4-wave shared-tile form are timed too
   std::vector<hyd_batch_config> space;
   for (int tw : {8, 16, 32})
     for (int un : {2, 4})
-      for (int wpc : {6, 8, 12}) space.push_back(hyd_batch_config{tw, un, wpc, 1});
+      for (int wpc : {8, 12, 16}) space.push_back(hyd_batch_config{tw, un, wpc, 1});
+  for (int wpc : {6, 10, 14}) space.push_back(hyd_batch_config{8, 4, wpc, 1});
   for (int un : {2, 4})
     for (int wpc : {8, 12, 16}) space.push_back(hyd_batch_config{8, un, wpc, 4});
   for (const hyd_batch_config &cfg : space) {

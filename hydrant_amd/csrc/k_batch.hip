@@ -104,18 +104,20 @@ __host__ __device__ constexpr int tap_slot(int c) {
 }
 
 // The tap base index is formed in float: exact while pad^3 < 2^24, one
-// conversion instead of three plus two integer products.
+// conversion instead of three plus two integer products.  The march only
+// resolves samples inside the brick (local coordinates in [apron - 0.5,
+// apron - 0.5 + bs] up to rounding, within [0, pad - 1) for an apron >= 1), so
+// the coordinates need no clamp; the index is still bounded by lim (as
+// unsigned, so a negative index is bounded too) to keep all eight taps inside
+// the page whatever the position.
 template <int PERM>
-__device__ __forceinline__ Tap local_tap(float qx, float qy, float qz, float hi, float fpad, uint32_t base) {
-  const float fx = __builtin_amdgcn_fmed3f(qx, 0.f, hi);
-  const float fy = __builtin_amdgcn_fmed3f(qy, 0.f, hi);
-  const float fz = __builtin_amdgcn_fmed3f(qz, 0.f, hi);
+__device__ __forceinline__ Tap local_tap(float fx, float fy, float fz, float fpad, uint32_t lim, uint32_t base) {
   const float flx = floorf(fx), fly = floorf(fy), flz = floorf(fz);
   Tap t;
   t.ax = fx - flx;
   t.ay = fy - fly;
   t.az = fz - flz;
-  t.off = base + (uint32_t)layout_index<PERM>(flx, fly, flz, fpad);
+  t.off = base + min((uint32_t)(int32_t)layout_index<PERM>(flx, fly, flz, fpad), lim);
   return t;
 }
 
@@ -261,11 +263,15 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope 
   const float ert = a.ert > 0.f ? a.ert : 3.f;   // opacity never exceeds 1: 3 disables termination
   const uint32_t pad = PAD > 0 ? (uint32_t)PAD : (uint32_t)a.pad;
   const float fpad = float(pad);
-  const float hi = fpad - 1.0001f;
+  const uint32_t lim = pad * pad * pad - pad * pad - pad - 2u;   // largest base index of a tap cell
   const char *pool = static_cast<const char *>(a.pools[PERM]);
   const float fbs = float(vol.bs), off = float(vol.apron) - 0.5f;
   uint32_t poff = 0xffffffffu;   // tap offset of the last resolved sample (none yet)
   TapVals<FMT> pv{};             // its taps
+  // fetched taps (layout order) of the group's samples; kept across groups so
+  // that a lane reusing taps leaves its registers untouched (stale values the
+  // select below never picks) instead of initialising them
+  TapVals<FMT> tv[UNROLL] = {};
   while (r.k < r.kend) {
     const float t0 = fmaf(0.5f, fminf(1.f, r.lf - float(r.k)), float(r.k)) * step;
     const float ipx = floorf(fmaf(r.dx, t0, r.ox));
@@ -289,35 +295,36 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope 
     const float qdx = r.dx * fbs * step, qdy = r.dy * fbs * step, qdz = r.dz * fbs * step;
     int k = r.k;
     bool done = false;
-    // groups of UNROLL samples with all tap loads in flight together; samples
-    // past kb get weight 0 (they composite nothing, and their taps stay inside
-    // the brick through the clamped local frame)
+    // groups of UNROLL samples with all tap loads in flight together; slots
+    // past kb repeat the brick's last sample with weight 0 (they composite
+    // nothing, and reuse that sample's taps, so they issue no loads)
+    const float klast = float(kb - 1);
     while (k < kb) {
+      const float kf_base = float(k);   // sample indices stay exact in float (< 2^24)
       const int nvalid = min(UNROLL, kb - k);
       Tap tp[UNROLL];
-      TapVals<FMT> tv[UNROLL];   // fetched taps, layout order
       bool fresh[UNROLL];        // the sample's cell differs from the previous sample's
       float wv[UNROLL];
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
-        const float kf0 = float(k + u);
+        const float kf0 = fminf(kf_base + float(u), klast);
         const float w = fminf(1.f, r.lf - kf0);
         wv[u] = u < nvalid ? w : 0.f;
         const float kf = fmaf(0.5f, w, kf0);
-        tp[u] = local_tap<PERM>(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), hi, fpad, base);
+        tp[u] = local_tap<PERM>(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), fpad, lim, base);
         fresh[u] = tp[u].off != (u == 0 ? poff : tp[u - 1].off);
-        TapVals<FMT> t{};
-        if (fresh[u]) t = fetch<FMT, PAD>(pool, tp[u].off, pad);
-        tv[u] = t;
+        if (fresh[u]) tv[u] = fetch<FMT, PAD>(pool, tp[u].off, pad);
       }
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
         // a sample in the previous sample's cell reuses its taps (same offset, same voxels)
-        // the flag passes through an empty asm so that the compiler cannot fold
-        // this select into the fetch branch (as "copy the previous taps, then
-        // load": the copy would wait for the previous sample's loads)
-        int keep = fresh[u] ? 0 : 1;
-        asm("" : "+v"(keep));
+        // the comparison is redone on an offset passed through an empty asm, so
+        // that the compiler cannot fold this select into the fetch branch (as
+        // "copy the previous taps, then load": the copy would wait for the
+        // previous sample's loads)
+        uint32_t o = tp[u].off;
+        asm("" : "+v"(o));
+        const bool keep = o == (u == 0 ? poff : tp[u - 1].off);
 #pragma unroll
         for (int c = 0; c < tap_regs<FMT>(); ++c) tv[u].v[c] = keep ? (u == 0 ? pv.v[c] : tv[u - 1].v[c]) : tv[u].v[c];
         const bool live = r.v.w <= ert;
