@@ -68,6 +68,43 @@ __device__ __forceinline__ void sample(const DiffArgs &a, const VolumeRef &vr, f
   sm.s = s;
 }
 
+// Taps of the last sample of sample_reuse: its first and last tap offsets
+// (which fix all eight) and their values.
+struct TapCache {
+  int id0 = -1, id7 = -1;
+  float v[8];
+};
+
+// sample() for consecutive samples of one ray: a sample whose eight taps are
+// the previous sample's (same first and last offset) loads nothing.  The
+// weights and the sum are the same operations, so the value is bit-identical.
+__device__ __forceinline__ void sample_reuse(const DiffArgs &a, const VolumeRef &vr, float px, float py, float pz,
+                                             Smp &sm, TapCache &tc) {
+  const float fx = float(a.nx) * px - 0.5f, fy = float(a.ny) * py - 0.5f, fz = float(a.nz) * pz - 0.5f;
+  const float flx = floorf(fx), fly = floorf(fy), flz = floorf(fz);
+  const float gx = fx - flx, gy = fy - fly, gz = fz - flz;
+  const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+  const int sx = a.nx * 4, sxy = a.nx * a.ny * 4;   // byte strides
+  const int x0 = iclamp(ix, 0, a.nx - 1) * 4, x1 = iclamp(ix + 1, 0, a.nx - 1) * 4;
+  const int y0 = iclamp(iy, 0, a.ny - 1) * sx, y1 = iclamp(iy + 1, 0, a.ny - 1) * sx;
+  const int z0 = iclamp(iz, 0, a.nz - 1) * sxy, z1 = iclamp(iz + 1, 0, a.nz - 1) * sxy;
+  const int r00 = z0 + y0, r01 = z0 + y1, r10 = z1 + y0, r11 = z1 + y1;
+  const int id[8] = {r00 + x0, r00 + x1, r01 + x0, r01 + x1, r10 + x0, r10 + x1, r11 + x0, r11 + x1};
+  if (id[0] != tc.id0 || id[7] != tc.id7) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) tc.v[q] = vol_at(vr, id[q]);
+    tc.id0 = id[0];
+    tc.id7 = id[7];
+  }
+  const float hx = 1.f - gx, hy = 1.f - gy, hz = 1.f - gz;
+  const float w[8] = {hx * hy * hz, gx * hy * hz, hx * gy * hz, gx * gy * hz,
+                      hx * hy * gz, gx * hy * gz, hx * gy * gz, gx * gy * gz};
+  float s = 0.f;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) s = fmaf(w[q], tc.v[q], s);
+  sm.s = s;
+}
+
 struct Shade {
   float c[3], a, alpha, g;
   int j0, j1;
@@ -290,6 +327,7 @@ __device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long 
   if (amask != 0ull) {
     const float *ck = a.ckpts + ray * a.nck;
     float S[3] = {0.f, 0.f, 0.f}, P = 1.f;
+    TapCache tc;
     const int nseg = a.nck - 1;
     float Tnext = active ? fabsf(ck[nseg]) : 1.f;
     for (int seg = nseg - 1; seg >= 0; --seg) {
@@ -303,7 +341,7 @@ __device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long 
           float px, py, pz;
           position(r, k, px, py, pz);
           Smp sm;
-          sample(a, vr, px, py, pz, sm);
+          sample_reuse(a, vr, px, py, pz, sm, tc);
           Shade sh;
           shade(tf, a.tf_n, a.sigma, r.dt, sm.s, sh);
           float Tk;
