@@ -176,6 +176,7 @@ __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs a) {
     float C0 = 0.f, C1 = 0.f, C2 = 0.f, T = 1.f;
     const int nseg = a.nck - 1;
     if (setup(a, a.views[view], x, y, r)) {
+      TapCache tcf;   // consecutive samples in one cell share their taps
       for (int seg = 0; seg < nseg; ++seg) {
         const float Tstart = T;
         float amax = 0.f;
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs a) {
           float px, py, pz;
           position(r, k, px, py, pz);
           Smp sm;
-          sample(a, vr, px, py, pz, sm);
+          sample_reuse(a, vr, px, py, pz, sm, tcf);
           Shade sh;
           shade(s_tf, a.tf_n, a.sigma, r.dt, sm.s, sh);
           const float ta = T * sh.alpha;
@@ -293,9 +294,10 @@ constexpr int kStage = 16;   // samples per ray staged in LDS before a store
 constexpr int kStagePitch = kStage + 1;
 
 // Back-to-front adjoint of one 8x8 ray tile; the wave runs the sample loop in
-// lockstep (inactive rays masked).  The TF table is read from global memory
-// (L1-resident) rather than LDS, so the TF lookups of a sample do not queue
-// behind the previous sample's LDS atomics.  Writes dL/ds of every sample of the
+// lockstep (inactive rays masked).  The TF table is read from LDS: the
+// kernel is bound by the vector-memory data path (tap gathers), and TF reads
+// from global memory would add two scattered 16 B lane-loads per sample.
+// Consecutive samples in one cell reuse their taps.  Writes dL/ds of every sample of the
 // active rays (dL/ds layout [view][y][x][k]: staged 16 samples at a time in
 // LDS so each ray's run leaves as one 64 B store) and every ray's max |dL/ds|
 // (0 for inactive rays); adds the TF-gradient terms to the fixed-point LDS
@@ -428,9 +430,11 @@ __global__ __launch_bounds__(256) void diff_adjoint_kernel(DiffArgs a) {
   __shared__ unsigned long long s_gtf[4 * kMaxTf];   // [channel][entry]
   __shared__ float s_b[kBoundParts];
   __shared__ float s_stage[4][64 * kStagePitch];
+  __shared__ float4 s_tfl[kMaxTf];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int i = tid; i < 4 * kMaxTf; i += 256) s_gtf[i] = 0ull;
   for (int i = tid; i < kBoundParts; i += 256) s_b[i] = a.bound[i];
+  for (int i = tid; i < a.tf_n; i += 256) s_tfl[i] = reinterpret_cast<const float4 *>(a.tf)[i];
   __syncthreads();
   float b = 0.f;
   for (int i = lane; i < kBoundParts; i += 64) b = fmaxf(b, s_b[i]);
@@ -446,7 +450,7 @@ __global__ __launch_bounds__(256) void diff_adjoint_kernel(DiffArgs a) {
     if (t >= total) break;
     const int view = t / per_view, rem = t - view * per_view;
     const int ty = rem / tiles_x, tx = rem - ty * tiles_x;
-    adjoint_tile(a, reinterpret_cast<const float4 *>(a.tf), s_gtf, s_stage[wave], scale, view, tx, ty, lane);
+    adjoint_tile(a, s_tfl, s_gtf, s_stage[wave], scale, view, tx, ty, lane);
   }
   __syncthreads();
   const double unscale = ldexp(1.0, -e);
