@@ -139,7 +139,7 @@ int32_t hyd_diff_backward(hyd_ctx *c, const hyd_diff_params *p, const hyd_view *
   const size_t ds_floats = diff_ds_floats(n_views, width, height, a.K);
   const size_t rays_off = (ds_floats * sizeof(float) + 255) & ~(size_t)255;
   a.view_groups = diff_view_groups(a.nx, a.ny, a.nz, n_views);
-  const size_t blocks_floats = (size_t)a.view_groups * diff_bricks(a.nx, a.ny, a.nz) * 729u;
+  const size_t blocks_floats = (size_t)a.view_groups * diff_bricks(a.nx, a.ny, a.nz) * kDiffBlock;
   const size_t need =
       rays_off + sizeof(float) * (8 * (size_t)n_rays + blocks_floats + (size_t)n_rays + kDiffBoundParts) + 64;
   if (c->diff_scratch_cap < need) {

@@ -34,7 +34,7 @@ struct DiffArgs {
   float *ds;                   // [view][y][x][K] dL/ds per sample
   float *dsmax;                // [view][y][x] max |dL/ds| of the ray (0: nothing to scatter)
   float4 *rays;                // [view][y][x][2] setup of active rays: (dx, dy, dz, tnear), (dt, 0, 0, 0)
-  float *blocks;               // [view group][brick][9^3] voxel-gradient blocks (brick + high-side halo)
+  float *blocks;               // [view group][brick][10^3] voxel-gradient blocks (brick + one-voxel halo)
   int32_t view_groups;         // scatter workgroups per brick (views split into contiguous groups)
   float *bound;                // [1] bound of one sample's TF-gradient term
   int32_t *tile_counter;       // [1] work queue of the ray-order pass
@@ -43,6 +43,7 @@ struct DiffArgs {
 };
 
 constexpr int kDiffBoundParts = 256;   // workgroups (partial maxima) of the adjoint's bound prepass
+constexpr int kDiffBlock = 1000;       // floats per voxel-gradient block of an 8^3 brick (10^3 corner slots)
 
 inline size_t diff_bricks(int32_t nx, int32_t ny, int32_t nz) {
   return (size_t)((nx + 7) / 8) * (size_t)((ny + 7) / 8) * (size_t)((nz + 7) / 8);

@@ -235,6 +235,13 @@ __device__ __forceinline__ unsigned long long fixed_term(float v, float scale) {
   return (unsigned long long)(long long)(int)rintf(v * scale);
 }
 
+// A term already scaled (|v| < 2^30) to fixed point by truncation: one
+// conversion; the bias (under one unit of 2^-30 of the bound per term) is far
+// below the FP32 rounding of the final sums.
+__device__ __forceinline__ unsigned long long fixed_trunc(float v) {
+  return (unsigned long long)(long long)(int)v;
+}
+
 // Scale 2^e with b 2^e < 2^30 (frexp: b < 2^E), less the headroom, in [2^-100, 2^100].
 __device__ __forceinline__ int fixed_exponent(float b, int headroom) {
   int E = 0;
@@ -476,26 +483,30 @@ __device__ __forceinline__ void slab(float o, float d, float lo, float hi, float
 
 // Scatter of dL/ds into the voxel gradient, one wave per 8^3 brick.  Every
 // sample belongs to the brick that holds its trilinear cell floor(n p - 0.5)
-// (cell -1 of the clamped low border goes to the first brick); its corners lie
-// in the brick or one voxel above it on each axis, so a brick accumulates a
-// kOwn^3 = 9^3 block: its voxels plus a high-side halo.  Per view: the pixels
-// in the projection of the box holding the brick's cells (plus a margin); the
-// rays that reach the box go to an LDS list with their sample range inside it
-// (conservative); the list's samples are split into 64 contiguous runs, one
-// per lane; each candidate sample's position and cell are recomputed with the
-// forward's operations and samples of other bricks' cells dropped.  Each
-// corner term goes to an int64 LDS accumulator (one scale per (brick, view)
-// from the rays' max |dL/ds|), folded per view into FP32 sums held in
-// registers and stored once as the brick's block in global scratch;
-// diff_combine_kernel then adds each voxel's block entries
-// (own brick and the halos of up to seven lower neighbours) in a fixed order.
+// (cell -1 of the clamped low border goes to the first brick); its corners
+// (unclamped) lie in the brick or one voxel beyond it on either side, so a
+// brick accumulates a kBlk^3 = 10^3 block indexed by the corner coordinates
+// relative to the brick, plus one: no border clamping here, and the eight
+// corner slots of a sample sit at fixed offsets from the first (0, 1, 10, 11,
+// 100, ...).  Per view: the pixels in the projection of the box holding the
+// brick's cells (plus a margin); the rays that reach the box go to an LDS
+// list with their sample range inside it (conservative); the list's samples
+// are split into 64 contiguous runs, one per lane; each candidate sample's
+// position and cell are recomputed with the forward's operations and samples
+// of other bricks' cells dropped.  Each corner term goes to an int64 LDS
+// accumulator (one scale per (brick, view) from the rays' max |dL/ds|),
+// folded per view into FP32 sums held in registers and stored once as the
+// brick's block in global scratch; diff_combine_kernel then adds, per voxel,
+// the block entries that map to it (own brick, the high halo of the lower
+// neighbour, and the border slots that clamp onto it) in a fixed order.
 // The views are split into groups (one workgroup per brick and group, each
 // with its own block) so that the launch has several waves per SIMD slot.
 // No float atomics: the voxel gradient is deterministic.
 constexpr int kList = 64;    // rays per scatter list
-constexpr int kOwn = kBrick + 1;
-constexpr int kOwn3 = kOwn * kOwn * kOwn;
-constexpr int kOwnPerLane = (kOwn3 + 63) / 64;
+constexpr int kBlk = kBrick + 2;                    // corner coordinates -1 .. kBrick, plus one
+constexpr int kBlk3 = kBlk * kBlk * kBlk;
+static_assert(kBlk3 == kDiffBlock, "block size shared with the host");
+constexpr int kBlkPerLane = (kBlk3 + 63) / 64;
 
 struct ScatterList {
   float dx[kList], dy[kList], dz[kList], tn[kList], dt[kList];
@@ -524,51 +535,59 @@ __device__ void scatter_list(const DiffArgs &a, const DiffView &v, const Scatter
   int next = (j + 1 < n) ? L.off[j + 1] : total;
   long long base = (long long)L.ray[j] * a.K + L.k0[j] - L.off[j];   // ds index of list sample `it`: base + it
   int kb = L.k0[j] - L.off[j];                                         // its sample index: kb + it
-  float dsv = start < end ? a.ds[base + start] : 0.f;
-  for (int i = 0; i < per; ++i) {
-    const int it = start + i;
-    if (it < end) {
-      if (it == next) {   // next ray of the list
-        ++j;
-        q.dx = L.dx[j]; q.dy = L.dy[j]; q.dz = L.dz[j]; q.tnear = L.tn[j]; q.dt = L.dt[j];
-        next = (j + 1 < n) ? L.off[j + 1] : total;
-        base = (long long)L.ray[j] * a.K + L.k0[j] - L.off[j];
-        kb = L.k0[j] - L.off[j];
-      }
-      const float cur = dsv;
-      const int k = kb + it;
-      if (it + 1 < end) {
-        const long long nb = (it + 1 == next) ? (long long)L.ray[j + 1] * a.K + L.k0[j + 1] - L.off[j + 1] : base;
-        dsv = a.ds[nb + it + 1];
-      }
-      if (cur != 0.f) {
-        float px, py, pz;
-        position(q, k, px, py, pz);
-        const float fx = float(a.nx) * px - 0.5f, fy = float(a.ny) * py - 0.5f, fz = float(a.nz) * pz - 0.5f;
-        const float flx = floorf(fx), fly = floorf(fy), flz = floorf(fz);
-        const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-        if (ix >= bb.cx0 && ix <= bb.cx1 && iy >= bb.cy0 && iy <= bb.cy1 && iz >= bb.cz0 && iz <= bb.cz1) {
-          const float gx = fx - flx, gy = fy - fly, gz = fz - flz;
-          const float vs = cur * scale;
-          const float wy[2] = {1.f - gy, gy}, wz[2] = {1.f - gz, gz};
-          const int xs[2] = {iclamp(ix, 0, a.nx - 1) - bb.X0, iclamp(ix + 1, 0, a.nx - 1) - bb.X0};
-          const int ys[2] = {(iclamp(iy, 0, a.ny - 1) - bb.Y0) * kOwn, (iclamp(iy + 1, 0, a.ny - 1) - bb.Y0) * kOwn};
-          const int zs[2] = {(iclamp(iz, 0, a.nz - 1) - bb.Z0) * kOwn * kOwn,
-                             (iclamp(iz + 1, 0, a.nz - 1) - bb.Z0) * kOwn * kOwn};
-#pragma unroll
-          for (int c = 0; c < 8; ++c) {
-            // w * (dL/ds * 2^e) as in the forward's weight order: (wx wy) wz, times the scaled dL/ds
-            const float w = (((c & 1) ? gx : 1.f - gx) * wy[(c >> 1) & 1]) * wz[c >> 2];
-            atomicAdd(&s_acc[zs[c >> 2] + ys[(c >> 1) & 1] + xs[c & 1]], fixed_term(w, vs));
-          }
-        }
-      }
+  // one step: item it with dL/ds `cur`; issues the load of item it + 1 and
+  // returns it.  Two steps per loop iteration, alternating two registers, so a
+  // load is consumed a whole step after it is issued; the load is issued
+  // unconditionally (past the run's end it rereads the run's last item) so that
+  // no branch forces an early wait for it.
+  const auto step = [&](int it, float cur) -> float {
+    const bool act = it < end;
+    if (act && it == next) {   // next ray of the list
+      ++j;
+      q.dx = L.dx[j]; q.dy = L.dy[j]; q.dz = L.dz[j]; q.tnear = L.tn[j]; q.dt = L.dt[j];
+      next = (j + 1 < n) ? L.off[j + 1] : total;
+      base = (long long)L.ray[j] * a.K + L.k0[j] - L.off[j];
+      kb = L.k0[j] - L.off[j];
     }
+    const int in = min(it + 1, end - 1);   // a listed item (end >= 1)
+    const long long nb = (in == next) ? (long long)L.ray[j + 1] * a.K + L.k0[j + 1] - L.off[j + 1] : base;
+    const float nxt = a.ds[nb + in];
+    float px, py, pz;
+    position(q, kb + it, px, py, pz);
+    const float fx = float(a.nx) * px - 0.5f, fy = float(a.ny) * py - 0.5f, fz = float(a.nz) * pz - 0.5f;
+    const float flx = floorf(fx), fly = floorf(fy), flz = floorf(fz);
+    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+    if (act && ix >= bb.cx0 && ix <= bb.cx1 && iy >= bb.cy0 && iy <= bb.cy1 && iz >= bb.cz0 && iz <= bb.cz1 &&
+        cur != 0.f) {
+      const float gx = fx - flx, gy = fy - fly, gz = fz - flz;
+      const float vs = cur * scale;
+      const float hx = 1.f - gx, hy = 1.f - gy;
+      const float wz0 = (1.f - gz) * vs, wz1 = gz * vs;   // z weights times the scaled dL/ds
+      const float w00 = hx * hy, w10 = gx * hy, w01 = hx * gy, w11 = gx * gy;
+      // corner slots relative to the brick, 0 .. kBlk - 1 on each axis (24-bit products)
+      const unsigned sxl = (unsigned)(ix - bb.X0 + 1), syl = (unsigned)(iy - bb.Y0 + 1),
+                     szl = (unsigned)(iz - bb.Z0 + 1);
+      unsigned long long *p = s_acc + (__umul24(__umul24(szl, kBlk) + syl, kBlk) + sxl);
+      atomicAdd(p, fixed_trunc(w00 * wz0));
+      atomicAdd(p + 1, fixed_trunc(w10 * wz0));
+      atomicAdd(p + kBlk, fixed_trunc(w01 * wz0));
+      atomicAdd(p + kBlk + 1, fixed_trunc(w11 * wz0));
+      atomicAdd(p + kBlk * kBlk, fixed_trunc(w00 * wz1));
+      atomicAdd(p + kBlk * kBlk + 1, fixed_trunc(w10 * wz1));
+      atomicAdd(p + kBlk * kBlk + kBlk, fixed_trunc(w01 * wz1));
+      atomicAdd(p + kBlk * kBlk + kBlk + 1, fixed_trunc(w11 * wz1));
+    }
+    return nxt;
+  };
+  float dA = a.ds[base + min(start, end - 1)], dB;
+  for (int i = 0; i < per; i += 2) {
+    dB = step(start + i, dA);
+    dA = step(start + i + 1, dB);
   }
 }
 
 __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
-  __shared__ unsigned long long s_acc[kOwn3];
+  __shared__ unsigned long long s_acc[kBlk3];
   __shared__ ScatterList L;
   const int lane = threadIdx.x;
   const int nbx = (a.nx + kBrick - 1) / kBrick, nby = (a.ny + kBrick - 1) / kBrick,
@@ -583,11 +602,11 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
   bb.cx0 = bb.X0 == 0 ? -1 : bb.X0; bb.cy0 = bb.Y0 == 0 ? -1 : bb.Y0; bb.cz0 = bb.Z0 == 0 ? -1 : bb.Z0;
   bb.cx1 = min(bb.X0 + kBrick - 1, a.nx - 1); bb.cy1 = min(bb.Y0 + kBrick - 1, a.ny - 1);
   bb.cz1 = min(bb.Z0 + kBrick - 1, a.nz - 1);
-  float *blk = a.blocks + ((size_t)grp * nbx * nby * nbz + bi) * kOwn3;
-  for (int i = lane; i < kOwn3; i += 64) s_acc[i] = 0ull;
-  float acc[kOwnPerLane];   // the block's FP32 sums over this group's views; entry lane + 64 i
+  float *blk = a.blocks + ((size_t)grp * nbx * nby * nbz + bi) * kBlk3;
+  for (int i = lane; i < kBlk3; i += 64) s_acc[i] = 0ull;
+  float acc[kBlkPerLane];   // the block's FP32 sums over this group's views; entry lane + 64 i
 #pragma unroll
-  for (int i = 0; i < kOwnPerLane; ++i) acc[i] = 0.f;
+  for (int i = 0; i < kBlkPerLane; ++i) acc[i] = 0.f;
   // the box of the owned cells: floor(n p - 0.5) in [c0, c1]  <=>  p in [(c0 + 0.5) / n, (c1 + 1.5) / n)
   const float delta = 1e-4f;
   const float lo[3] = {(float(bb.cx0) + 0.5f) / float(a.nx) - delta, (float(bb.cy0) + 0.5f) / float(a.ny) - delta,
@@ -696,9 +715,9 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
     __syncthreads();
     const double unscale = ldexp(1.0, -e);
 #pragma unroll
-    for (int i = 0; i < kOwnPerLane; ++i) {
+    for (int i = 0; i < kBlkPerLane; ++i) {
       const int idx = lane + 64 * i;
-      if (idx < kOwn3) {
+      if (idx < kBlk3) {
         const long long val = (long long)s_acc[idx];
         if (val != 0) {
           acc[i] += (float)((double)val * unscale);
@@ -709,15 +728,29 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < kOwnPerLane; ++i) {
+  for (int i = 0; i < kBlkPerLane; ++i) {
     const int idx = lane + 64 * i;
-    if (idx < kOwn3) blk[idx] = acc[i];
+    if (idx < kBlk3) blk[idx] = acc[i];
   }
 }
 
-// grad_vol[v] = over the view groups, the owning brick's block entry plus the
-// high-side halo entries of the (up to seven) bricks below it on each axis,
-// summed in a fixed order.
+// Block entries that map to voxel coordinate v on one axis of extent n: the
+// own brick's slot v - X0 + 1; slot kBrick + 1 of the brick below when v is
+// its brick's first voxel (the lower neighbour's high halo); slot 0 of the
+// first brick for v = 0 (corner -1, clamped) and slot n - X0 + 1 of the last
+// brick for v = n - 1 (corner n, clamped).  Returns the count (<= 3).
+__device__ __forceinline__ int axis_sources(int v, int n, int *db, int *slot) {
+  const int b = v / kBrick, l = v - b * kBrick;
+  int c = 0;
+  db[c] = 0; slot[c] = l + 1; ++c;
+  if (l == 0 && b > 0) { db[c] = -1; slot[c] = kBrick + 1; ++c; }
+  if (v == 0) { db[c] = 0; slot[c] = 0; ++c; }
+  if (v == n - 1) { db[c] = 0; slot[c] = n - b * kBrick + 1; ++c; }
+  return c;
+}
+
+// grad_vol[v] = over the view groups, every block entry that maps to v
+// (axis_sources on each axis), summed in a fixed order.
 __global__ __launch_bounds__(256) void diff_combine_kernel(DiffArgs a) {
   const size_t nvox = (size_t)a.nx * a.ny * a.nz;
   const int nbx = (a.nx + kBrick - 1) / kBrick, nby = (a.ny + kBrick - 1) / kBrick;
@@ -725,18 +758,17 @@ __global__ __launch_bounds__(256) void diff_combine_kernel(DiffArgs a) {
   for (size_t vi = (size_t)blockIdx.x * 256 + threadIdx.x; vi < nvox; vi += (size_t)gridDim.x * 256) {
     const int X = (int)(vi % a.nx), Y = (int)((vi / a.nx) % a.ny), Z = (int)(vi / ((size_t)a.nx * a.ny));
     const int bx = X / kBrick, by = Y / kBrick, bz = Z / kBrick;
-    const int lx = X - bx * kBrick, ly = Y - by * kBrick, lz = Z - bz * kBrick;
+    int dbx[3], sx[3], dby[3], sy[3], dbz[3], sz[3];
+    const int cx = axis_sources(X, a.nx, dbx, sx), cy = axis_sources(Y, a.ny, dby, sy),
+              cz = axis_sources(Z, a.nz, dbz, sz);
     float g = 0.f;
     for (int grp = 0; grp < a.view_groups; ++grp) {
-#pragma unroll
-      for (int s = 0; s < 8; ++s) {
-        const int sx = s & 1, sy = (s >> 1) & 1, sz = s >> 2;
-        // a lower neighbour's halo covers only its first layer above it: local index kBrick
-        if ((sx && (lx != 0 || bx == 0)) || (sy && (ly != 0 || by == 0)) || (sz && (lz != 0 || bz == 0))) continue;
-        const size_t b = grp * nb + ((size_t)(bz - sz) * nby + (by - sy)) * nbx + (bx - sx);
-        const int ox = lx + sx * kBrick, oy = ly + sy * kBrick, oz = lz + sz * kBrick;
-        g += a.blocks[b * kOwn3 + (oz * kOwn + oy) * kOwn + ox];
-      }
+      for (int k = 0; k < cz; ++k)
+        for (int j = 0; j < cy; ++j)
+          for (int i = 0; i < cx; ++i) {
+            const size_t b = grp * nb + ((size_t)(bz + dbz[k]) * nby + (by + dby[j])) * nbx + (bx + dbx[i]);
+            g += a.blocks[b * kBlk3 + (sz[k] * kBlk + sy[j]) * kBlk + sx[i]];
+          }
     }
     a.grad_vol[vi] = g;
   }
