@@ -119,3 +119,41 @@ def test_batch_rejects_bad_config(gpu_ctx):
                                         None)
         assert code == L.HYD_ERR_ARG
     b.close()
+
+
+def test_pool_change_refreshes_oriented_layouts():
+    """The batched renderer reads the pool through y- and z-fastest layouts derived at
+    its first launch after an upload or bind: a new pool (uploaded, then bound from
+    device memory) must render exactly as a fresh context does."""
+    import dataclasses
+
+    import torch
+    from hydrant_amd.hip import Context
+
+    wl = _small(L.FMT_F16, n_views=4)
+    pad = wl.bs + 2 * wl.apron
+    ctx = Context(0)
+    b = _setup(ctx, wl)
+    cfg = batch_config(8, 4, 8)
+    first = b.render(cfg, wl.density, wl.ert)[0].clone()
+    # the same pages with the voxel order reversed within each page
+    bricks = np.ascontiguousarray(wl.pool).reshape(-1, pad ** 3)
+    other = np.ascontiguousarray(bricks[:, ::-1]).reshape(wl.pool.shape)
+    ctx.upload_pool(wl.fmt, pad, other)
+    second = b.render(cfg, wl.density, wl.ert)[0].clone()
+    ctx2 = Context(0)
+    b2 = _setup(ctx2, dataclasses.replace(wl, pool=other))
+    ref = b2.render(cfg, wl.density, wl.ert)[0]
+    torch.cuda.synchronize()
+    assert not torch.equal(second, first)
+    assert torch.equal(second, ref)
+    # the original pool again, now bound from device memory
+    dev = torch.from_numpy(np.ascontiguousarray(wl.pool).view(np.uint8).reshape(-1).copy()).cuda()
+    ctx.bind_pool_device(wl.fmt, pad, bricks.shape[0], dev.data_ptr())
+    third = b.render(cfg, wl.density, wl.ert)[0]
+    torch.cuda.synchronize()
+    assert torch.equal(third, first)
+    b.close()
+    b2.close()
+    ctx2.close()
+    ctx.close()
