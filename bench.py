@@ -11,6 +11,9 @@ HBM.  Workloads:
   diff     the differentiable form (hydrant_amd/diff.py): 32 views at 256^2 of a
            128^3 FP32 volume, 512 samples per ray, transmittance checkpoints every
            64; forward and forward + adjoint times, and a 200-step Adam fit.
+  paged    one 512x512 view of a 1024^3 u8 Perlin volume in 64^3 + 1 bricks, 40%
+           resident, reference schedule (ms/frame).
+  iso      one 1024x1024 isosurface view of a 512^3 FP16 sphere SDF (ms/frame).
 Unit of work: one composited sample (a march step that does a TF lookup and a
 front-to-back composite); the count comes from the kernel's counting variant
 and equals the oracle's.  Prints ONE JSON line.
@@ -118,6 +121,104 @@ def bench_plain(torch, args):
     }
     ctx.close()
     return res
+
+
+def bench_paged(torch, args):
+    """Paged Volume: 1024^3 u8 Perlin fBm in 64^3 + 1 bricks, 40% of the bricks resident
+    through a seeded page table (a ray stops at a non-resident brick), empty bricks
+    skipped by the chebyshev grid, 512^2, reference schedule, u8 pool."""
+    from hydrant_amd import workloads as W
+    from hydrant_amd.hip import Context
+
+    t0 = time.perf_counter()
+    sc = W.paged_volume()
+    gen_s = time.perf_counter() - t0
+    ctx = Context(0)
+    ctx.load(sc.bricked, sc.tf)
+    p = sc.volume_params()
+    view = sc.view()
+    rgba = torch.empty((sc.H, sc.W, 4), dtype=torch.float32, device="cuda")
+    n_samples = int(ctx.render_volume(p, view, sc.W, sc.H, film=False, count=True, rgba=rgba)["samples"].item())
+
+    def step():
+        ctx.render_volume(p, view, sc.W, sc.H, film=False, count=False, rgba=rgba)
+
+    wall, ev = time_kernel(torch, step, args.steps, args.warmup)
+    ms = wall * 1e3 / args.steps
+    k_ms = float(np.mean(ev))
+    alg = n_samples * 8 * 1 + sc.W * sc.H * 16
+    ctx.close()
+    return {"ms_per_frame": round(ms, 4), "kernel_ms": round(k_ms, 4), "samples_per_frame": n_samples,
+            "msamples_per_s": round(n_samples / (ms * 1e-3) / 1e6, 3),
+            "achieved_gbs": round(alg / (k_ms * 1e-3) / 1e9, 1),
+            "alg_bytes_rule": "samples x 8 taps x 1 B + pixels x 16 B",
+            "resident_pages": int((sc.bricked.vaddr >= 0).sum()), "bricks": int(sc.bricked.vaddr.size),
+            "volume": 1024, "res": sc.W, "gen_s": round(gen_s, 1), "scene": sc, "rgba": rgba}
+
+
+def bench_iso(torch, args):
+    """Isosurface: 512^3 FP16 SDF of 64 seeded spheres (smooth union), 64^3 + 1 bricks,
+    1024^2, iso 0, bracket on the sign change then 8 bisections, Phong colour."""
+    from hydrant_amd import _lib as L
+    from hydrant_amd import workloads as W
+    from hydrant_amd.hip import Context
+
+    sc = W.isosurface()
+    ctx = Context(0)
+    ctx.load(sc.bricked)
+    p = W.iso_params(sc, L.ISO_COLOR)
+    view = sc.view()
+    px = torch.empty((sc.H, sc.W, 6), dtype=torch.float32, device="cuda")
+    n_samples = int(ctx.render_iso(p, view, sc.W, sc.H, count=True, px=px)["samples"].item())
+
+    def step():
+        ctx.render_iso(p, view, sc.W, sc.H, count=False, px=px)
+
+    wall, ev = time_kernel(torch, step, args.steps, args.warmup)
+    ms = wall * 1e3 / args.steps
+    k_ms = float(np.mean(ev))
+    alg = n_samples * 8 * 2 + sc.W * sc.H * 24
+    hits = int((px[..., 5].view(torch.int32) == 1).sum().item())
+    ctx.close()
+    return {"ms_per_frame": round(ms, 4), "kernel_ms": round(k_ms, 4), "samples_per_frame": n_samples,
+            "msamples_per_s": round(n_samples / (ms * 1e-3) / 1e6, 3),
+            "achieved_gbs": round(alg / (k_ms * 1e-3) / 1e9, 1),
+            "alg_bytes_rule": "samples x 8 taps x 2 B + pixels x 24 B",
+            "hits": hits, "volume": 512, "res": sc.W, "scene": sc, "params": p, "px": px}
+
+
+def paged_iso_checks(paged, iso):
+    """The paged image against the FP32 oracle on every 7th pixel (reference schedule, the
+    kernel reproduces the CPU backend's arithmetic); the isosurface hit mask (XOR count)
+    and hit depth against the FP32 oracle's full frame."""
+    from oracle import oracle as O
+    from tests.helpers import oracle_view, oracle_volume
+
+    out = {}
+    if paged is not None:
+        sc = paged["scene"]
+        xs, ys = np.meshgrid(np.arange(0, sc.W, 7), np.arange(0, sc.H, 7))
+        xy = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.int32)
+        ref, _ = oracle_volume(sc).render_pixels(oracle_view(sc), xy, prec=32, nthreads=cpu_threads())
+        got = paged["rgba"].cpu().numpy()[xy[:, 1], xy[:, 0]].astype(np.float64)
+        out["paged_max_abs_vs_oracle"] = float(np.max(np.abs(got - ref)))
+        out["paged_pixels_checked"] = int(xy.shape[0])
+    if iso is not None:
+        sc, p = iso["scene"], iso["params"]
+        b = sc.bricked
+        osc = O.IsoScene(sc.W, sc.H, sc.bbox_min, sc.bbox_max, p.step, p.du, np.array(p.to_world[:]), p.eye_pos[:],
+                         p.light_pos[:], p.surface_color[:], p.isovalue, b.cheb, b.vaddr, b.page_k, b.page_b,
+                         b.pool_values(), b.pad, mode=p.mode, corrected=p.corrected, bisect_steps=p.bisect_steps,
+                         max_steps=p.max_steps)
+        r = osc.render(oracle_view(sc), 32, nthreads=cpu_threads())
+        px = iso["px"].cpu().numpy()
+        g_hit, o_hit = px[..., 5].view(np.int32), r["px"][..., 5].astype(np.int32)
+        hit = g_hit == 1
+        out["iso_hit_xor"] = int(np.count_nonzero(g_hit != o_hit))
+        out["iso_max_depth_diff_voxel"] = float(np.max(np.abs(px[..., 4][hit] - r["px"][..., 4][hit])) * b.bs) \
+            if hit.any() else 0.0
+        out["iso_samples_equal"] = bool(r["samples"] == iso["samples_per_frame"])
+    return out
 
 
 def bench_batched(torch, args, plain_ms_per_sample=None):
@@ -292,7 +393,8 @@ def main():
                     help="headline workload (the other one is reported alongside)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg and the FP64 error check")
     ap.add_argument("--err-views", type=int, default=6, help="batched views checked against the FP64 oracle")
-    ap.add_argument("--only", choices=["plain", "batched", "diff"],
+    ap.add_argument("--no-paged-iso", action="store_true", help="skip the paged and isosurface legs")
+    ap.add_argument("--only", choices=["plain", "batched", "diff", "paged", "iso"],
                     help="profiling aid: run only this workload's timed launches and print its line")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-form leg")
     ap.add_argument("--diff-fit-steps", type=int, default=200, help="Adam steps of the differentiable fit")
@@ -324,7 +426,8 @@ def main():
         print(json.dumps(bench_diff(torch, args)), flush=True)
         return
     if args.only:
-        res = bench_plain(torch, args) if args.only == "plain" else bench_batched(torch, args)
+        res = {"plain": bench_plain, "batched": bench_batched, "paged": bench_paged, "iso": bench_iso}[args.only](
+            torch, args)
         print(json.dumps({k: v for k, v in res.items() if isinstance(v, (int, float, str, dict))}), flush=True)
         return
 
@@ -339,6 +442,15 @@ def main():
     if diff is not None:
         print(f"# diff: fwd {diff['fwd_ms']:.3f} ms, fwd+adjoint {diff['fwd_adjoint_ms']:.3f} ms "
               f"(x{diff['ratio_fwd_adjoint_vs_fwd']:.2f})", file=sys.stderr, flush=True)
+
+    paged = iso = None
+    if not args.no_paged_iso:
+        paged = bench_paged(torch, args)
+        print(f"# paged u8: {paged['ms_per_frame']:.3f} ms/frame, {paged['msamples_per_s']:.1f} Msamples/s",
+              file=sys.stderr, flush=True)
+        iso = bench_iso(torch, args)
+        print(f"# isosurface fp16: {iso['ms_per_frame']:.3f} ms/frame, {iso['msamples_per_s']:.1f} Msamples/s",
+              file=sys.stderr, flush=True)
 
     head = batched if args.workload == "batched" else plain
     plain_info = {"msamples_per_s": round(plain["msamples_per_s"], 3), "ms_per_frame": round(plain["ms_per_step"], 4),
@@ -378,6 +490,9 @@ def main():
         "plain_fp32": plain_info,
         "batched_fp16": batched_info,
         "diff": diff,
+        "paged_u8": None if paged is None else {k: v for k, v in paged.items() if k not in ("scene", "rgba")},
+        "isosurface_f16": None if iso is None else {k: v for k, v in iso.items()
+                                                    if k not in ("scene", "params", "px")},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(kern),
@@ -386,6 +501,12 @@ def main():
                      "alg_bytes_rule": f"samples x 8 taps x {bytes_per_voxel} B + pixels x 16 B"},
     }
     if not args.no_cpu:
+        if paged is not None or iso is not None:
+            chk = paged_iso_checks(paged, iso)
+            if paged is not None:
+                out["paged_u8"].update({k: v for k, v in chk.items() if k.startswith("paged_")})
+            if iso is not None:
+                out["isosurface_f16"].update({k: v for k, v in chk.items() if k.startswith("iso_")})
         errs = batched_errors(batched, args.err_views)
         out["batched_fp16"]["max_abs_vs_fp64"] = max(e["max_abs"] for e in errs)
         out["batched_fp16"]["per_view_error"] = errs

@@ -224,10 +224,10 @@ class Context:
         L.call("hyd_render_blocks", self.h, C.byref(params), C.byref(view), W, H, C.c_void_p(rgba.data_ptr()))
         return rgba
 
-    def render_iso(self, params: L.IsoParams, view: L.View, W: int, H: int, count: bool = False):
+    def render_iso(self, params: L.IsoParams, view: L.View, W: int, H: int, count: bool = False, px=None):
         t = self.torch
         self.sync_stream()
-        out = t.empty((H, W, 6), dtype=t.float32, device=self.dev)
+        out = px if px is not None else t.empty((H, W, 6), dtype=t.float32, device=self.dev)
         cnt = t.zeros(1, dtype=t.int64, device=self.dev) if count else None
         L.call("hyd_render_iso", self.h, C.byref(params), C.byref(view), W, H, C.c_void_p(out.data_ptr()),
                C.c_void_p(cnt.data_ptr() if count else 0))
