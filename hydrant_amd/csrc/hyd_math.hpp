@@ -23,6 +23,15 @@ HD float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
 // Integer clamp for lo <= hi (one v_med3_i32 on the device).
 HD int iclamp(int x, int lo, int hi) { return min(max(x, lo), hi); }
 
+// glm mod(x, d) = x - d * floor(x / d) for d > 0 (the sampler's Wrap mode).
+// For 0 <= x <= d (1 - 2^-20) the correctly rounded quotient is below 1, so
+// the result is x itself and the division is skipped; otherwise the full
+// expression is evaluated.  Bit-identical to the expression for every x.
+HD float wrap_mod(float x, float d) {
+  if (x >= 0.f && x <= d * 0.99999904632568359375f) return x;
+  return x - d * floorf(x / d);
+}
+
 HD f3 add(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 HD f3 sub(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 HD f3 mul(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }

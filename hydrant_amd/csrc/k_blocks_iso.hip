@@ -117,6 +117,63 @@ __device__ float iso_sample(const IsoArgs &a, const float *lut, int pgid, f3 x) 
   return y0 * bz + y1 * w[2];
 }
 
+// The last march sample's page descriptor and taps (first and last tap index,
+// which fix all eight, and their values).
+struct IsoTaps {
+  int pgid = -1;
+  float4 pg;
+  size_t id0 = ~(size_t)0, id7 = ~(size_t)0;
+  float v[8];
+};
+
+// iso_sample for consecutive march samples of one ray: the page descriptor and
+// the eight taps are loaded only when they change (at a quarter-voxel step
+// most samples share their predecessor's cell).  Same operations, so the value
+// is bit-identical to iso_sample's.
+template <int FMT>
+__device__ float iso_sample_reuse(const IsoArgs &a, const float *lut, int pgid, f3 x, IsoTaps &c) {
+  if (pgid != c.pgid) {
+    c.pg = a.page[pgid];
+    c.pgid = pgid;
+  }
+  const float4 pg = c.pg;
+  const int pad = a.pad;
+  const float fd = float(pad);
+  float u[3] = {x.x * pg.x + pg.y, x.y * pg.x + pg.z, x.z * pg.x + pg.w};
+  float w[3];
+  int i[3], j[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float fx = fd * u[k];
+    fx = wrap_mod(fx, fd);   // Wrap: glm mod(fx, fd)
+    fx = fx - 0.5f;
+    float flr = floorf(fx);
+    w[k] = fx - flr;
+    i[k] = iclamp((int)flr, 0, pad - 1);
+    j[k] = iclamp((int)ceilf(fx), 0, pad - 1);
+  }
+  const size_t base = (size_t)pgid * pad * pad * pad;
+  const size_t id0 = base + ((size_t)i[2] * pad + i[1]) * pad + i[0];
+  const size_t id7 = base + ((size_t)j[2] * pad + j[1]) * pad + j[0];
+  if (id0 != c.id0 || id7 != c.id7) {
+    auto at = [&](int xx, int yy, int zz) { return vox<FMT>(a.pool, base + ((size_t)zz * pad + yy) * pad + xx, lut); };
+    c.v[0] = at(i[0], i[1], i[2]); c.v[1] = at(j[0], i[1], i[2]);
+    c.v[2] = at(i[0], j[1], i[2]); c.v[3] = at(j[0], j[1], i[2]);
+    c.v[4] = at(i[0], i[1], j[2]); c.v[5] = at(j[0], i[1], j[2]);
+    c.v[6] = at(i[0], j[1], j[2]); c.v[7] = at(j[0], j[1], j[2]);
+    c.id0 = id0;
+    c.id7 = id7;
+  }
+  const float bx = 1.f - w[0], by = 1.f - w[1], bz = 1.f - w[2];
+  float x00 = c.v[0] * bx + c.v[1] * w[0];
+  float x10 = c.v[2] * bx + c.v[3] * w[0];
+  float x01 = c.v[4] * bx + c.v[5] * w[0];
+  float x11 = c.v[6] * bx + c.v[7] * w[0];
+  float y0 = x00 * by + x10 * w[1];
+  float y1 = x01 * by + x11 * w[1];
+  return y0 * bz + y1 * w[2];
+}
+
 __device__ __forceinline__ float gsign(float x) { return float((0.f < x) - (x < 0.f)); }
 
 // IsosurfaceShaderKernel::linear_to_srgb: double arithmetic on a float input
@@ -156,6 +213,7 @@ __global__ __launch_bounds__(64) void iso_kernel(IsoArgs a) {
     const float cdu = 1.f / comp_max(abs3(d));
     float prev_value = 0.f;
     bool have_prev = false;
+    IsoTaps taps;
     while (nsteps > 0) {
       const f3 ip = floor3(o);
       const int cd = a.cheb[grid_index(ip, a.cdim)];
@@ -165,7 +223,7 @@ __global__ __launch_bounds__(64) void iso_kernel(IsoArgs a) {
       } else {
         const int pgid = a.vaddr[grid_index(ip, a.vdim)];
         if (pgid == -1) break;
-        const float value = iso_sample<FMT>(a, s_lut, pgid, sub(o, ip));
+        const float value = iso_sample_reuse<FMT>(a, s_lut, pgid, sub(o, ip), taps);
         ++count;
         bool crossing = gsign(value - a.isovalue) != gsign(prev_value - a.isovalue);
         if (a.corrected) crossing = crossing && have_prev;

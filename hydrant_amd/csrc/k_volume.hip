@@ -65,7 +65,7 @@ __device__ __forceinline__ float brick_sample(const void *pool, int pad, int pgi
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     float fx = fd * u[c];
-    fx = fx - fd * floorf(fx / fd);   // Wrap: glm mod(fx, fd)
+    fx = wrap_mod(fx, fd);   // Wrap: glm mod(fx, fd)
     fx = fx - 0.5f;
     float flr = floorf(fx);
     a[c] = fx - flr;
@@ -87,6 +87,60 @@ __device__ __forceinline__ float brick_sample(const void *pool, int pad, int pgi
   float x10 = v010 * bx + v110 * ax;
   float x01 = v001 * bx + v101 * ax;
   float x11 = v011 * bx + v111 * ax;
+  const float ay = a[1], by = 1.f - ay;
+  float y0 = x00 * by + x10 * ay;
+  float y1 = x01 * by + x11 * ay;
+  const float az = a[2], bz = 1.f - az;
+  return y0 * bz + y1 * az;
+}
+
+// The last march sample's taps: first and last tap index (which fix all eight)
+// and their values.
+struct VolTaps {
+  size_t id0 = ~(size_t)0, id7 = ~(size_t)0;
+  float v[8];
+};
+
+// brick_sample for consecutive march samples of one ray: the eight taps are
+// loaded only when they change (at half-voxel steps about half the samples
+// share their predecessor's cell).  Same operations, so the value is
+// bit-identical to brick_sample's.
+template <int FMT>
+__device__ __forceinline__ float brick_sample_reuse(const void *pool, int pad, int pgid, float4 pg, f3 x,
+                                                    const float *lut, TapIdx *taps, VolTaps &c) {
+  const float fd = float(pad);
+  float u[3] = {x.x * pg.x + pg.y, x.y * pg.x + pg.z, x.z * pg.x + pg.w};
+  float a[3];
+  int i[3], j[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float fx = fd * u[k];
+    fx = wrap_mod(fx, fd);   // Wrap: glm mod(fx, fd)
+    fx = fx - 0.5f;
+    float flr = floorf(fx);
+    a[k] = fx - flr;
+    i[k] = iclamp((int)flr, 0, pad - 1);
+    j[k] = iclamp((int)ceilf(fx), 0, pad - 1);
+  }
+  if (taps) *taps = TapIdx{i[0], i[1], i[2], j[0], j[1], j[2]};
+  const size_t base = (size_t)pgid * pad * pad * pad;
+  const size_t row_ii = base + ((size_t)i[2] * pad + i[1]) * pad;
+  const size_t row_jj = base + ((size_t)j[2] * pad + j[1]) * pad;
+  if (row_ii + i[0] != c.id0 || row_jj + j[0] != c.id7) {
+    const size_t row_ji = base + ((size_t)i[2] * pad + j[1]) * pad;
+    const size_t row_ij = base + ((size_t)j[2] * pad + i[1]) * pad;
+    c.v[0] = Voxel<FMT>::at(pool, row_ii + i[0], lut); c.v[1] = Voxel<FMT>::at(pool, row_ii + j[0], lut);
+    c.v[2] = Voxel<FMT>::at(pool, row_ji + i[0], lut); c.v[3] = Voxel<FMT>::at(pool, row_ji + j[0], lut);
+    c.v[4] = Voxel<FMT>::at(pool, row_ij + i[0], lut); c.v[5] = Voxel<FMT>::at(pool, row_ij + j[0], lut);
+    c.v[6] = Voxel<FMT>::at(pool, row_jj + i[0], lut); c.v[7] = Voxel<FMT>::at(pool, row_jj + j[0], lut);
+    c.id0 = row_ii + i[0];
+    c.id7 = row_jj + j[0];
+  }
+  const float ax = a[0], bx = 1.f - ax;
+  float x00 = c.v[0] * bx + c.v[1] * ax;
+  float x10 = c.v[2] * bx + c.v[3] * ax;
+  float x01 = c.v[4] * bx + c.v[5] * ax;
+  float x11 = c.v[6] * bx + c.v[7] * ax;
   const float ay = a[1], by = 1.f - ay;
   float y0 = x00 * by + x10 * ay;
   float y1 = x01 * by + x11 * ay;
@@ -167,6 +221,9 @@ __device__ void march_reference(const VolumeArgs &a, const float *tf, const floa
   f3 o = st.o;
   int nsteps = st.nsteps;
   int step_size = st.step_size;
+  VolTaps tc;                 // taps of the last sample
+  int cpg = -1;               // page of the cached descriptor
+  float4 pgd = make_float4(0.f, 0.f, 0.f, 0.f);
   while (nsteps > 0) {
     const f3 ip = floor3(o);
     const int cd = a.cheb[grid_index(ip, a.cdim)];
@@ -182,9 +239,13 @@ __device__ void march_reference(const VolumeArgs &a, const float *tf, const floa
       const int gi = grid_index(ip, a.vdim);
       const int pgid = a.vaddr[gi];
       if (pgid == -1) break;
-      const float4 pg = a.page[pgid];
+      if (pgid != cpg) {
+        pgd = a.page[pgid];
+        cpg = pgid;
+      }
       TapIdx taps;
-      const float s = brick_sample<FMT>(a.pool, a.pad, pgid, pg, sub(o, ip), lut, TRACE ? &taps : nullptr);
+      const float s = brick_sample_reuse<FMT>(a.pool, a.pad, pgid, pgd, sub(o, ip), lut, TRACE ? &taps : nullptr,
+                                              tc);
       if (TRACE && ntrace < max_trace) {
         const int gx = gi % a.vdim[0], gy = (gi / a.vdim[0]) % a.vdim[1], gz = gi / (a.vdim[0] * a.vdim[1]);
         trace[ntrace++] = hyd_trace_rec{gx, gy, gz, pgid, taps.ix, taps.iy, taps.iz, taps.jx, taps.jy, taps.jz};
@@ -219,6 +280,9 @@ __device__ void march_lattice(const VolumeArgs &a, const float *tf, const float 
   int k = st.step_size;
   const int kend = (int)ceilf(lf);
   const float cdu = 1.f / comp_max(abs3(d));
+  VolTaps tc;                 // taps of the last sample
+  int cpg = -1;               // page of the cached descriptor
+  float4 pgd = make_float4(0.f, 0.f, 0.f, 0.f);
   while (k < kend) {
     const float w = gmin(1.f, lf - float(k));       // coverage of sample k
     const float t = (float(k) + 0.5f * w) * a.step;  // midpoint of the covered segment
@@ -235,8 +299,11 @@ __device__ void march_lattice(const VolumeArgs &a, const float *tf, const float 
     }
     const int pgid = a.vaddr[grid_index(ip, a.vdim)];
     if (pgid == -1) break;
-    const float4 pg = a.page[pgid];
-    const float s = brick_sample<FMT>(a.pool, a.pad, pgid, pg, sub(o, ip), lut, nullptr);
+    if (pgid != cpg) {
+      pgd = a.page[pgid];
+      cpg = pgid;
+    }
+    const float s = brick_sample_reuse<FMT>(a.pool, a.pad, pgid, pgd, sub(o, ip), lut, nullptr, tc);
     shade_composite(a, tf, s, pgid, st, w);
     if (COUNT) ++count;
     ++k;
