@@ -44,30 +44,15 @@ namespace {
 
 constexpr int kMaxTf = 256;
 
-// Raw stored values; the interpolated value is multiplied once by the
+// Stored voxel types.  The interpolated raw value is multiplied once by the
 // volume's value_scale (1/255 for bricks that hold 8-bit counts).  Voxels are
 // addressed by a 32-bit byte offset from the (uniform) pool base, so a load is
 // one global_load with a scalar base, and the neighbour offsets of a brick
 // edge known at compile time fold into the load's immediate offset.
 template <int FMT> struct BVox;
-template <> struct BVox<HYD_FMT_F32> {
-  using T = float;
-  static __device__ __forceinline__ float at(const char *base, uint32_t byte_off, uint32_t d) {
-    return reinterpret_cast<const float *>(base + byte_off)[d];
-  }
-};
-template <> struct BVox<HYD_FMT_F16> {
-  using T = __half;
-  static __device__ __forceinline__ float at(const char *base, uint32_t byte_off, uint32_t d) {
-    return __half2float(reinterpret_cast<const __half *>(base + byte_off)[d]);
-  }
-};
-template <> struct BVox<HYD_FMT_U8> {
-  using T = uint8_t;
-  static __device__ __forceinline__ float at(const char *base, uint32_t byte_off, uint32_t d) {
-    return float(reinterpret_cast<const uint8_t *>(base + byte_off)[d]);
-  }
-};
+template <> struct BVox<HYD_FMT_F32> { using T = float; };
+template <> struct BVox<HYD_FMT_F16> { using T = __half; };
+template <> struct BVox<HYD_FMT_U8> { using T = uint8_t; };
 
 struct RayState {
   float ox, oy, oz, dx, dy, dz;
