@@ -1,6 +1,7 @@
 // C ABI of the batched renderer: multi-volume set, batch plans, HIP-graph
 // capture/replay and the launch-shape autotuner.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -71,22 +72,37 @@ int32_t ensure_tiles(hyd_batch *b, int tw, int group, hipStream_t s) {
   return HYD_OK;
 }
 
-// The pool's y- and z-fastest layouts, made once per pool upload / bind (the
-// stream is synchronised so that graphs captured on another stream see them).
 size_t layout_bytes(const Ctx *c) {
   const size_t elem = c->pool.fmt == HYD_FMT_U8 ? 1 : c->pool.fmt == HYD_FMT_F16 ? 2 : 4;
   return (size_t)c->pool.pad * c->pool.pad * c->pool.pad * (size_t)c->pool.n_pages * elem;
 }
 
+// The pool's y- and z-fastest layouts, made once per pool upload / bind (the
+// stream is synchronised so that graphs captured on another stream see them).
+// When the device cannot hold them (twice the pool's bytes), or when the
+// environment sets HYD_BATCH_SINGLE_LAYOUT=1, the renderer runs on the
+// x-fastest layout alone: same images, every tile reads layout 0.
 int32_t ensure_oriented(Ctx *c) {
-  if (c->own_oriented && c->oriented_gen == c->pool_gen) return HYD_OK;
+  if (c->oriented_gen == c->pool_gen) return HYD_OK;
   const size_t bytes = layout_bytes(c);
+  const char *single = std::getenv("HYD_BATCH_SINGLE_LAYOUT");
+  if (single && std::strcmp(single, "1") == 0) {
+    if (c->own_oriented) (void)hipFree(c->own_oriented);
+    c->own_oriented = nullptr;
+    c->oriented_cap = 0;
+    c->oriented_gen = c->pool_gen;
+    return HYD_OK;
+  }
   if (c->oriented_cap < 2 * bytes) {
     if (c->own_oriented) (void)hipFree(c->own_oriented);
     c->own_oriented = nullptr;
     c->oriented_cap = 0;
-    HYD_REQUIRE(hipMalloc(&c->own_oriented, 2 * bytes) == hipSuccess, HYD_ERR_HIP,
-                "hipMalloc of the oriented pool layouts (%zu bytes) failed", 2 * bytes);
+    if (hipMalloc(&c->own_oriented, 2 * bytes) != hipSuccess) {
+      (void)hipGetLastError();   // clear the allocation error: the fallback needs no memory
+      c->own_oriented = nullptr;
+      c->oriented_gen = c->pool_gen;
+      return HYD_OK;
+    }
     c->oriented_cap = 2 * bytes;
   }
   HYD_CHECK_HIP(launch_orient_pool(c->pool.fmt, c->pool.data, c->own_oriented,
@@ -119,8 +135,9 @@ int32_t fill_args(Ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float densi
   a.vols = static_cast<const BatchVolDev *>(c->own_vols);
   a.cells = c->own_cells;
   a.pools[0] = c->pool.data;
-  a.pools[1] = c->own_oriented;
-  a.pools[2] = static_cast<const char *>(c->own_oriented) + layout_bytes(c);
+  a.oriented = c->own_oriented != nullptr;
+  a.pools[1] = a.oriented ? c->own_oriented : c->pool.data;
+  a.pools[2] = a.oriented ? static_cast<const char *>(c->own_oriented) + layout_bytes(c) : c->pool.data;
   a.pad = c->pool.pad;
   a.page_elems = (uint32_t)c->pool.pad * c->pool.pad * c->pool.pad;
   a.tf = c->tf;

@@ -34,6 +34,7 @@ struct BatchArgs {
   const BatchVolDev *vols;
   const int32_t *cells;        // concatenated cell codes of every volume
   const void *pools[3];         // the pool in its three layouts (x, y, z fastest)
+  int32_t oriented;             // 0: only the x-fastest layout is held (every tile reads layout 0)
   int32_t pad;
   uint32_t page_elems;         // pad^3
   const float *tf;

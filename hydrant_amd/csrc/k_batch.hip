@@ -387,7 +387,7 @@ __global__ __launch_bounds__(64 * GROUP) void batch_kernel(BatchArgs a) {
         const float ny = comp_max(abs3(dy)) * rsqrtf(dot3(dy, dy) + 1e-30f);
         cols = ny > nx ? 1 : 0;
         const f3 m = abs3(cols ? dy : dx);
-        perm = (m.x >= m.y && m.x >= m.z) ? 0 : (m.y >= m.z ? 1 : 2);
+        perm = !a.oriented ? 0 : (m.x >= m.y && m.x >= m.z) ? 0 : (m.y >= m.z ? 1 : 2);
         perm = __builtin_amdgcn_readfirstlane(perm);
         cols = __builtin_amdgcn_readfirstlane(cols);
       }

@@ -157,3 +157,23 @@ def test_pool_change_refreshes_oriented_layouts():
     b2.close()
     ctx2.close()
     ctx.close()
+
+
+def test_single_layout_fallback_renders_the_same(monkeypatch):
+    """Without the oriented layouts (HYD_BATCH_SINGLE_LAYOUT=1, the path taken when the
+    device cannot hold them) every tile reads the x-fastest layout: same image, bitwise."""
+    import torch
+    from hydrant_amd.hip import Context
+
+    wl = _small(L.FMT_F16, n_views=4)
+    cfg = batch_config(8, 4, 8)
+    ctx = Context(0)
+    b = _setup(ctx, wl)
+    ref = b.render(cfg, wl.density, wl.ert)[0].clone()
+    monkeypatch.setenv("HYD_BATCH_SINGLE_LAYOUT", "1")
+    ctx.upload_pool(wl.fmt, wl.bs + 2 * wl.apron, wl.pool)   # the switch is read when the pool changes
+    out = b.render(cfg, wl.density, wl.ert)[0]
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    b.close()
+    ctx.close()
