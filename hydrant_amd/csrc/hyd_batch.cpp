@@ -1,6 +1,7 @@
 // C ABI of the batched renderer: multi-volume set, batch plans, HIP-graph
 // capture/replay and the launch-shape autotuner.
 #include <algorithm>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -77,39 +78,59 @@ size_t layout_bytes(const Ctx *c) {
   return (size_t)c->pool.pad * c->pool.pad * c->pool.pad * (size_t)c->pool.n_pages * elem;
 }
 
-// The pool's y- and z-fastest layouts, made once per pool upload / bind (the
-// stream is synchronised so that graphs captured on another stream see them).
-// When the device cannot hold them (twice the pool's bytes), or when the
-// environment sets HYD_BATCH_SINGLE_LAYOUT=1, the renderer runs on the
-// x-fastest layout alone: same images, every tile reads layout 0.
+// Holds at least `bytes` of oriented-layout memory; false (and nothing held)
+// when the device cannot give it.
+bool reserve_oriented(Ctx *c, size_t bytes) {
+  if (c->oriented_cap >= bytes) return true;
+  if (c->own_oriented) (void)hipFree(c->own_oriented);
+  c->own_oriented = nullptr;
+  c->oriented_cap = 0;
+  if (hipMalloc(&c->own_oriented, bytes) != hipSuccess) {
+    (void)hipGetLastError();   // clear the allocation error: the fallbacks need less or no memory
+    c->own_oriented = nullptr;
+    return false;
+  }
+  c->oriented_cap = bytes;
+  return true;
+}
+
+bool env_is_1(const char *name) {
+  const char *v = std::getenv(name);
+  return v && std::strcmp(v, "1") == 0;
+}
+
+// The pool's oriented layouts, made once per pool upload / bind (the stream is
+// synchronised so that graphs captured on another stream see them).  FP16
+// pools get the three row-pair layouts (6x the pool's bytes) while their byte
+// offsets fit 32 bits; otherwise, or when the device cannot hold them, the
+// y- and z-fastest layouts (2x).  When the device cannot hold those either, or
+// when the environment sets HYD_BATCH_SINGLE_LAYOUT=1, the renderer runs on
+// the x-fastest layout alone.  HYD_BATCH_NO_PAIRS=1 skips the row-pair
+// layouts.  Every choice renders the same images.
 int32_t ensure_oriented(Ctx *c) {
   if (c->oriented_gen == c->pool_gen) return HYD_OK;
   const size_t bytes = layout_bytes(c);
-  const char *single = std::getenv("HYD_BATCH_SINGLE_LAYOUT");
-  if (single && std::strcmp(single, "1") == 0) {
+  c->oriented_gen = c->pool_gen;
+  c->oriented_pairs = false;
+  if (env_is_1("HYD_BATCH_SINGLE_LAYOUT")) {
     if (c->own_oriented) (void)hipFree(c->own_oriented);
     c->own_oriented = nullptr;
     c->oriented_cap = 0;
-    c->oriented_gen = c->pool_gen;
     return HYD_OK;
   }
-  if (c->oriented_cap < 2 * bytes) {
-    if (c->own_oriented) (void)hipFree(c->own_oriented);
-    c->own_oriented = nullptr;
-    c->oriented_cap = 0;
-    if (hipMalloc(&c->own_oriented, 2 * bytes) != hipSuccess) {
-      (void)hipGetLastError();   // clear the allocation error: the fallback needs no memory
-      c->own_oriented = nullptr;
-      c->oriented_gen = c->pool_gen;
-      return HYD_OK;
-    }
-    c->oriented_cap = 2 * bytes;
+  const bool pairs_fit = c->pool.fmt == HYD_FMT_F16 && 2 * bytes <= (size_t)UINT32_MAX;
+  if (pairs_fit && !env_is_1("HYD_BATCH_NO_PAIRS") && reserve_oriented(c, 6 * bytes)) {
+    HYD_CHECK_HIP(launch_orient_pairs(c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages, c->n_cus,
+                                      c->stream));
+    c->oriented_pairs = true;
+  } else if (reserve_oriented(c, 2 * bytes)) {
+    HYD_CHECK_HIP(launch_orient_pool(c->pool.fmt, c->pool.data, c->own_oriented,
+                                     static_cast<char *>(c->own_oriented) + bytes, c->pool.pad, c->pool.n_pages,
+                                     c->n_cus, c->stream));
+  } else {
+    return HYD_OK;
   }
-  HYD_CHECK_HIP(launch_orient_pool(c->pool.fmt, c->pool.data, c->own_oriented,
-                                   static_cast<char *>(c->own_oriented) + bytes, c->pool.pad, c->pool.n_pages,
-                                   c->n_cus, c->stream));
   HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
-  c->oriented_gen = c->pool_gen;
   return HYD_OK;
 }
 
@@ -134,10 +155,16 @@ int32_t fill_args(Ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float densi
   a.views = b->d_views;
   a.vols = static_cast<const BatchVolDev *>(c->own_vols);
   a.cells = c->own_cells;
-  a.pools[0] = c->pool.data;
   a.oriented = c->own_oriented != nullptr;
-  a.pools[1] = a.oriented ? c->own_oriented : c->pool.data;
-  a.pools[2] = a.oriented ? static_cast<const char *>(c->own_oriented) + layout_bytes(c) : c->pool.data;
+  a.pairs = c->oriented_pairs;
+  if (a.pairs) {
+    const char *o = static_cast<const char *>(c->own_oriented);
+    for (int p = 0; p < 3; ++p) a.pools[p] = o + p * 2 * layout_bytes(c);
+  } else {
+    a.pools[0] = c->pool.data;
+    a.pools[1] = a.oriented ? c->own_oriented : c->pool.data;
+    a.pools[2] = a.oriented ? static_cast<const char *>(c->own_oriented) + layout_bytes(c) : c->pool.data;
+  }
   a.pad = c->pool.pad;
   a.page_elems = (uint32_t)c->pool.pad * c->pool.pad * c->pool.pad;
   a.tf = c->tf;

@@ -35,6 +35,7 @@ struct BatchArgs {
   const int32_t *cells;        // concatenated cell codes of every volume
   const void *pools[3];         // the pool in its three layouts (x, y, z fastest)
   int32_t oriented;             // 0: only the x-fastest layout is held (every tile reads layout 0)
+  int32_t pairs;                // 1: pools[] are the FP16 row-pair layouts (4 B per element)
   int32_t pad;
   uint32_t page_elems;         // pad^3
   const float *tf;
@@ -47,6 +48,10 @@ struct BatchArgs {
 };
 
 hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int group, int grid, hipStream_t s);
+
+// Writes the three FP16 row-pair layouts of an x-fastest FP16 pool (layout p at
+// dst + p * 4 pad^3 n_pages bytes).
+hipError_t launch_orient_pairs(const void *src, void *dst, int pad, int n_pages, int n_cus, hipStream_t s);
 
 // Writes the y-fastest (dst1) and z-fastest (dst2) layouts of an x-fastest pool.
 hipError_t launch_orient_pool(int fmt, const void *src, void *dst1, void *dst2, int pad, int n_pages, int n_cus,

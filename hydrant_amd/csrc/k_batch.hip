@@ -143,6 +143,27 @@ __device__ __forceinline__ TapVals<FMT> fetch(const char *pool, uint32_t off, ui
   return r;
 }
 
+// FP16 row-pair layouts: element (a0, a1, a2) of layout PERM holds the two
+// halves (v(a0, a1, a2), v(a0, a1 + 1, a2)) (a1 + 1 clamped at the page edge),
+// so the four taps of one plane are the 8 bytes of elements a0 and a0 + 1: one
+// 4-byte-aligned dwordx2 per plane, two loads per sample.  Taps: word
+// b0 + 2 b2, half b1 of the layout bits (b0, b1, b2).
+template <int PERM>
+__host__ __device__ constexpr int pair_slot(int c) {
+  return PERM == 1 ? 2 * (((c >> 1) & 1) + 2 * (c & 1)) + (c >> 2)
+       : PERM == 2 ? 2 * ((c >> 2) + 2 * ((c >> 1) & 1)) + (c & 1)
+                   : 2 * ((c & 1) + 2 * (c >> 2)) + ((c >> 1) & 1);
+}
+
+template <int PAD>
+__device__ __forceinline__ TapVals<HYD_FMT_F16> fetch_pairs(const char *pool, uint32_t off, uint32_t pad_rt) {
+  const uint32_t pad = PAD > 0 ? (uint32_t)PAD : pad_rt;
+  TapVals<HYD_FMT_F16> r;
+  __builtin_memcpy(&r.v[0], pool + off * 4u, 8);
+  __builtin_memcpy(&r.v[2], pool + (off + pad * pad) * 4u, 8);
+  return r;
+}
+
 // Tap with x-y-z bits c of layout PERM, as a float (u8 / FP32).
 template <int PERM, int FMT>
 __device__ __forceinline__ float tap_f(const TapVals<FMT> &t, int c) {
@@ -151,9 +172,9 @@ __device__ __forceinline__ float tap_f(const TapVals<FMT> &t, int c) {
 }
 
 // FP16 tap with x-y-z bits c of layout PERM (slot s: word s / 2, high half if s is odd).
-template <int PERM>
+template <int PERM, bool PAIRS>
 __device__ __forceinline__ __half tap_h(const TapVals<HYD_FMT_F16> &t, int c) {
-  const int s = tap_slot<PERM>(c);
+  const int s = PAIRS ? pair_slot<PERM>(c) : tap_slot<PERM>(c);
   const uint32_t w = t.v[s >> 1];
   return __ushort_as_half((unsigned short)((s & 1) ? (w >> 16) : (w & 0xffffu)));
 }
@@ -172,12 +193,12 @@ __device__ __forceinline__ float lerp8(const TapVals<FMT> &v, const Tap &t) {
 
 // FP16: x-lerps as (v0 - a v0) + a v1, both FMAs taking the half taps directly
 // (v_fma_mix_f32), no separate conversions.
-template <int PERM>
+template <int PERM, bool PAIRS>
 __device__ __forceinline__ float lerp8_h(const TapVals<HYD_FMT_F16> &v, const Tap &t) {
   float x[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const __half2 h = __halves2half2(tap_h<PERM>(v, 2 * i), tap_h<PERM>(v, 2 * i + 1));
+    const __half2 h = __halves2half2(tap_h<PERM, PAIRS>(v, 2 * i), tap_h<PERM, PAIRS>(v, 2 * i + 1));
     const float h0 = __low2float(h), h1 = __high2float(h);
     x[i] = fmaf(t.ax, h1, fmaf(-t.ax, h0, h0));
   }
@@ -186,9 +207,9 @@ __device__ __forceinline__ float lerp8_h(const TapVals<HYD_FMT_F16> &v, const Ta
   return fmaf(t.az, y1 - y0, y0);
 }
 
-template <int PERM, int FMT>
+template <int PERM, int FMT, bool PAIRS>
 __device__ __forceinline__ float interpolate(const TapVals<FMT> &v, const Tap &t) {
-  if constexpr (FMT == HYD_FMT_F16) return lerp8_h<PERM>(v, t);
+  if constexpr (FMT == HYD_FMT_F16) return lerp8_h<PERM, PAIRS>(v, t);
   else return lerp8<PERM, FMT>(v, t);
 }
 
@@ -240,7 +261,7 @@ __device__ __forceinline__ float box_exit(const RayState &r, float ipx, float ip
   return fminf(fminf(tx, ty), tz);
 }
 
-template <int FMT, int UNROLL, bool COUNT, int PAD, int PERM>
+template <int FMT, int UNROLL, bool COUNT, int PAD, int PERM, bool PAIRS>
 __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope *ext, RayState &r,
                       unsigned &count) {
   const float step = vol.step, inv_step = 1.f / vol.step;
@@ -298,7 +319,10 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope 
         const float kf = fmaf(0.5f, w, kf0);
         tp[u] = local_tap<PERM>(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), fpad, lim, base);
         fresh[u] = tp[u].off != (u == 0 ? poff : tp[u - 1].off);
-        if (fresh[u]) tv[u] = fetch<FMT, PAD>(pool, tp[u].off, pad);
+        if (fresh[u]) {
+          if constexpr (PAIRS) tv[u] = fetch_pairs<PAD>(pool, tp[u].off, pad);
+          else tv[u] = fetch<FMT, PAD>(pool, tp[u].off, pad);
+        }
       }
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
@@ -313,7 +337,7 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope 
 #pragma unroll
         for (int c = 0; c < tap_regs<FMT>(); ++c) tv[u].v[c] = keep ? (u == 0 ? pv.v[c] : tv[u - 1].v[c]) : tv[u].v[c];
         const bool live = r.v.w <= ert;
-        composite(ext, fsc, smax, interpolate<PERM, FMT>(tv[u], tp[u]), wv[u], live, r.v);
+        composite(ext, fsc, smax, interpolate<PERM, FMT, PAIRS>(tv[u], tp[u]), wv[u], live, r.v);
         if (COUNT) count += (live && u < nvalid) ? 1u : 0u;
       }
       poff = tp[UNROLL - 1].off;
@@ -334,7 +358,7 @@ __device__ __forceinline__ unsigned claim(unsigned *counter) {
 // GROUP 1: single-wave workgroups, each claiming TW x 8 tiles.  GROUP 4: four
 // waves per workgroup claim a 16 x 16 block and take one 8 x 8 tile each, so
 // the four rays packets that touch the same bricks share one CU's L1.
-template <int FMT, int TW, int UNROLL, bool COUNT, int PAD, int GROUP>
+template <int FMT, int TW, int UNROLL, bool COUNT, int PAD, int GROUP, bool PAIRS>
 __global__ __launch_bounds__(64 * GROUP) void batch_kernel(BatchArgs a) {
   extern __shared__ TfSlope s_ext[];   // tf_n + 1 entries (dynamic LDS)
   __shared__ unsigned long long s_count;
@@ -411,9 +435,9 @@ __global__ __launch_bounds__(64 * GROUP) void batch_kernel(BatchArgs a) {
           r.k = 0;
           r.kend = (int)ceilf(r.lf);
           r.cdu = 1.f / comp_max(abs3(d));
-          if (perm == 0) march<FMT, UNROLL, COUNT, PAD, 0>(a, vol, s_ext, r, count);
-          else if (perm == 1) march<FMT, UNROLL, COUNT, PAD, 1>(a, vol, s_ext, r, count);
-          else march<FMT, UNROLL, COUNT, PAD, 2>(a, vol, s_ext, r, count);
+          if (perm == 0) march<FMT, UNROLL, COUNT, PAD, 0, PAIRS>(a, vol, s_ext, r, count);
+          else if (perm == 1) march<FMT, UNROLL, COUNT, PAD, 1, PAIRS>(a, vol, s_ext, r, count);
+          else march<FMT, UNROLL, COUNT, PAD, 2, PAIRS>(a, vol, s_ext, r, count);
         }
         a.out[vw.out_offset + (int64_t)y * vw.W + x] = r.v;
       }
@@ -440,23 +464,58 @@ __global__ __launch_bounds__(256) void orient_pool_kernel(const T *src, T *dst1,
   }
 }
 
+// The three FP16 row-pair layouts of an x-fastest pool (layout p at dst + p *
+// 2 n halves): element (a0, a1, a2) = (v(a0, a1, a2), v(a0, min(a1 + 1, pad -
+// 1), a2)) in layout p's axes (p 0: x, y, z; 1: y, z, x; 2: z, x, y).
+__global__ __launch_bounds__(256) void orient_pairs_kernel(const uint16_t *src, uint32_t *dst, int pad, size_t n) {
+  const size_t p3 = (size_t)pad * pad * pad;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < 3 * n; i += (size_t)gridDim.x * 256) {
+    const int p = (int)(i / n);
+    const size_t e = i - (size_t)p * n;
+    const size_t page = e / p3;
+    const int r = (int)(e - page * p3);
+    const int a0 = r % pad, a1 = (r / pad) % pad, a2 = r / (pad * pad);
+    const int a1n = a1 + 1 < pad ? a1 + 1 : pad - 1;
+    int x0, y0, z0, x1, y1, z1;   // voxel of (a0, a1, a2) and of (a0, a1n, a2)
+    if (p == 0) { x0 = a0; y0 = a1; z0 = a2; x1 = a0; y1 = a1n; z1 = a2; }
+    else if (p == 1) { y0 = a0; z0 = a1; x0 = a2; y1 = a0; z1 = a1n; x1 = a2; }
+    else { z0 = a0; x0 = a1; y0 = a2; z1 = a0; x1 = a1n; y1 = a2; }
+    const uint16_t *pg = src + page * p3;
+    const uint32_t v0 = pg[((size_t)z0 * pad + y0) * pad + x0], v1 = pg[((size_t)z1 * pad + y1) * pad + x1];
+    dst[i] = v0 | (v1 << 16);
+  }
+}
+
 __global__ void reset_counters(unsigned *c, int n) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
 }
 
-// 64^3 bricks with a one-voxel apron (pad 66) get the compile-time brick edge.
-template <int FMT, int TW, int UNROLL, int GROUP>
-hipError_t launch4(const BatchArgs &a, int grid, hipStream_t s) {
+// 64^3 bricks with a one-voxel apron (pad 66) get the compile-time brick edge;
+// FP16 pools read through row-pair layouts when the host built them.
+template <int FMT, int TW, int UNROLL, int GROUP, bool PAIRS>
+hipError_t launch5(const BatchArgs &a, int grid, hipStream_t s) {
   const size_t shmem = (size_t)(a.tf_n + 1) * sizeof(TfSlope);
   const dim3 blk(64 * GROUP);
   if (a.pad == 66) {
-    if (a.samples) hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, true, 66, GROUP>), dim3(grid), blk, shmem, s, a);
-    else hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, false, 66, GROUP>), dim3(grid), blk, shmem, s, a);
+    if (a.samples)
+      hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, true, 66, GROUP, PAIRS>), dim3(grid), blk, shmem, s, a);
+    else
+      hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, false, 66, GROUP, PAIRS>), dim3(grid), blk, shmem, s, a);
   } else {
-    if (a.samples) hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, true, 0, GROUP>), dim3(grid), blk, shmem, s, a);
-    else hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, false, 0, GROUP>), dim3(grid), blk, shmem, s, a);
+    if (a.samples)
+      hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, true, 0, GROUP, PAIRS>), dim3(grid), blk, shmem, s, a);
+    else
+      hipLaunchKernelGGL((batch_kernel<FMT, TW, UNROLL, false, 0, GROUP, PAIRS>), dim3(grid), blk, shmem, s, a);
   }
   return hipGetLastError();
+}
+
+template <int FMT, int TW, int UNROLL, int GROUP>
+hipError_t launch4(const BatchArgs &a, int grid, hipStream_t s) {
+  if constexpr (FMT == HYD_FMT_F16) {
+    if (a.pairs) return launch5<FMT, TW, UNROLL, GROUP, true>(a, grid, s);
+  }
+  return launch5<FMT, TW, UNROLL, GROUP, false>(a, grid, s);
 }
 
 template <int FMT, int TW, int UNROLL>
@@ -503,6 +562,13 @@ hipError_t launch_orient_pool(int fmt, const void *src, void *dst1, void *dst2, 
                          static_cast<uint32_t *>(dst1), static_cast<uint32_t *>(dst2), pad, n);
       break;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_orient_pairs(const void *src, void *dst, int pad, int n_pages, int n_cus, hipStream_t s) {
+  const size_t n = (size_t)pad * pad * pad * (size_t)n_pages;
+  hipLaunchKernelGGL(orient_pairs_kernel, dim3(n_cus * 8), dim3(256), 0, s, static_cast<const uint16_t *>(src),
+                     static_cast<uint32_t *>(dst), pad, n);
   return hipGetLastError();
 }
 

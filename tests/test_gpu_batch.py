@@ -159,9 +159,12 @@ def test_pool_change_refreshes_oriented_layouts():
     ctx.close()
 
 
-def test_single_layout_fallback_renders_the_same(monkeypatch):
-    """Without the oriented layouts (HYD_BATCH_SINGLE_LAYOUT=1, the path taken when the
-    device cannot hold them) every tile reads the x-fastest layout: same image, bitwise."""
+@pytest.mark.parametrize("switch", ["HYD_BATCH_NO_PAIRS", "HYD_BATCH_SINGLE_LAYOUT"])
+def test_layout_fallbacks_render_the_same(monkeypatch, switch):
+    """The FP16 row-pair layouts (the default for FP16 pools), the y-/z-fastest layouts
+    (HYD_BATCH_NO_PAIRS=1, the path taken when the device cannot hold the pairs) and the
+    x-fastest layout alone (HYD_BATCH_SINGLE_LAYOUT=1, when it cannot hold either) hold the
+    same voxels: same image, bitwise."""
     import torch
     from hydrant_amd.hip import Context
 
@@ -170,7 +173,7 @@ def test_single_layout_fallback_renders_the_same(monkeypatch):
     ctx = Context(0)
     b = _setup(ctx, wl)
     ref = b.render(cfg, wl.density, wl.ert)[0].clone()
-    monkeypatch.setenv("HYD_BATCH_SINGLE_LAYOUT", "1")
+    monkeypatch.setenv(switch, "1")
     ctx.upload_pool(wl.fmt, wl.bs + 2 * wl.apron, wl.pool)   # the switch is read when the pool changes
     out = b.render(cfg, wl.density, wl.ert)[0]
     torch.cuda.synchronize()
