@@ -34,7 +34,11 @@
 // fastest axis is best aligned with that direction at the tile: the quad's
 // taps then lie along one voxel row.  The eight taps are put back in x-y-z
 // order before the interpolation and a ray's work does not depend on its lane,
-// so every choice gives bit-identical images.
+// so every choice gives bit-identical images.  FP16 pools are held as row
+// pairs (each 4-byte element also holds the voxel's neighbour along the
+// layout's middle axis): a sample's taps are then two 4-byte-aligned 8-byte
+// loads, which the data path serves at the cost of two dword gathers, instead
+// of four 2-byte-aligned dwords, which cost two accesses per lane each.
 #include <hip/hip_fp16.h>
 
 #include "hyd_batch.hpp"

@@ -177,10 +177,12 @@ int32_t hyd_mat4_inverse(const float m[16], float out[16]);
 int32_t hyd_upload_pool(hyd_ctx *ctx, int32_t fmt, int32_t pad_edge, int32_t n_pages, const void *host);
 /* Same, but the pool already lives on the device; the context keeps the pointer
  * (not a copy), which must stay valid while the context renders from it.
- * The batched renderer reads the pool through two more layouts (y- and
- * z-fastest) that it derives at its first launch after an upload or bind:
- * after changing a bound pool's contents in place, call this again before the
- * next batched launch.  Those layouts take twice the pool's bytes. */
+ * The batched renderer reads the pool through oriented copies that it derives
+ * at its first launch after an upload or bind: after changing a bound pool's
+ * contents in place, call this again before the next batched launch.  The
+ * copies take 6x the pool's bytes for FP16 pools (three row-pair layouts),
+ * else 2x (y- and z-fastest layouts); with less free memory the renderer
+ * reads the pool alone.  Every choice renders the same image. */
 int32_t hyd_bind_pool_device(hyd_ctx *ctx, int32_t fmt, int32_t pad_edge, int32_t n_pages, const void *pool_dev);
 /* Page table (BlockPaging, block_paging.hpp:38-44): vaddr grid of dims[0]*dims[1]*dims[2]
  * int32 (x fastest, -1 = not resident) and per-page BlockSamplerMapping {k, b[3]}
