@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "hyd_batch.hpp"
@@ -52,6 +53,13 @@ struct hyd_batch {
 
 namespace {
 
+// Interleaves the bits of x and y (< 256 each): x in the even bits.
+uint32_t morton2(uint32_t x, uint32_t y) {
+  uint32_t k = 0;
+  for (int i = 0; i < 8; ++i) k |= ((x >> i) & 1u) << (2 * i) | ((y >> i) & 1u) << (2 * i + 1);
+  return k;
+}
+
 int32_t ensure_tiles(hyd_batch *b, int tw, int group, hipStream_t s) {
   const int slot = plan_slot(tw, group);
   if (b->d_tiles[slot]) return HYD_OK;
@@ -60,8 +68,15 @@ int32_t ensure_tiles(hyd_batch *b, int tw, int group, hipStream_t s) {
   for (size_t v = 0; v < b->views.size(); ++v) {
     const int nx = (b->views[v].width + bw - 1) / bw, ny = (b->views[v].height + bh - 1) / bh;
     HYD_REQUIRE(nx <= 256 && ny <= 256, HYD_ERR_RANGE, "view %zu: %d x %d tiles exceeds 256 per axis", v, nx, ny);
+    // Z order (bits of tx and ty interleaved): the tiles that run together on
+    // one XCD form a compact block of the view, whose rays cross fewer bricks
+    // than a strip of whole rows does, so more of their taps hit the L2
+    std::vector<std::pair<uint32_t, uint32_t>> keyed;
+    keyed.reserve((size_t)nx * ny);
     for (int ty = 0; ty < ny; ++ty)
-      for (int tx = 0; tx < nx; ++tx) tiles.push_back((uint32_t)v << 16 | (uint32_t)ty << 8 | (uint32_t)tx);
+      for (int tx = 0; tx < nx; ++tx) keyed.emplace_back(morton2(tx, ty), (uint32_t)ty << 8 | (uint32_t)tx);
+    std::sort(keyed.begin(), keyed.end());
+    for (const auto &k : keyed) tiles.push_back((uint32_t)v << 16 | k.second);
   }
   std::vector<int32_t> &ch = b->chunks[slot];
   ch.resize(kXcds + 1);
