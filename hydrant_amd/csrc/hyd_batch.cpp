@@ -158,6 +158,11 @@ int32_t fill_args(Ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float densi
   HYD_REQUIRE(c->pool.data && c->tf, HYD_ERR_STATE, "batched render needs a pool and a transfer function");
   HYD_REQUIRE(c->pool.pad == c->vol_pad, HYD_ERR_STATE, "pool pad edge %d differs from the volume set's %d",
               c->pool.pad, c->vol_pad);
+  // the kernel addresses voxels by 32-bit byte offsets from the pool base and
+  // forms a tap's index in a page in float (exact up to 2^24 = 256^3)
+  HYD_REQUIRE(layout_bytes(c) <= (size_t)UINT32_MAX && c->pool.pad <= 256, HYD_ERR_RANGE,
+              "batched render needs a pool of at most 4 GiB with a padded brick edge of at most 256 "
+              "(%zu bytes, edge %d)", layout_bytes(c), c->pool.pad);
   const int group = group_of(cfg);
   int32_t st = ensure_tiles(b, cfg->tile_w, group, c->stream);
   if (st) return st;

@@ -214,14 +214,23 @@ __global__ __launch_bounds__(64) void iso_kernel(IsoArgs a) {
     float prev_value = 0.f;
     bool have_prev = false;
     IsoTaps taps;
+    // the brick of the last step and its chebyshev / page-table entries: the
+    // steps inside one brick read them once
+    f3 cip = {-INFINITY, -INFINITY, -INFINITY};
+    int ccd = 0, cpg = -1;
     while (nsteps > 0) {
       const f3 ip = floor3(o);
-      const int cd = a.cheb[grid_index(ip, a.cdim)];
+      if (ip.x != cip.x || ip.y != cip.y || ip.z != cip.z) {
+        cip = ip;
+        ccd = a.cheb[grid_index(ip, a.cdim)];
+        if (ccd == 0) cpg = a.vaddr[grid_index(ip, a.vdim)];
+      }
+      const int cd = ccd;
       if (cd) {
         nsteps -= skip_nblock_steps(o, d, ip, cd, cdu, a.step);
         have_prev = false;
       } else {
-        const int pgid = a.vaddr[grid_index(ip, a.vdim)];
+        const int pgid = cpg;
         if (pgid == -1) break;
         const float value = iso_sample_reuse<FMT>(a, s_lut, pgid, sub(o, ip), taps);
         ++count;

@@ -209,6 +209,23 @@ __device__ __forceinline__ void shade_composite(const VolumeArgs &a, const float
   shade_composite_impl<true>(a, tf, s, pgid, st, w);
 }
 
+// The chebyshev and page-table entries of the brick the last sample fell in:
+// consecutive samples inside one brick read them once (the page entry only
+// for a non-empty brick).
+struct BrickLookup {
+  f3 ip = {-INFINITY, -INFINITY, -INFINITY};
+  int cd = 0, gi = 0, pg = -1;
+  __device__ __forceinline__ void update(const VolumeArgs &a, f3 p) {
+    if (p.x == ip.x && p.y == ip.y && p.z == ip.z) return;
+    ip = p;
+    cd = a.cheb[grid_index(p, a.cdim)];
+    if (cd == 0) {
+      gi = grid_index(p, a.vdim);
+      pg = a.vaddr[gi];
+    }
+  }
+};
+
 // VolumeShaderKernel::main (volume_shader.cu:32-81), reference step schedule.
 // The escalation tests compare float alpha with the double literals 0.93 and
 // 0.85; for binary32 inputs `x > 0.93` equals `x >= 0.93f` (0.93f rounds up),
@@ -224,9 +241,11 @@ __device__ void march_reference(const VolumeArgs &a, const float *tf, const floa
   VolTaps tc;                 // taps of the last sample
   int cpg = -1;               // page of the cached descriptor
   float4 pgd = make_float4(0.f, 0.f, 0.f, 0.f);
+  BrickLookup bl;
   while (nsteps > 0) {
     const f3 ip = floor3(o);
-    const int cd = a.cheb[grid_index(ip, a.cdim)];
+    bl.update(a, ip);
+    const int cd = bl.cd;
     if (cd) {
       // skip_nblock_steps (volume_shader.cu:5-14) with step * step_size
       const float sstep = a.step * float(step_size);
@@ -236,8 +255,8 @@ __device__ void march_reference(const VolumeArgs &a, const float *tf, const floa
       o = add(o, muls(muls(d, di), sstep));
       nsteps -= (int)di;
     } else {
-      const int gi = grid_index(ip, a.vdim);
-      const int pgid = a.vaddr[gi];
+      const int gi = bl.gi;
+      const int pgid = bl.pg;
       if (pgid == -1) break;
       if (pgid != cpg) {
         pgd = a.page[pgid];
@@ -283,12 +302,14 @@ __device__ void march_lattice(const VolumeArgs &a, const float *tf, const float 
   VolTaps tc;                 // taps of the last sample
   int cpg = -1;               // page of the cached descriptor
   float4 pgd = make_float4(0.f, 0.f, 0.f, 0.f);
+  BrickLookup bl;
   while (k < kend) {
     const float w = gmin(1.f, lf - float(k));       // coverage of sample k
     const float t = (float(k) + 0.5f * w) * a.step;  // midpoint of the covered segment
     const f3 o = add(o0, muls(d, t));
     const f3 ip = floor3(o);
-    const int cd = a.cheb[grid_index(ip, a.cdim)];
+    bl.update(a, ip);
+    const int cd = bl.cd;
     if (cd) {
       float tn, tfar;
       intersect_box(o, d, ip, f3{ip.x + 1.f, ip.y + 1.f, ip.z + 1.f}, tn, tfar);
@@ -297,7 +318,7 @@ __device__ void march_lattice(const VolumeArgs &a, const float *tf, const float 
       k = kn > k + 1 ? kn : k + 1;
       continue;
     }
-    const int pgid = a.vaddr[grid_index(ip, a.vdim)];
+    const int pgid = bl.pg;
     if (pgid == -1) break;
     if (pgid != cpg) {
       pgd = a.page[pgid];

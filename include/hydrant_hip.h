@@ -303,7 +303,9 @@ int32_t hyd_batch_destroy(hyd_batch *batch);
 /* Pixels the output buffer must hold (max over views of out_offset + w*h). */
 int32_t hyd_batch_output_pixels(const hyd_batch *batch, int64_t *n_out);
 /* One persistent launch (a counter-reset kernel + the march); RGBA float out.
- * samples_dev, if not NULL, accumulates the composited-sample count. */
+ * samples_dev, if not NULL, accumulates the composited-sample count.  The pool
+ * must be at most 4 GiB with a padded brick edge of at most 256 (voxels are
+ * addressed by 32-bit byte offsets): HYD_ERR_RANGE otherwise. */
 int32_t hyd_batch_render(hyd_ctx *ctx, hyd_batch *batch, const hyd_batch_config *cfg, float density, float ert,
                          float *rgba_dev, unsigned long long *samples_dev);
 /* Record that launch into a HIP graph owned by the batch (no memset nodes). */

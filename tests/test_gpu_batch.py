@@ -180,3 +180,24 @@ def test_layout_fallbacks_render_the_same(monkeypatch, switch):
     assert torch.equal(out, ref)
     b.close()
     ctx.close()
+
+
+def test_batch_rejects_pools_beyond_32_bit_offsets():
+    """The batched kernel addresses voxels by 32-bit byte offsets: a pool of more than
+    4 GiB is refused by the host before any kernel runs (the pool is only declared that
+    large; no launch reads it)."""
+    import torch
+    from hydrant_amd.hip import Context
+
+    wl = _small(L.FMT_F16, n_views=2)
+    pad = wl.bs + 2 * wl.apron
+    ctx = Context(0)
+    b = _setup(ctx, wl)
+    dev = torch.from_numpy(np.ascontiguousarray(wl.pool).view(np.uint8).reshape(-1).copy()).cuda()
+    pages = (1 << 32) // (pad ** 3 * 2) + 1
+    ctx.bind_pool_device(wl.fmt, pad, pages, dev.data_ptr())
+    with pytest.raises(L.HydError) as e:
+        b.render(batch_config(8, 4, 8), wl.density, wl.ert)
+    assert e.value.code == L.HYD_ERR_RANGE
+    b.close()
+    ctx.close()
