@@ -52,48 +52,6 @@ __device__ __forceinline__ int grid_index(f3 p, const int32_t dim[3]) {
 
 struct TapIdx { int ix, iy, iz, jx, jy, jz; };
 
-// Brick sample: BlockSamplerMapping (k*x + b) then CpuSampler Wrap + Linear +
-// normalised coordinates over a pad^3 brick (block_paging.hpp:11-36,
-// cpu_sampler.hpp:88-104,142-166).
-template <int FMT>
-__device__ __forceinline__ float brick_sample(const void *pool, int pad, int pgid, float4 pg, f3 x,
-                                              const float *lut, TapIdx *taps) {
-  const float fd = float(pad);
-  float u[3] = {x.x * pg.x + pg.y, x.y * pg.x + pg.z, x.z * pg.x + pg.w};
-  float a[3];
-  int i[3], j[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    float fx = fd * u[c];
-    fx = wrap_mod(fx, fd);   // Wrap: glm mod(fx, fd)
-    fx = fx - 0.5f;
-    float flr = floorf(fx);
-    a[c] = fx - flr;
-    i[c] = iclamp((int)flr, 0, pad - 1);
-    j[c] = iclamp((int)ceilf(fx), 0, pad - 1);
-  }
-  if (taps) *taps = TapIdx{i[0], i[1], i[2], j[0], j[1], j[2]};
-  const size_t base = (size_t)pgid * pad * pad * pad;
-  const size_t row_ii = base + ((size_t)i[2] * pad + i[1]) * pad;
-  const size_t row_ji = base + ((size_t)i[2] * pad + j[1]) * pad;
-  const size_t row_ij = base + ((size_t)j[2] * pad + i[1]) * pad;
-  const size_t row_jj = base + ((size_t)j[2] * pad + j[1]) * pad;
-  float v000 = Voxel<FMT>::at(pool, row_ii + i[0], lut), v100 = Voxel<FMT>::at(pool, row_ii + j[0], lut);
-  float v010 = Voxel<FMT>::at(pool, row_ji + i[0], lut), v110 = Voxel<FMT>::at(pool, row_ji + j[0], lut);
-  float v001 = Voxel<FMT>::at(pool, row_ij + i[0], lut), v101 = Voxel<FMT>::at(pool, row_ij + j[0], lut);
-  float v011 = Voxel<FMT>::at(pool, row_jj + i[0], lut), v111 = Voxel<FMT>::at(pool, row_jj + j[0], lut);
-  const float ax = a[0], bx = 1.f - ax;
-  float x00 = v000 * bx + v100 * ax;
-  float x10 = v010 * bx + v110 * ax;
-  float x01 = v001 * bx + v101 * ax;
-  float x11 = v011 * bx + v111 * ax;
-  const float ay = a[1], by = 1.f - ay;
-  float y0 = x00 * by + x10 * ay;
-  float y1 = x01 * by + x11 * ay;
-  const float az = a[2], bz = 1.f - az;
-  return y0 * bz + y1 * az;
-}
-
 // The last march sample's taps: first and last tap index (which fix all eight)
 // and their values.
 struct VolTaps {
@@ -101,10 +59,12 @@ struct VolTaps {
   float v[8];
 };
 
-// brick_sample for consecutive march samples of one ray: the eight taps are
-// loaded only when they change (at half-voxel steps about half the samples
-// share their predecessor's cell).  Same operations, so the value is
-// bit-identical to brick_sample's.
+// Brick sample: BlockSamplerMapping (k*x + b) then CpuSampler Wrap + Linear +
+// normalised coordinates over a pad^3 brick (block_paging.hpp:11-36,
+// cpu_sampler.hpp:88-104,142-166), for consecutive march samples of one ray:
+// the eight taps are loaded only when they change (at half-voxel steps about
+// half the samples share their predecessor's cell); the value does not depend
+// on whether they were reloaded.
 template <int FMT>
 __device__ __forceinline__ float brick_sample_reuse(const void *pool, int pad, int pgid, float4 pg, f3 x,
                                                     const float *lut, TapIdx *taps, VolTaps &c) {
@@ -473,6 +433,7 @@ __global__ __launch_bounds__(64) void segment_kernel(VolumeArgs a) {
       const int kend = (int)ceilf(lf);
       const float cdu = 1.f / comp_max(abs3(d));
       int k = 0;
+      VolTaps tc;   // taps of the last sample
       while (k < kend && ord < a.max_slots) {
         float w = gmin(1.f, lf - float(k));
         float t = (float(k) + 0.5f * w) * a.step;
@@ -497,7 +458,7 @@ __global__ __launch_bounds__(64) void segment_kernel(VolumeArgs a) {
         const float4 pg = pgid >= 0 ? a.page[pgid] : make_float4(0.f, 0.f, 0.f, 0.f);
         while (true) {
           if (pgid >= 0) {
-            const float s = brick_sample<FMT>(a.pool, a.pad, pgid, pg, sub(o, ip), s_lut, nullptr);
+            const float s = brick_sample_reuse<FMT>(a.pool, a.pad, pgid, pg, sub(o, ip), s_lut, nullptr, tc);
             shade_composite(a, s_tf, s, pgid, seg, w);
             if (COUNT) ++count;
           }
