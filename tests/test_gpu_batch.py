@@ -201,3 +201,32 @@ def test_batch_rejects_pools_beyond_32_bit_offsets():
     assert e.value.code == L.HYD_ERR_RANGE
     b.close()
     ctx.close()
+
+
+def test_batch_64_bricks_vs_fp64_oracle(monkeypatch):
+    """64^3 bricks with a one-voxel apron take the kernels compiled for a padded edge of
+    66 (the 96-view benchmark's bricks), through each pool layout: row pairs (the FP16
+    default), the single-voxel oriented layouts and the x-fastest pool alone.  Error
+    against the FP64 oracle within 1e-3, and the three layouts give the same image."""
+    import torch
+    from hydrant_amd.hip import Context
+
+    wl = W.batched(sizes=(128, 192), n_views=4, res_choices=(96, 128), fmt=L.FMT_F16, bs=64, apron=1, seed=91)
+    ctx = Context(0)
+    b = _setup(ctx, wl)
+    ref = None
+    for switch in (None, "HYD_BATCH_NO_PAIRS", "HYD_BATCH_SINGLE_LAYOUT"):
+        if switch is not None:
+            monkeypatch.setenv(switch, "1")
+            ctx.upload_pool(wl.fmt, wl.bs + 2 * wl.apron, wl.pool)   # the switch is read when the pool changes
+        out, cnt = b.render(batch_config(8, 4, 16), wl.density, wl.ert, count=True)
+        torch.cuda.synchronize()
+        errs = _view_errors(wl, out, b, 64, exact=True)
+        print(f"pad 66 ({switch}): per-view max-abs vs FP64 {['%.2e' % e for e in errs]}, "
+              f"samples {int(cnt.item())}", flush=True)
+        assert max(errs) <= 1e-3
+        if ref is None:
+            ref = out.clone()
+        assert torch.equal(out, ref)
+    b.close()
+    ctx.close()
