@@ -186,23 +186,30 @@ __device__ __forceinline__ float iso_srgb(float linear) {
 
 // IsosurfaceShaderKernel::main (isosurface_shader.cu:50-125) when
 // corrected == 0; bracketing + bisection + oriented central differences when
-// corrected == 1.
+// corrected == 1.  Two kernels: the march finds each ray's first crossing and
+// leaves its state in the output pixel (v = march position and previous value,
+// depth = value at the crossing, hit = kIsoPending); the hit kernel then
+// refines and shades those pixels.  The march kernel carries none of the
+// refinement's registers, so more of its waves fit on a SIMD to cover the
+// latency of the tap loads.
+constexpr int32_t kIsoPending = -1;
+
 template <int FMT>
-__global__ __launch_bounds__(64) void iso_kernel(IsoArgs a) {
+__global__ __launch_bounds__(64) void iso_march_kernel(IsoArgs a) {
   __shared__ float s_lut[FMT == HYD_FMT_U8 ? 256 : 1];
   if (FMT == HYD_FMT_U8)
     for (int i = threadIdx.x; i < 256; i += blockDim.x) s_lut[i] = a.u8_lut[i];
   __syncthreads();
   const int x = blockIdx.x * kTile + (threadIdx.x % kTile);
   const int y = blockIdx.y * kTile + (threadIdx.x / kTile);
-  unsigned count = 0;
   if (x >= a.W || y >= a.H) return;
+  unsigned count = 0;
   const f3 d = pixel_ray_dir(a.trans, a.ctg, a.ray_o, x, y, a.W, a.H);
-  const f3 origin = a.ray_o;
   f3 o = a.ray_o;
-  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-  float depth = INFINITY;
-  int hit = 0;
+  hyd_iso_pixel px;
+  px.v[0] = px.v[1] = px.v[2] = px.v[3] = 0.f;
+  px.depth = INFINITY;
+  px.hit = 0;
   float tnear, tfar;
   if (intersect_box(o, d, a.bmin, a.bmax, tnear, tfar)) {
     if (tnear < 0) tnear = 0;
@@ -237,71 +244,9 @@ __global__ __launch_bounds__(64) void iso_kernel(IsoArgs a) {
         bool crossing = gsign(value - a.isovalue) != gsign(prev_value - a.isovalue);
         if (a.corrected) crossing = crossing && have_prev;
         if (crossing) {
-          const f3 p = o;
-          const float dt = a.step;
-          v = make_float4(a.surface_color.x, a.surface_color.y, a.surface_color.z, 1.f);
-          f3 inter_p, nn;
-          if (!a.corrected) {
-            const float al = (a.isovalue - prev_value) / (value - prev_value);
-            const float bl = 1.f - al;
-            inter_p = f3{bl * (p.x - dt * d.x) + al * p.x, bl * (p.y - dt * d.y) + al * p.y,
-                         bl * (p.z - dt * d.z) + al * p.z};
-            const f3 q = sub(inter_p, ip);
-            const float h = a.du * 2.f;
-            nn = f3{iso_sample<FMT>(a, s_lut, pgid, f3{q.x - h, q.y - 0.f, q.z - 0.f}) -
-                        iso_sample<FMT>(a, s_lut, pgid, f3{q.x + h, q.y + 0.f, q.z + 0.f}),
-                    iso_sample<FMT>(a, s_lut, pgid, f3{q.x - 0.f, q.y - h, q.z - 0.f}) -
-                        iso_sample<FMT>(a, s_lut, pgid, f3{q.x + 0.f, q.y + h, q.z + 0.f}),
-                    iso_sample<FMT>(a, s_lut, pgid, f3{q.x - 0.f, q.y - 0.f, q.z - h}) -
-                        iso_sample<FMT>(a, s_lut, pgid, f3{q.x + 0.f, q.y + 0.f, q.z + h})};
-          } else {
-            f3 plo = p;
-            f3 phi = f3{p.x - dt * d.x, p.y - dt * d.y, p.z - dt * d.z};
-            const float sprev = gsign(prev_value - a.isovalue);
-            for (int it = 0; it < a.bisect_steps; ++it) {
-              const f3 mid = {(plo.x + phi.x) * 0.5f, (plo.y + phi.y) * 0.5f, (plo.z + phi.z) * 0.5f};
-              const f3 mip = floor3(mid);
-              const int mpg = a.vaddr[grid_index(mip, a.vdim)];
-              const float mv = mpg == -1 ? value : iso_sample<FMT>(a, s_lut, mpg, sub(mid, mip));
-              if (gsign(mv - a.isovalue) == sprev) phi = mid; else plo = mid;
-            }
-            inter_p = f3{(plo.x + phi.x) * 0.5f, (plo.y + phi.y) * 0.5f, (plo.z + phi.z) * 0.5f};
-            f3 gip = floor3(inter_p);
-            int gpg = a.vaddr[grid_index(gip, a.vdim)];
-            if (gpg == -1) { gpg = pgid; gip = ip; }
-            const f3 q = sub(inter_p, gip);
-            const float h = a.du;
-            const f3 g = {iso_sample<FMT>(a, s_lut, gpg, f3{q.x + h, q.y + 0.f, q.z + 0.f}) -
-                              iso_sample<FMT>(a, s_lut, gpg, f3{q.x - h, q.y - 0.f, q.z - 0.f}),
-                          iso_sample<FMT>(a, s_lut, gpg, f3{q.x + 0.f, q.y + h, q.z + 0.f}) -
-                              iso_sample<FMT>(a, s_lut, gpg, f3{q.x - 0.f, q.y - h, q.z - 0.f}),
-                          iso_sample<FMT>(a, s_lut, gpg, f3{q.x + 0.f, q.y + 0.f, q.z + h}) -
-                              iso_sample<FMT>(a, s_lut, gpg, f3{q.x - 0.f, q.y - 0.f, q.z - h})};
-            nn = muls(g, sprev);
-          }
-          float wp4[4], n4[4];
-          mat4_vec4(a.to_world, inter_p.x, inter_p.y, inter_p.z, 1.f, wp4);
-          mat4_vec4(a.to_world, nn.x, nn.y, nn.z, 0.f, n4);
-          const f3 world_p = {wp4[0], wp4[1], wp4[2]};
-          const f3 nrm = normalize3(f3{n4[0], n4[1], n4[2]});
-          const f3 light_dir = normalize3(sub(a.light_pos, world_p));
-          const f3 hv = a.corrected ? normalize3(add(light_dir, normalize3(sub(a.eye_pos, world_p))))
-                                    : normalize3(sub(light_dir, world_p));
-          const float ambient = 0.2f;
-          const float diffuse = .6f * gclamp(dot3(light_dir, nrm), 0.f, 1.f);
-          const float specular = .2f * powf(gclamp(dot3(hv, nrm), 0.f, 1.f), 100.f);
-          const float dist = length3(sub(world_p, a.eye_pos)) / 2.f;
-          if (a.mode == HYD_ISO_COLOR) {
-            const float sc = ambient + (diffuse + specular) / dist;
-            v = make_float4(v.x * sc, v.y * sc, v.z * sc, v.w * sc);
-            v = make_float4(iso_srgb(v.x), iso_srgb(v.y), iso_srgb(v.z), v.w);
-          } else if (a.mode == HYD_ISO_POSITION) {
-            v = make_float4(inter_p.x / a.bmax.x, inter_p.y / a.bmax.y, inter_p.z / a.bmax.z, 1.f);
-          } else {
-            v = make_float4(nrm.x, nrm.y, nrm.z, 1.f);
-          }
-          depth = a.corrected ? length3(sub(inter_p, origin)) : length3(sub(origin, o));
-          hit = 1;
+          px.v[0] = o.x; px.v[1] = o.y; px.v[2] = o.z; px.v[3] = prev_value;
+          px.depth = value;
+          px.hit = kIsoPending;
           break;
         }
         if (a.corrected) { prev_value = value; have_prev = true; }
@@ -310,12 +255,100 @@ __global__ __launch_bounds__(64) void iso_kernel(IsoArgs a) {
       nsteps -= 1;
     }
   }
+  a.out[(size_t)y * a.W + x] = px;
+  if (a.samples && count) atomicAdd(a.samples, (unsigned long long)count);
+}
+
+// Refinement and shading of the pixels the march left pending: the crossing's
+// march position o (its brick ip = floor(o) and page from the page table, as
+// the march had them), the previous and the crossing value.
+template <int FMT>
+__global__ __launch_bounds__(64) void iso_hit_kernel(IsoArgs a) {
+  __shared__ float s_lut[FMT == HYD_FMT_U8 ? 256 : 1];
+  if (FMT == HYD_FMT_U8)
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_lut[i] = a.u8_lut[i];
+  __syncthreads();
+  const int x = blockIdx.x * kTile + (threadIdx.x % kTile);
+  const int y = blockIdx.y * kTile + (threadIdx.x / kTile);
+  if (x >= a.W || y >= a.H) return;
+  hyd_iso_pixel *out = a.out + (size_t)y * a.W + x;
+  const hyd_iso_pixel st = *out;
+  if (st.hit != kIsoPending) return;
+  const f3 d = pixel_ray_dir(a.trans, a.ctg, a.ray_o, x, y, a.W, a.H);
+  const f3 origin = a.ray_o;
+  const f3 o = f3{st.v[0], st.v[1], st.v[2]};
+  const float prev_value = st.v[3], value = st.depth;
+  const f3 ip = floor3(o);
+  const int pgid = a.vaddr[grid_index(ip, a.vdim)];
+  const f3 p = o;
+  const float dt = a.step;
+  float4 v = make_float4(a.surface_color.x, a.surface_color.y, a.surface_color.z, 1.f);
+  f3 inter_p, nn;
+  if (!a.corrected) {
+    const float al = (a.isovalue - prev_value) / (value - prev_value);
+    const float bl = 1.f - al;
+    inter_p = f3{bl * (p.x - dt * d.x) + al * p.x, bl * (p.y - dt * d.y) + al * p.y,
+                 bl * (p.z - dt * d.z) + al * p.z};
+    const f3 q = sub(inter_p, ip);
+    const float h = a.du * 2.f;
+    nn = f3{iso_sample<FMT>(a, s_lut, pgid, f3{q.x - h, q.y - 0.f, q.z - 0.f}) -
+                iso_sample<FMT>(a, s_lut, pgid, f3{q.x + h, q.y + 0.f, q.z + 0.f}),
+            iso_sample<FMT>(a, s_lut, pgid, f3{q.x - 0.f, q.y - h, q.z - 0.f}) -
+                iso_sample<FMT>(a, s_lut, pgid, f3{q.x + 0.f, q.y + h, q.z + 0.f}),
+            iso_sample<FMT>(a, s_lut, pgid, f3{q.x - 0.f, q.y - 0.f, q.z - h}) -
+                iso_sample<FMT>(a, s_lut, pgid, f3{q.x + 0.f, q.y + 0.f, q.z + h})};
+  } else {
+    f3 plo = p;
+    f3 phi = f3{p.x - dt * d.x, p.y - dt * d.y, p.z - dt * d.z};
+    const float sprev = gsign(prev_value - a.isovalue);
+    for (int it = 0; it < a.bisect_steps; ++it) {
+      const f3 mid = {(plo.x + phi.x) * 0.5f, (plo.y + phi.y) * 0.5f, (plo.z + phi.z) * 0.5f};
+      const f3 mip = floor3(mid);
+      const int mpg = a.vaddr[grid_index(mip, a.vdim)];
+      const float mv = mpg == -1 ? value : iso_sample<FMT>(a, s_lut, mpg, sub(mid, mip));
+      if (gsign(mv - a.isovalue) == sprev) phi = mid; else plo = mid;
+    }
+    inter_p = f3{(plo.x + phi.x) * 0.5f, (plo.y + phi.y) * 0.5f, (plo.z + phi.z) * 0.5f};
+    f3 gip = floor3(inter_p);
+    int gpg = a.vaddr[grid_index(gip, a.vdim)];
+    if (gpg == -1) { gpg = pgid; gip = ip; }
+    const f3 q = sub(inter_p, gip);
+    const float h = a.du;
+    const f3 g = {iso_sample<FMT>(a, s_lut, gpg, f3{q.x + h, q.y + 0.f, q.z + 0.f}) -
+                      iso_sample<FMT>(a, s_lut, gpg, f3{q.x - h, q.y - 0.f, q.z - 0.f}),
+                  iso_sample<FMT>(a, s_lut, gpg, f3{q.x + 0.f, q.y + h, q.z + 0.f}) -
+                      iso_sample<FMT>(a, s_lut, gpg, f3{q.x - 0.f, q.y - h, q.z - 0.f}),
+                  iso_sample<FMT>(a, s_lut, gpg, f3{q.x + 0.f, q.y + 0.f, q.z + h}) -
+                      iso_sample<FMT>(a, s_lut, gpg, f3{q.x - 0.f, q.y - 0.f, q.z - h})};
+    nn = muls(g, sprev);
+  }
+  float wp4[4], n4[4];
+  mat4_vec4(a.to_world, inter_p.x, inter_p.y, inter_p.z, 1.f, wp4);
+  mat4_vec4(a.to_world, nn.x, nn.y, nn.z, 0.f, n4);
+  const f3 world_p = {wp4[0], wp4[1], wp4[2]};
+  const f3 nrm = normalize3(f3{n4[0], n4[1], n4[2]});
+  const f3 light_dir = normalize3(sub(a.light_pos, world_p));
+  const f3 hv = a.corrected ? normalize3(add(light_dir, normalize3(sub(a.eye_pos, world_p))))
+                            : normalize3(sub(light_dir, world_p));
+  const float ambient = 0.2f;
+  const float diffuse = .6f * gclamp(dot3(light_dir, nrm), 0.f, 1.f);
+  const float specular = .2f * powf(gclamp(dot3(hv, nrm), 0.f, 1.f), 100.f);
+  const float dist = length3(sub(world_p, a.eye_pos)) / 2.f;
+  if (a.mode == HYD_ISO_COLOR) {
+    const float sc = ambient + (diffuse + specular) / dist;
+    v = make_float4(v.x * sc, v.y * sc, v.z * sc, v.w * sc);
+    v = make_float4(iso_srgb(v.x), iso_srgb(v.y), iso_srgb(v.z), v.w);
+  } else if (a.mode == HYD_ISO_POSITION) {
+    v = make_float4(inter_p.x / a.bmax.x, inter_p.y / a.bmax.y, inter_p.z / a.bmax.z, 1.f);
+  } else {
+    v = make_float4(nrm.x, nrm.y, nrm.z, 1.f);
+  }
+  const float depth = a.corrected ? length3(sub(inter_p, origin)) : length3(sub(origin, o));
   hyd_iso_pixel px;
   px.v[0] = v.x; px.v[1] = v.y; px.v[2] = v.z; px.v[3] = v.w;
   px.depth = depth;
-  px.hit = hit;
-  a.out[(size_t)y * a.W + x] = px;
-  if (a.samples && count) atomicAdd(a.samples, (unsigned long long)count);
+  px.hit = 1;
+  *out = px;
 }
 
 }  // namespace
@@ -329,9 +362,17 @@ hipError_t launch_blocks(const BlocksArgs &a, hipStream_t s) {
 hipError_t launch_iso(const IsoArgs &a, int fmt, hipStream_t s) {
   dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile);
   switch (fmt) {
-    case HYD_FMT_U8: hipLaunchKernelGGL(iso_kernel<HYD_FMT_U8>, grid, dim3(64), 0, s, a); break;
-    case HYD_FMT_F16: hipLaunchKernelGGL(iso_kernel<HYD_FMT_F16>, grid, dim3(64), 0, s, a); break;
-    default: hipLaunchKernelGGL(iso_kernel<HYD_FMT_F32>, grid, dim3(64), 0, s, a);
+    case HYD_FMT_U8:
+      hipLaunchKernelGGL(iso_march_kernel<HYD_FMT_U8>, grid, dim3(64), 0, s, a);
+      hipLaunchKernelGGL(iso_hit_kernel<HYD_FMT_U8>, grid, dim3(64), 0, s, a);
+      break;
+    case HYD_FMT_F16:
+      hipLaunchKernelGGL(iso_march_kernel<HYD_FMT_F16>, grid, dim3(64), 0, s, a);
+      hipLaunchKernelGGL(iso_hit_kernel<HYD_FMT_F16>, grid, dim3(64), 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL(iso_march_kernel<HYD_FMT_F32>, grid, dim3(64), 0, s, a);
+      hipLaunchKernelGGL(iso_hit_kernel<HYD_FMT_F32>, grid, dim3(64), 0, s, a);
   }
   return hipGetLastError();
 }
