@@ -125,27 +125,33 @@ bool env_is_1(const char *name) {
 int32_t ensure_oriented(Ctx *c) {
   if (c->oriented_gen == c->pool_gen) return HYD_OK;
   const size_t bytes = layout_bytes(c);
-  c->oriented_gen = c->pool_gen;
+  // until the copies below are complete the context holds none (a failed
+  // launch leaves the generation stale, so the next launch tries again)
   c->oriented_pairs = false;
+  bool pairs = false;
   if (env_is_1("HYD_BATCH_SINGLE_LAYOUT")) {
     if (c->own_oriented) (void)hipFree(c->own_oriented);
     c->own_oriented = nullptr;
     c->oriented_cap = 0;
+    c->oriented_gen = c->pool_gen;
     return HYD_OK;
   }
   const bool pairs_fit = c->pool.fmt == HYD_FMT_F16 && 2 * bytes <= (size_t)UINT32_MAX;
   if (pairs_fit && !env_is_1("HYD_BATCH_NO_PAIRS") && reserve_oriented(c, 6 * bytes)) {
     HYD_CHECK_HIP(launch_orient_pairs(c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages, c->n_cus,
                                       c->stream));
-    c->oriented_pairs = true;
+    pairs = true;
   } else if (reserve_oriented(c, 2 * bytes)) {
     HYD_CHECK_HIP(launch_orient_pool(c->pool.fmt, c->pool.data, c->own_oriented,
                                      static_cast<char *>(c->own_oriented) + bytes, c->pool.pad, c->pool.n_pages,
                                      c->n_cus, c->stream));
   } else {
+    c->oriented_gen = c->pool_gen;   // no memory for copies: the pool alone
     return HYD_OK;
   }
   HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
+  c->oriented_pairs = pairs;
+  c->oriented_gen = c->pool_gen;
   return HYD_OK;
 }
 
