@@ -175,6 +175,7 @@ int32_t hyd_upload_pool(hyd_ctx *c, int32_t fmt, int32_t pad_edge, int32_t n_pag
   if (st) return st;
   c->pool = Pool{c->own_pool, fmt, pad_edge, n_pages, true};
   ++c->pool_gen;
+  ++c->batch_state_gen;
   return HYD_OK;
 }
 
@@ -186,6 +187,7 @@ int32_t hyd_bind_pool_device(hyd_ctx *c, int32_t fmt, int32_t pad_edge, int32_t 
   free_dev(c->own_pool);
   c->pool = Pool{pool_dev, fmt, pad_edge, n_pages, false};
   ++c->pool_gen;
+  ++c->batch_state_gen;
   return HYD_OK;
 }
 
@@ -249,6 +251,7 @@ int32_t hyd_set_tf(hyd_ctx *c, int32_t n, const float *rgba) {
   if (st) return st;
   c->tf = c->own_tf;
   c->tf_n = n;
+  ++c->batch_state_gen;
   return HYD_OK;
 }
 

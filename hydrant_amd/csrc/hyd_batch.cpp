@@ -48,6 +48,7 @@ struct hyd_batch {
   // captured graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  uint64_t captured_gen = 0;   // the context's batch_state_gen the graph's pointers belong to
   hipStream_t capture_stream = nullptr;
 };
 
@@ -247,6 +248,7 @@ int32_t hyd_set_volumes(hyd_ctx *c, int32_t n, const hyd_volume_desc *vols) {
   c->own_vols = nullptr;
   c->own_cells = nullptr;
   c->n_vols = 0;
+  ++c->batch_state_gen;
   void *pv = nullptr;
   if ((st = dev_upload(&pv, dv.data(), dv.size() * sizeof(BatchVolDev), c->stream))) return st;
   c->own_vols = pv;
@@ -345,6 +347,7 @@ int32_t hyd_batch_capture(hyd_ctx *c, hyd_batch *b, const hyd_batch_config *cfg,
   HYD_CHECK_HIP(e2);
   b->graph = g;
   HYD_CHECK_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->captured_gen = c->batch_state_gen;
   return HYD_OK;
 }
 
@@ -352,6 +355,9 @@ int32_t hyd_batch_replay(hyd_ctx *c, hyd_batch *b) {
   int32_t st = activate(c);
   if (st) return st;
   HYD_REQUIRE(b && b->exec, HYD_ERR_STATE, "no captured graph: call hyd_batch_capture first");
+  // the graph holds device pointers of the state it was captured with
+  HYD_REQUIRE(b->captured_gen == c->batch_state_gen, HYD_ERR_STATE,
+              "pool, transfer function or volume set changed since the capture: call hyd_batch_capture again");
   HYD_CHECK_HIP(hipGraphLaunch(b->exec, c->stream));
   return HYD_OK;
 }

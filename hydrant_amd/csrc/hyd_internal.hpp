@@ -71,6 +71,9 @@ struct Ctx {
   const float *tf = nullptr;
   int32_t tf_n = 0;
   uint64_t pool_gen = 0;          // bumped by every pool upload / bind
+  // bumped whenever device memory a captured batch graph reads may move
+  // (pool upload / bind, transfer function, volume set)
+  uint64_t batch_state_gen = 0;
   // owned device allocations (freed at destroy)
   void *own_pool = nullptr;
   void *own_oriented = nullptr;   // the pool's y- and z-fastest layouts (batched renderer)

@@ -311,7 +311,11 @@ int32_t hyd_batch_render(hyd_ctx *ctx, hyd_batch *batch, const hyd_batch_config 
 /* Record that launch into a HIP graph owned by the batch (no memset nodes). */
 int32_t hyd_batch_capture(hyd_ctx *ctx, hyd_batch *batch, const hyd_batch_config *cfg, float density, float ert,
                           float *rgba_dev);
-/* Replay the captured graph on the context's stream. */
+/* Replay the captured graph on the context's stream.  The graph holds the
+ * device pointers of the pool, transfer function and volume set it was
+ * captured with: after hyd_upload_pool, hyd_bind_pool_device, hyd_set_tf or
+ * hyd_set_volumes, replay returns HYD_ERR_STATE until the batch is captured
+ * again. */
 int32_t hyd_batch_replay(hyd_ctx *ctx, hyd_batch *batch);
 /* Time every launch shape (tile_w x unroll x waves_per_cu) with `reps`
  * launches each and return the fastest; writes the output buffer. */

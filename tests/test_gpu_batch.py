@@ -108,6 +108,34 @@ def test_graph_replay_matches_eager(gpu_ctx):
     b.close()
 
 
+def test_replay_refused_after_state_change(gpu_ctx):
+    """A captured graph holds the device pointers of the pool, transfer function and
+    volume set: after any of them is replaced, replay is refused until the batch is
+    captured again, and the new capture replays the eager image."""
+    import torch
+
+    wl = _small(L.FMT_F16, n_views=3)
+    b = _setup(gpu_ctx, wl)
+    cfg = batch_config(8, 4, 8)
+    out = b.new_output()
+    b.capture(cfg, wl.density, wl.ert, out)
+    b.replay()
+    for change in (lambda: gpu_ctx.set_tf(wl.tf),
+                   lambda: gpu_ctx.set_volumes(wl.volume_descs()),
+                   lambda: gpu_ctx.upload_pool(wl.fmt, wl.bs + 2 * wl.apron, wl.pool)):
+        change()
+        with pytest.raises(L.HydError) as e:
+            b.replay()
+        assert e.value.code == L.HYD_ERR_STATE
+        b.capture(cfg, wl.density, wl.ert, out)
+        out.fill_(-1.0)
+        b.replay()
+        ref, _ = b.render(cfg, wl.density, wl.ert)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref)
+    b.close()
+
+
 def test_batch_rejects_bad_config(gpu_ctx):
     import ctypes as C
 
