@@ -45,6 +45,7 @@ struct hyd_batch {
   std::vector<int32_t> chunks[kPlans];
   unsigned *d_counters = nullptr;
   int64_t total_px = 0;
+  int32_t max_volume = 0;   // largest volume index the views name
   // captured graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -162,6 +163,8 @@ int32_t fill_args(Ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float densi
               "invalid batch config (tile_w 8/16/32, unroll 1/2/4, waves_per_cu 1..32, group 1 or 4 with tile_w 8)");
   HYD_REQUIRE(rgba_dev, HYD_ERR_ARG, "null output");
   HYD_REQUIRE(c->own_cells && c->n_vols > 0, HYD_ERR_STATE, "no volume set: call hyd_set_volumes first");
+  HYD_REQUIRE(b->max_volume < c->n_vols, HYD_ERR_STATE,
+              "the batch's views name volume %d but the volume set holds %d volumes", b->max_volume, c->n_vols);
   HYD_REQUIRE(c->pool.data && c->tf, HYD_ERR_STATE, "batched render needs a pool and a transfer function");
   HYD_REQUIRE(c->pool.pad == c->vol_pad, HYD_ERR_STATE, "pool pad edge %d differs from the volume set's %d",
               c->pool.pad, c->vol_pad);
@@ -283,6 +286,7 @@ int32_t hyd_batch_create(hyd_ctx *c, const hyd_batch_view *views, int32_t n_view
     dv[i].volume = v.volume;
     dv[i].out_offset = v.out_offset;
     b->total_px = std::max<int64_t>(b->total_px, v.out_offset + (int64_t)v.width * v.height);
+    b->max_volume = std::max(b->max_volume, v.volume);
   }
   void *p = nullptr;
   if ((st = dev_upload(&p, dv.data(), dv.size() * sizeof(BatchViewDev), c->stream))) { delete b; return st; }

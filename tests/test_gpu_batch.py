@@ -136,6 +136,19 @@ def test_replay_refused_after_state_change(gpu_ctx):
     b.close()
 
 
+def test_batch_refuses_views_beyond_the_volume_set(gpu_ctx):
+    """A batch whose views name a volume that a later volume set no longer holds is
+    refused before any launch."""
+    wl = _small(L.FMT_F16, n_views=6)
+    b = _setup(gpu_ctx, wl)
+    assert max(v["volume"] for v in wl.views) > 0
+    gpu_ctx.set_volumes(wl.volume_descs()[:1])
+    with pytest.raises(L.HydError) as e:
+        b.render(batch_config(8, 4, 8), wl.density, wl.ert)
+    assert e.value.code == L.HYD_ERR_STATE
+    b.close()
+
+
 def test_batch_rejects_bad_config(gpu_ctx):
     import ctypes as C
 
