@@ -163,6 +163,8 @@ int32_t fill_args(Ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float densi
               "invalid batch config (tile_w 8/16/32, unroll 1/2/4, waves_per_cu 1..32, group 1 or 4 with tile_w 8)");
   HYD_REQUIRE(rgba_dev, HYD_ERR_ARG, "null output");
   HYD_REQUIRE(c->own_cells && c->n_vols > 0, HYD_ERR_STATE, "no volume set: call hyd_set_volumes first");
+  HYD_REQUIRE(c->vol_max_page < c->pool.n_pages, HYD_ERR_STATE,
+              "the volume set names page %d but the pool holds %d pages", c->vol_max_page, c->pool.n_pages);
   HYD_REQUIRE(b->max_volume < c->n_vols, HYD_ERR_STATE,
               "the batch's views name volume %d but the volume set holds %d volumes", b->max_volume, c->n_vols);
   HYD_REQUIRE(c->pool.data && c->tf, HYD_ERR_STATE, "batched render needs a pool and a transfer function");
@@ -218,7 +220,7 @@ int32_t hyd_set_volumes(hyd_ctx *c, int32_t n, const hyd_volume_desc *vols) {
   HYD_REQUIRE(n >= 1 && vols, HYD_ERR_ARG, "empty volume set");
   std::vector<BatchVolDev> dv(n);
   std::vector<int32_t> cells;
-  int pad = -1;
+  int pad = -1, max_page = -1;
   for (int v = 0; v < n; ++v) {
     const hyd_volume_desc &d = vols[v];
     HYD_REQUIRE(d.grid[0] > 0 && d.grid[1] > 0 && d.grid[2] > 0 && d.vaddr && d.chebyshev, HYD_ERR_ARG,
@@ -243,6 +245,7 @@ int32_t hyd_set_volumes(hyd_ctx *c, int32_t n, const hyd_volume_desc *vols) {
       HYD_REQUIRE(cd >= 0 && pg >= -1, HYD_ERR_RANGE, "volume %d cell %zu: invalid chebyshev/vaddr", v, i);
       if (cd == 0 && c->pool.data)
         HYD_REQUIRE(pg < c->pool.n_pages, HYD_ERR_RANGE, "volume %d cell %zu: page %d beyond the pool", v, i, pg);
+      if (cd == 0) max_page = std::max(max_page, pg);
       cells.push_back(cd > 0 ? -1 - cd : pg);
     }
   }
@@ -260,6 +263,7 @@ int32_t hyd_set_volumes(hyd_ctx *c, int32_t n, const hyd_volume_desc *vols) {
   c->own_cells = static_cast<int32_t *>(pcells);
   c->n_vols = n;
   c->vol_pad = pad;
+  c->vol_max_page = max_page;
   return HYD_OK;
 }
 

@@ -92,6 +92,7 @@ struct Ctx {
   int32_t *own_cells = nullptr;   // concatenated cell codes
   int32_t n_vols = 0;
   int32_t vol_pad = 0;            // padded brick edge shared by every volume
+  int32_t vol_max_page = -1;   // largest page a cell of the volume set names (-1: none)
   int32_t n_cus = 0;
   // differentiable renderer: device copy of the views
   void *own_diff_views = nullptr;

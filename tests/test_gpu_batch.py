@@ -149,6 +149,18 @@ def test_batch_refuses_views_beyond_the_volume_set(gpu_ctx):
     b.close()
 
 
+def test_batch_refuses_a_pool_smaller_than_the_volume_set(gpu_ctx):
+    """A pool uploaded after the volume set with fewer pages than its cells name is
+    refused before any launch."""
+    wl = _small(L.FMT_F16, n_views=2)
+    b = _setup(gpu_ctx, wl)
+    gpu_ctx.upload_pool(wl.fmt, wl.bs + 2 * wl.apron, np.ascontiguousarray(wl.pool[:1]))
+    with pytest.raises(L.HydError) as e:
+        b.render(batch_config(8, 4, 8), wl.density, wl.ert)
+    assert e.value.code == L.HYD_ERR_STATE
+    b.close()
+
+
 def test_batch_rejects_bad_config(gpu_ctx):
     import ctypes as C
 
