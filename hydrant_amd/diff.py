@@ -32,7 +32,16 @@ class DiffRenderer:
         L.call("hyd_diff_ckpt_floats", C.byref(p), C.byref(nck))
         self.nck = nck.value
 
+    def _check_inputs(self, volume, tf):
+        n_vox = int(np.prod([int(d) for d in self.params.dims]))
+        if volume.numel() != n_vox or not volume.is_contiguous():
+            raise ValueError(f"volume must be a contiguous tensor of {n_vox} voxels "
+                             f"(dims {list(self.params.dims)}); got {tuple(volume.shape)}")
+        if tf.dim() != 2 or tf.shape[1] != 4 or not 1 <= tf.shape[0] <= 256 or not tf.is_contiguous():
+            raise ValueError(f"tf must be a contiguous (n, 4) tensor with 1 <= n <= 256; got {tuple(tf.shape)}")
+
     def forward_raw(self, volume, tf, count: bool = False):
+        self._check_inputs(volume, tf)
         t = self.ctx.torch
         self.ctx.sync_stream()
         rgba = t.empty((self.n_views, self.H, self.W, 4), dtype=t.float32, device=self.ctx.dev)
@@ -44,6 +53,13 @@ class DiffRenderer:
         return rgba, ck, cnt
 
     def backward_raw(self, volume, tf, ck, grad_rgba):
+        self._check_inputs(volume, tf)
+        if tuple(ck.shape) != (self.n_views, self.H, self.W, self.nck) or not ck.is_contiguous():
+            raise ValueError(f"checkpoints must be a contiguous {(self.n_views, self.H, self.W, self.nck)} tensor; "
+                             f"got {tuple(ck.shape)}")
+        if tuple(grad_rgba.shape) != (self.n_views, self.H, self.W, 4):
+            raise ValueError(f"grad_rgba must have shape {(self.n_views, self.H, self.W, 4)}; "
+                             f"got {tuple(grad_rgba.shape)}")
         t = self.ctx.torch
         self.ctx.sync_stream()
         gv = t.empty_like(volume)   # both outputs are overwritten by the adjoint

@@ -68,3 +68,4 @@ def test_fp32_forward_close_to_fp64():
     vol, tf, view, xy, wts, K = _setup(seed=3)
     sc = O.DiffScene(6, 5, vol, tf, K=K, sigma=8.0)
     assert np.max(np.abs(sc.render(view, xy, prec=32) - sc.render(view, xy, prec=64))) < 1e-5
+

@@ -1,7 +1,9 @@
-"""The C-ABI library loads and exports every symbol the public headers declare
-(no compute calls: this runs without a GPU)."""
+"""The C-ABI library loads and exports every symbol the public headers declare, and
+the host wrappers check their arguments (no compute calls: this runs without a GPU)."""
 import re
 from pathlib import Path
+
+import pytest
 
 from hydrant_amd import _lib as L
 
@@ -44,3 +46,29 @@ def test_host_only_calls_without_gpu():
     assert L.lib().hyd_version() == 1
     code = L.lib().hyd_orbit_camera(None, None, 60.0, None)
     assert code == L.HYD_ERR_ARG and b"null" in L.lib().hyd_last_error()
+
+
+def test_diff_wrapper_rejects_mismatched_tensors():
+    """The differentiable wrapper checks tensor sizes against its parameters before any
+    device call (a short volume, a malformed TF, or wrong checkpoint / gradient shapes
+    would otherwise reach the kernels)."""
+    import torch
+
+    from hydrant_amd import _lib as L
+    from hydrant_amd.diff import DiffRenderer
+
+    view = L.View()
+    r = DiffRenderer(object(), [view, view], 16, 8, (4, 5, 6), n_steps=32, ckpt_every=8)
+    vol, tf = torch.zeros(6, 5, 4), torch.zeros(16, 4)
+    ck = torch.zeros(2, 8, 16, r.nck)
+    g = torch.zeros(2, 8, 16, 4)
+    with pytest.raises(ValueError, match="volume"):
+        r.forward_raw(torch.zeros(6, 5, 3), tf)
+    with pytest.raises(ValueError, match="tf"):
+        r.forward_raw(vol, torch.zeros(16, 3))
+    with pytest.raises(ValueError, match="tf"):
+        r.forward_raw(vol, torch.zeros(300, 4))
+    with pytest.raises(ValueError, match="checkpoints"):
+        r.backward_raw(vol, tf, ck[:, :, :8], g)
+    with pytest.raises(ValueError, match="grad_rgba"):
+        r.backward_raw(vol, tf, ck, g[:1])
