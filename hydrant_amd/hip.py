@@ -259,6 +259,18 @@ class Context:
         return Batch(self, views)
 
 
+def check_batch_output(out, n_pixels: int):
+    """The batched kernel writes RGBA floats for n_pixels pixels straight into `out`:
+    refuse a buffer that cannot hold them."""
+    import torch
+
+    if out.dtype != torch.float32 or not out.is_contiguous() or out.numel() < 4 * n_pixels:
+        raise ValueError(f"batched output must be a contiguous float32 tensor of at least {4 * n_pixels} "
+                         f"floats ({n_pixels} RGBA pixels); got {out.dtype} {tuple(out.shape)}")
+    if not out.is_cuda:
+        raise ValueError("batched output must be a device tensor")
+
+
 class Batch:
     """A batched-render plan: many views, one persistent launch (optionally a HIP graph)."""
 
@@ -297,12 +309,14 @@ class Batch:
         t = self.ctx.torch
         self.ctx.sync_stream()
         out = self.new_output() if out is None else out
+        check_batch_output(out, self.n_pixels)
         cnt = t.zeros(1, dtype=t.int64, device=self.ctx.dev) if count else None
         L.call("hyd_batch_render", self.ctx.h, self.h, C.byref(cfg), float(density), float(ert),
                C.c_void_p(out.data_ptr()), C.c_void_p(cnt.data_ptr() if count else 0))
         return out, cnt
 
     def capture(self, cfg: L.BatchConfig, density: float, ert: float, out):
+        check_batch_output(out, self.n_pixels)
         self.ctx.sync_stream()
         L.call("hyd_batch_capture", self.ctx.h, self.h, C.byref(cfg), float(density), float(ert),
                C.c_void_p(out.data_ptr()))
@@ -312,6 +326,7 @@ class Batch:
         L.call("hyd_batch_replay", self.ctx.h, self.h)
 
     def autotune(self, density: float, ert: float, out, reps: int = 2):
+        check_batch_output(out, self.n_pixels)
         self.ctx.sync_stream()
         best = L.BatchConfig()
         ms = C.c_float(0)

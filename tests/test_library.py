@@ -72,3 +72,18 @@ def test_diff_wrapper_rejects_mismatched_tensors():
         r.backward_raw(vol, tf, ck[:, :, :8], g)
     with pytest.raises(ValueError, match="grad_rgba"):
         r.backward_raw(vol, tf, ck, g[:1])
+
+
+def test_batch_output_check():
+    """The batched wrapper refuses an output buffer too small, of the wrong type or in host
+    memory before it reaches the kernel."""
+    import torch
+
+    from hydrant_amd.hip import check_batch_output
+
+    with pytest.raises(ValueError, match="at least 400"):
+        check_batch_output(torch.zeros(99, 4), 100)
+    with pytest.raises(ValueError, match="float32"):
+        check_batch_output(torch.zeros(100, 4, dtype=torch.float64), 100)
+    with pytest.raises(ValueError, match="device"):
+        check_batch_output(torch.zeros(100, 4), 100)
