@@ -139,6 +139,8 @@ class Context:
         t = self.torch
         self.sync_stream()
         film_t = t.empty((H, W, 16), dtype=t.float32, device=self.dev) if film else None
+        if rgba is not None:
+            check_output(rgba, H * W * 4, "volume output")
         rgba_t = rgba if rgba is not None else t.empty((H, W, 4), dtype=t.float32, device=self.dev)
         cnt = t.zeros(1, dtype=t.int64, device=self.dev) if count else None
         L.call("hyd_render_volume", self.h, C.byref(params), C.byref(view), W, H,
@@ -227,6 +229,8 @@ class Context:
     def render_iso(self, params: L.IsoParams, view: L.View, W: int, H: int, count: bool = False, px=None):
         t = self.torch
         self.sync_stream()
+        if px is not None:
+            check_output(px, H * W * 6, "isosurface output")
         out = px if px is not None else t.empty((H, W, 6), dtype=t.float32, device=self.dev)
         cnt = t.zeros(1, dtype=t.int64, device=self.dev) if count else None
         L.call("hyd_render_iso", self.h, C.byref(params), C.byref(view), W, H, C.c_void_p(out.data_ptr()),
@@ -259,16 +263,16 @@ class Context:
         return Batch(self, views)
 
 
-def check_batch_output(out, n_pixels: int):
-    """The batched kernel writes RGBA floats for n_pixels pixels straight into `out`:
-    refuse a buffer that cannot hold them."""
+def check_output(out, n_floats: int, what: str):
+    """Kernels write n_floats floats straight into a caller's `out`: refuse a buffer that
+    cannot hold them."""
     import torch
 
-    if out.dtype != torch.float32 or not out.is_contiguous() or out.numel() < 4 * n_pixels:
-        raise ValueError(f"batched output must be a contiguous float32 tensor of at least {4 * n_pixels} "
-                         f"floats ({n_pixels} RGBA pixels); got {out.dtype} {tuple(out.shape)}")
+    if out.dtype != torch.float32 or not out.is_contiguous() or out.numel() < n_floats:
+        raise ValueError(f"{what} must be a contiguous float32 tensor of at least {n_floats} floats; "
+                         f"got {out.dtype} {tuple(out.shape)}")
     if not out.is_cuda:
-        raise ValueError("batched output must be a device tensor")
+        raise ValueError(f"{what} must be a device tensor")
 
 
 class Batch:
@@ -309,14 +313,14 @@ class Batch:
         t = self.ctx.torch
         self.ctx.sync_stream()
         out = self.new_output() if out is None else out
-        check_batch_output(out, self.n_pixels)
+        check_output(out, 4 * self.n_pixels, "batched output")
         cnt = t.zeros(1, dtype=t.int64, device=self.ctx.dev) if count else None
         L.call("hyd_batch_render", self.ctx.h, self.h, C.byref(cfg), float(density), float(ert),
                C.c_void_p(out.data_ptr()), C.c_void_p(cnt.data_ptr() if count else 0))
         return out, cnt
 
     def capture(self, cfg: L.BatchConfig, density: float, ert: float, out):
-        check_batch_output(out, self.n_pixels)
+        check_output(out, 4 * self.n_pixels, "batched output")
         self.ctx.sync_stream()
         L.call("hyd_batch_capture", self.ctx.h, self.h, C.byref(cfg), float(density), float(ert),
                C.c_void_p(out.data_ptr()))
@@ -326,7 +330,7 @@ class Batch:
         L.call("hyd_batch_replay", self.ctx.h, self.h)
 
     def autotune(self, density: float, ert: float, out, reps: int = 2):
-        check_batch_output(out, self.n_pixels)
+        check_output(out, 4 * self.n_pixels, "batched output")
         self.ctx.sync_stream()
         best = L.BatchConfig()
         ms = C.c_float(0)

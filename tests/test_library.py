@@ -74,16 +74,16 @@ def test_diff_wrapper_rejects_mismatched_tensors():
         r.backward_raw(vol, tf, ck, g[:1])
 
 
-def test_batch_output_check():
-    """The batched wrapper refuses an output buffer too small, of the wrong type or in host
-    memory before it reaches the kernel."""
+def test_output_check():
+    """The wrappers refuse a caller's output buffer that is too small, of the wrong type or
+    in host memory before it reaches a kernel."""
     import torch
 
-    from hydrant_amd.hip import check_batch_output
+    from hydrant_amd.hip import check_output
 
     with pytest.raises(ValueError, match="at least 400"):
-        check_batch_output(torch.zeros(99, 4), 100)
+        check_output(torch.zeros(99, 4), 400, "batched output")
     with pytest.raises(ValueError, match="float32"):
-        check_batch_output(torch.zeros(100, 4, dtype=torch.float64), 100)
+        check_output(torch.zeros(100, 4, dtype=torch.float64), 400, "batched output")
     with pytest.raises(ValueError, match="device"):
-        check_batch_output(torch.zeros(100, 4), 100)
+        check_output(torch.zeros(100, 4), 400, "batched output")
