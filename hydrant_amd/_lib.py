@@ -119,6 +119,7 @@ SIGNATURES = {
     "hyd_mat4_inverse": (I32, [PF, PF]),
     "hyd_upload_pool": (I32, [P, I32, I32, I32, P]),
     "hyd_bind_pool_device": (I32, [P, I32, I32, I32, P]),
+    "hyd_write_pool_pages": (I32, [P, I32, I32, P]),
     "hyd_set_page_table": (I32, [P, PI32, PI32, I32, PF, PF, I32]),
     "hyd_set_empty_grid": (I32, [P, PI32, PI32]),
     "hyd_set_mean_grid": (I32, [P, PI32, PF]),
@@ -159,6 +160,14 @@ SIGNATURES = {
     "hyd_tf_alpha_zero_on": (I32, [PF, I32, C.c_float, C.c_float]),
     "hyd_chebyshev": (I32, [PI32, C.POINTER(C.c_uint8), PI32]),
     "hyd_seeded_page_table": (I32, [PI32, C.POINTER(C.c_uint8), I32, C.c_uint64, PI32, PI32, PI32]),
+    # hydrant_residency.h
+    "hyd_cull_bricks": (I32, [C.POINTER(View), PI32, PI32, PI32, PI32, C.c_int64, PI32, C.POINTER(C.c_int64)]),
+    "hyd_residency_create": (I32, [PI32, I32, I32, PI32, C.POINTER(P)]),
+    "hyd_residency_destroy": (I32, [P]),
+    "hyd_residency_update": (I32, [P, C.POINTER(View), PI32, PI32, C.POINTER(C.c_int64)]),
+    "hyd_residency_commit": (I32, [P, I32, PI32]),
+    "hyd_residency_vaddr": (I32, [P, PI32]),
+    "hyd_residency_stats": (I32, [P, C.POINTER(C.c_int64), PI32]),
     # hydrant_renderer.h
     "hyd_renderer_candidates": (I32, [C.c_char_p, I32, C.POINTER(I32)]),
     "hyd_renderer_create": (I32, [P, C.POINTER(Dataset), C.c_char_p, C.POINTER(P)]),

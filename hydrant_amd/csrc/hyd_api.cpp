@@ -191,6 +191,22 @@ int32_t hyd_bind_pool_device(hyd_ctx *c, int32_t fmt, int32_t pad_edge, int32_t 
   return HYD_OK;
 }
 
+int32_t hyd_write_pool_pages(hyd_ctx *c, int32_t first_page, int32_t n_pages, const void *host) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(c->pool.data, HYD_ERR_STATE, "no brick pool: call hyd_upload_pool or hyd_bind_pool_device first");
+  HYD_REQUIRE(host && n_pages >= 0 && first_page >= 0 && (int64_t)first_page + n_pages <= c->pool.n_pages,
+              HYD_ERR_RANGE, "pages [%d, %d) outside the pool of %d pages", first_page, first_page + n_pages,
+              c->pool.n_pages);
+  const size_t page = (size_t)c->pool.pad * c->pool.pad * c->pool.pad * fmt_bytes(c->pool.fmt);
+  char *dst = static_cast<char *>(const_cast<void *>(c->pool.data)) + (size_t)first_page * page;
+  HYD_CHECK_HIP(hipMemcpyAsync(dst, host, (size_t)n_pages * page, hipMemcpyHostToDevice, c->stream));
+  HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
+  ++c->pool_gen;   // the batched renderer's oriented copies are stale
+  ++c->batch_state_gen;
+  return HYD_OK;
+}
+
 int32_t hyd_set_page_table(hyd_ctx *c, const int32_t dims[3], const int32_t *vaddr, int32_t n_pages,
                            const float *page_k, const float *page_b, int32_t lowest_blkcnt) {
   int32_t st = activate(c);

@@ -99,6 +99,11 @@ class Context:
         pool = np.ascontiguousarray(pool)
         L.call("hyd_upload_pool", self.h, fmt, pad, pool.shape[0], pool.ctypes.data)
 
+    def write_pool_pages(self, first_page: int, pages: np.ndarray):
+        """Overwrite pool pages [first_page, first_page + len(pages)) with host bricks."""
+        pages = np.ascontiguousarray(pages)
+        L.call("hyd_write_pool_pages", self.h, int(first_page), pages.shape[0], pages.ctypes.data)
+
     def bind_pool_device(self, fmt: int, pad: int, n_pages: int, ptr: int):
         L.call("hyd_bind_pool_device", self.h, fmt, pad, n_pages, C.c_void_p(ptr))
 

@@ -1,0 +1,193 @@
+"""Frustum culling and realtime residency (host code of the library) against the numpy
+restatement in oracle/culler.py, whose geometry is pinned here by properties that do
+not restate it: points on rays through the image lie inside the frustum, bricks deep
+inside the frustum are kept, a volume entirely in view is kept whole, and `limit`
+keeps the nearest bricks."""
+import numpy as np
+import pytest
+
+from hydrant_amd import _lib as L
+from hydrant_amd import dataset as D
+from hydrant_amd.hip import make_view, orbit_camera
+from hydrant_amd.residency import Residency, cull_bricks
+from oracle import culler as OC
+
+ARMS = [(25.0, 65.0, 2.2), (200.0, -30.0, 1.6), (90.0, 10.0, 0.9), (310.0, 80.0, 3.0), (0.0, 0.0, 0.45)]
+
+
+def _grid(n=10, seed=3, frac_empty=0.35):
+    rng = np.random.default_rng(seed)
+    return (rng.random((n, n, n)) < frac_empty).astype(np.int32)   # [z][y][x], nonzero = empty
+
+
+def _view(arm, n, fovy=60.0):
+    g = D.geometry((8 * n,) * 3, 8)
+    return make_view(orbit_camera(arm, (0.0, 0.0, 0.0), fovy), g.center, g.f_dim)
+
+
+def _trans(v):
+    return np.array(v.trans[:], np.float32)
+
+
+def _dir(v, u, w):
+    """World direction of the camera-space ray (u, w, -ctg)."""
+    return np.array(OC.mat4_vec4(_trans(v), u, w, -v.ctg_fovy_2, 0.0)[:3], np.float64)
+
+
+@pytest.mark.parametrize("arm", ARMS)
+def test_cull_matches_restatement(arm):
+    n = 10
+    cheb = _grid(n)
+    v = _view(arm, n)
+    got = cull_bricks(v, cheb)
+    ref, _ = OC.cull(_trans(v), v.ctg_fovy_2, cheb)
+    assert np.array_equal(got, ref)
+    for limit in (0, 1, 17, 10 ** 6):
+        ref_l, _ = OC.cull(_trans(v), v.ctg_fovy_2, cheb, limit=limit)
+        assert np.array_equal(cull_bricks(v, cheb, limit=limit), ref_l)
+    box = ((2, 1, 3), (9, 7, 10))
+    ref_b, _ = OC.cull(_trans(v), v.ctg_fovy_2, cheb, bbox=box)
+    assert np.array_equal(cull_bricks(v, cheb, bbox=box), ref_b)
+
+
+@pytest.mark.parametrize("arm", ARMS)
+def test_restated_frustum_geometry(arm):
+    """Points along rays through the image's interior are inside; points along rays well
+    outside the screen rect are not."""
+    v = _view(arm, 10)
+    f = OC.Frustum(_trans(v), v.ctg_fovy_2)
+    o = np.array(f.orig, np.float64)
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        u, w = rng.uniform(-0.95, 0.95, 2)
+        t = rng.uniform(0.1, 20.0)
+        assert f.contains(o + t * _dir(v, u, w))
+        u_out = rng.choice([-1, 1]) * rng.uniform(1.1, 3.0)
+        assert not f.contains(o + t * _dir(v, u_out, w))
+        assert not f.contains(o + t * _dir(v, w, u_out))
+
+
+@pytest.mark.parametrize("arm", ARMS)
+def test_restated_cull_keeps_bricks_deep_inside(arm):
+    """A non-empty brick whose centre lies more than its half-diagonal inside all four
+    planes has every corner inside, so the cull keeps it; empty bricks never appear."""
+    n = 10
+    cheb = _grid(n)
+    v = _view(arm, n)
+    kept, f = OC.cull(_trans(v), v.ctg_fovy_2, cheb)
+    kept = set(kept.tolist())
+    o = np.array(f.orig, np.float64)
+    norms = [np.array(m, np.float64) / np.linalg.norm(m) for m in f.norm]
+    for z in range(n):
+        for y in range(n):
+            for x in range(n):
+                i = (z * n + y) * n + x
+                if cheb[z, y, x]:
+                    assert i not in kept
+                    continue
+                c = np.array([x + 0.5, y + 0.5, z + 0.5]) - o
+                if min(float(m @ c) for m in norms) > 0.9:
+                    assert i in kept, (x, y, z)
+
+
+def test_restated_cull_keeps_a_volume_in_full_view():
+    n = 8
+    cheb = _grid(n, seed=9)
+    v = _view((40.0, 30.0, 6.0), n, fovy=60.0)
+    kept, _ = OC.cull(_trans(v), v.ctg_fovy_2, cheb)
+    assert np.array_equal(kept, np.flatnonzero(cheb.ravel() == 0).astype(np.int32))
+
+
+@pytest.mark.parametrize("arm", ARMS[:3])
+def test_restated_limit_keeps_the_nearest(arm):
+    n = 10
+    cheb = _grid(n)
+    v = _view(arm, n)
+    full, f = OC.cull(_trans(v), v.ctg_fovy_2, cheb)
+    o = np.array(f.orig, np.float64)
+
+    def d(i):
+        x, y, z = i % n, (i // n) % n, i // (n * n)
+        return float(np.sum((np.array([x + 0.5, y + 0.5, z + 0.5]) - o) ** 2))
+
+    for k in (1, 9, len(full) // 2):
+        part, _ = OC.cull(_trans(v), v.ctg_fovy_2, cheb, limit=k)
+        assert len(part) == k and set(part) <= set(full)
+        assert max(d(i) for i in part) <= min(d(i) for i in set(full) - set(part)) + 1e-4
+
+
+def _dist(f, n):
+    return lambda i: OC.centre_dist2(f.orig, i % n, (i // n) % n, i // (n * n))
+
+
+def _check_invariants(res, model, first, max_pages):
+    va = res.vaddr().ravel()
+    assert np.array_equal(va, model.vaddr)
+    slots = [va[i] for i in model.present]
+    assert len(set(slots)) == len(slots)
+    assert all(first <= s < first + max_pages for s in slots)
+    others = np.ones(va.size, bool)
+    others[list(model.present)] = False
+    assert np.array_equal(va[others], model.basic[others])
+    assert res.stats() == (len(model.present), model.n_slots)
+
+
+PATH = [(25.0 + 6 * k, 65.0 - 4 * k, 2.2 - 0.12 * k) for k in range(8)]   # a camera moving a little per frame
+
+
+@pytest.mark.parametrize("max_pages,first", [(40, 0), (25, 7), (120, 2), (1000, 3)])
+def test_residency_follows_the_reference_sets(max_pages, first):
+    """Missing sets (nearest first), slot assignment (new slots, then the last redundant
+    brick's slot with its entry reset to the basic one) and the abandoned / artifact cases
+    follow the reference's bookkeeping, view after view."""
+    n = 10
+    cheb = _grid(n)
+    basic = np.random.default_rng(4).integers(-1, 3, size=cheb.shape).astype(np.int32)
+    res = Residency(cheb.shape, max_pages, first_slot=first, basic_vaddr=basic)
+    model = OC.ResidencyModel(cheb.size, max_pages, first, basic)
+    kept = swapped = 0
+    for arm in PATH + ARMS + ARMS[:2]:
+        v = _view(arm, n)
+        missing = res.update(v, cheb)
+        required, f = OC.cull(_trans(v), v.ctg_fovy_2, cheb, limit=max_pages)
+        want = model.update(required, _dist(f, n))
+        assert missing.tolist() == want
+        # commit every missing brick, one already present brick, and a repeat
+        kept += len(set(required.tolist()) & model.present)
+        extra = [int(next(iter(model.present)))] if model.present else []
+        for i in list(missing) + extra + list(missing[:1]):
+            n_slots = model.n_slots
+            s = model.commit(int(i))
+            swapped += s >= 0 and model.n_slots == n_slots
+            assert res.commit(int(i)) == s
+        _check_invariants(res, model, first, max_pages)
+    assert kept > 0                               # frames share resident bricks
+    assert swapped > 0 or max_pages >= 1000      # and a full pool reuses redundant slots
+    res.close()
+
+
+def test_residency_artifact_when_nothing_can_be_freed():
+    n = 10
+    cheb = _grid(n)
+    res = Residency(cheb.shape, 3)
+    v = _view(ARMS[0], n)
+    missing = res.update(v, cheb)
+    assert len(missing) == 3
+    assert [res.commit(int(i)) for i in missing] == [0, 1, 2]
+    other = int(np.flatnonzero(cheb.ravel() == 0)[-1])
+    if other in missing:
+        other = int(np.flatnonzero(cheb.ravel() == 0)[0])
+    assert res.commit(other) == -1          # pool full, nothing redundant
+    assert res.update(v, cheb).size == 0    # the same view needs nothing new
+    res.close()
+
+
+def test_residency_rejects_bad_arguments():
+    with pytest.raises(L.HydError):
+        Residency((0, 4, 4), 8)
+    res = Residency((4, 4, 4), 8)
+    with pytest.raises(L.HydError):
+        res.commit(64)
+    with pytest.raises(ValueError):
+        res.update(_view(ARMS[0], 4), np.zeros((4, 4, 5), np.int32))
+    res.close()
