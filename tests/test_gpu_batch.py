@@ -279,15 +279,16 @@ def test_batch_rejects_pools_beyond_32_bit_offsets():
     ctx.close()
 
 
-def test_batch_64_bricks_vs_fp64_oracle(monkeypatch):
+@pytest.mark.parametrize("fmt", [L.FMT_F16, L.FMT_U8, L.FMT_F32])
+def test_batch_64_bricks_vs_fp64_oracle(monkeypatch, fmt):
     """64^3 bricks with a one-voxel apron take the kernels compiled for a padded edge of
     66 (the 96-view benchmark's bricks), through each pool layout: row pairs (the FP16
     default), the single-voxel oriented layouts and the x-fastest pool alone.  Error
-    against the FP64 oracle within 1e-3, and the three layouts give the same image."""
+    against the FP64 oracle within 1e-3, and the layouts give the same image."""
     import torch
     from hydrant_amd.hip import Context
 
-    wl = W.batched(sizes=(128, 192), n_views=4, res_choices=(96, 128), fmt=L.FMT_F16, bs=64, apron=1, seed=91)
+    wl = W.batched(sizes=(128, 192), n_views=4, res_choices=(96, 128), fmt=fmt, bs=64, apron=1, seed=91)
     ctx = Context(0)
     b = _setup(ctx, wl)
     ref = None
@@ -298,7 +299,7 @@ def test_batch_64_bricks_vs_fp64_oracle(monkeypatch):
         out, cnt = b.render(batch_config(8, 4, 16), wl.density, wl.ert, count=True)
         torch.cuda.synchronize()
         errs = _view_errors(wl, out, b, 64, exact=True)
-        print(f"pad 66 ({switch}): per-view max-abs vs FP64 {['%.2e' % e for e in errs]}, "
+        print(f"pad 66 fmt {fmt} ({switch}): per-view max-abs vs FP64 {['%.2e' % e for e in errs]}, "
               f"samples {int(cnt.item())}", flush=True)
         assert max(errs) <= 1e-3
         if ref is None:
