@@ -202,9 +202,9 @@ int32_t hyd_write_pool_pages(hyd_ctx *c, int32_t first_page, int32_t n_pages, co
   char *dst = static_cast<char *>(const_cast<void *>(c->pool.data)) + (size_t)first_page * page;
   HYD_CHECK_HIP(hipMemcpyAsync(dst, host, (size_t)n_pages * page, hipMemcpyHostToDevice, c->stream));
   HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
-  ++c->pool_gen;   // the batched renderer's oriented copies are stale
-  ++c->batch_state_gen;
-  return HYD_OK;
+  // the batched renderer's oriented copies of these pages follow (in place, so
+  // captured graphs stay valid)
+  return refresh_oriented_pages(c, first_page, n_pages);
 }
 
 int32_t hyd_set_page_table(hyd_ctx *c, const int32_t dims[3], const int32_t *vaddr, int32_t n_pages,

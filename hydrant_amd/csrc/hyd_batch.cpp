@@ -140,13 +140,13 @@ int32_t ensure_oriented(Ctx *c) {
   }
   const bool pairs_fit = c->pool.fmt == HYD_FMT_F16 && 2 * bytes <= (size_t)UINT32_MAX;
   if (pairs_fit && !env_is_1("HYD_BATCH_NO_PAIRS") && reserve_oriented(c, 6 * bytes)) {
-    HYD_CHECK_HIP(launch_orient_pairs(c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages, c->n_cus,
-                                      c->stream));
+    HYD_CHECK_HIP(launch_orient_pairs(c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages, 0,
+                                      c->pool.n_pages, c->stream));
     pairs = true;
   } else if (reserve_oriented(c, 2 * bytes)) {
     HYD_CHECK_HIP(launch_orient_pool(c->pool.fmt, c->pool.data, c->own_oriented,
-                                     static_cast<char *>(c->own_oriented) + bytes, c->pool.pad, c->pool.n_pages,
-                                     c->n_cus, c->stream));
+                                     static_cast<char *>(c->own_oriented) + bytes, c->pool.pad, 0, c->pool.n_pages,
+                                     c->stream));
   } else {
     c->oriented_gen = c->pool_gen;   // no memory for copies: the pool alone
     return HYD_OK;
@@ -156,6 +156,27 @@ int32_t ensure_oriented(Ctx *c) {
   c->oriented_gen = c->pool_gen;
   return HYD_OK;
 }
+
+}  // namespace
+
+// Pages [first, first + n) of the pool were overwritten in place: bring the
+// current oriented copies up to date for those pages (nothing to do when the
+// copies are stale anyway, or when the renderer reads the pool alone).
+int32_t hyd::refresh_oriented_pages(Ctx *c, int32_t first, int32_t n) {
+  if (c->oriented_gen != c->pool_gen || !c->own_oriented || n <= 0) return HYD_OK;
+  if (c->oriented_pairs) {
+    HYD_CHECK_HIP(launch_orient_pairs(c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages, first, n,
+                                      c->stream));
+  } else {
+    HYD_CHECK_HIP(launch_orient_pool(c->pool.fmt, c->pool.data, c->own_oriented,
+                                     static_cast<char *>(c->own_oriented) + layout_bytes(c), c->pool.pad, first, n,
+                                     c->stream));
+  }
+  HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return HYD_OK;
+}
+
+namespace {
 
 int32_t fill_args(Ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float density, float ert, float *rgba_dev,
                   unsigned long long *samples_dev, BatchArgs &a, int &grid) {

@@ -49,12 +49,15 @@ struct BatchArgs {
 
 hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int group, int grid, hipStream_t s);
 
-// Writes the three FP16 row-pair layouts of an x-fastest FP16 pool (layout p at
-// dst + p * 4 pad^3 n_pages bytes).
-hipError_t launch_orient_pairs(const void *src, void *dst, int pad, int n_pages, int n_cus, hipStream_t s);
+// Writes pages [first_page, first_page + n_pages) of the three FP16 row-pair
+// layouts of an x-fastest FP16 pool of total_pages pages (layout p at dst + p *
+// 4 pad^3 total_pages bytes).
+hipError_t launch_orient_pairs(const void *src, void *dst, int pad, int total_pages, int first_page, int n_pages,
+                               hipStream_t s);
 
-// Writes the y-fastest (dst1) and z-fastest (dst2) layouts of an x-fastest pool.
-hipError_t launch_orient_pool(int fmt, const void *src, void *dst1, void *dst2, int pad, int n_pages, int n_cus,
+// Writes pages [first_page, first_page + n_pages) of the y-fastest (dst1) and
+// z-fastest (dst2) layouts of an x-fastest pool.
+hipError_t launch_orient_pool(int fmt, const void *src, void *dst1, void *dst2, int pad, int first_page, int n_pages,
                               hipStream_t s);
 
 }  // namespace hyd

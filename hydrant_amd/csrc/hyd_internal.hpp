@@ -195,6 +195,9 @@ struct IsoArgs {
 hipError_t launch_iso(const IsoArgs &a, int fmt, hipStream_t s);
 
 int32_t activate(Ctx *c);
+// After pool pages [first, first + n) were overwritten in place: updates the
+// batched renderer's oriented copies of those pages if they are current.
+int32_t refresh_oriented_pages(Ctx *c, int32_t first, int32_t n);
 
 }  // namespace hyd
 

@@ -454,39 +454,44 @@ __global__ __launch_bounds__(64 * GROUP) void batch_kernel(BatchArgs a) {
   }
 }
 
-// The two other layouts of a pool (PERM 1 and 2) from the x-fastest one.
+// The two other layouts of a pool (PERM 1 and 2) from the x-fastest one, for
+// pages first_page + blockIdx.x; workgroup (page, z) copies one z plane.
 template <typename T>
-__global__ __launch_bounds__(256) void orient_pool_kernel(const T *src, T *dst1, T *dst2, int pad, size_t n) {
+__global__ __launch_bounds__(256) void orient_pool_kernel(const T *src, T *dst1, T *dst2, int pad, int first_page) {
+  const size_t page = (size_t)first_page + blockIdx.x;
   const size_t p3 = (size_t)pad * pad * pad;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-    const size_t page = i / p3;
-    const int r = (int)(i - page * p3);
-    const int x = r % pad, y = (r / pad) % pad, z = r / (pad * pad);
-    const T v = src[i];
-    dst1[page * p3 + ((size_t)x * pad + z) * pad + y] = v;
-    dst2[page * p3 + ((size_t)y * pad + x) * pad + z] = v;
+  const int z = blockIdx.y;
+  const T *sp = src + page * p3 + (size_t)z * pad * pad;
+  T *d1 = dst1 + page * p3, *d2 = dst2 + page * p3;
+  for (int r = threadIdx.x; r < pad * pad; r += 256) {
+    const int x = r % pad, y = r / pad;
+    const T v = sp[r];
+    d1[(x * pad + z) * pad + y] = v;
+    d2[(y * pad + x) * pad + z] = v;
   }
 }
 
-// The three FP16 row-pair layouts of an x-fastest pool (layout p at dst + p *
-// 2 n halves): element (a0, a1, a2) = (v(a0, a1, a2), v(a0, min(a1 + 1, pad -
-// 1), a2)) in layout p's axes (p 0: x, y, z; 1: y, z, x; 2: z, x, y).
-__global__ __launch_bounds__(256) void orient_pairs_kernel(const uint16_t *src, uint32_t *dst, int pad, size_t n) {
+// The three FP16 row-pair layouts of an x-fastest pool of total_pages pages
+// (layout p at dst + p * pad^3 * total_pages words): element (a0, a1, a2) =
+// (v(a0, a1, a2), v(a0, min(a1 + 1, pad - 1), a2)) in layout p's axes (p 0:
+// x, y, z; 1: y, z, x; 2: z, x, y).  Workgroup (page, a2, p) writes one plane
+// of page first_page + blockIdx.x.
+__global__ __launch_bounds__(256) void orient_pairs_kernel(const uint16_t *src, uint32_t *dst, int pad,
+                                                           int total_pages, int first_page) {
+  const size_t page = (size_t)first_page + blockIdx.x;
   const size_t p3 = (size_t)pad * pad * pad;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < 3 * n; i += (size_t)gridDim.x * 256) {
-    const int p = (int)(i / n);
-    const size_t e = i - (size_t)p * n;
-    const size_t page = e / p3;
-    const int r = (int)(e - page * p3);
-    const int a0 = r % pad, a1 = (r / pad) % pad, a2 = r / (pad * pad);
+  const int a2 = blockIdx.y, p = blockIdx.z;
+  const uint16_t *pg = src + page * p3;
+  uint32_t *out = dst + (size_t)p * p3 * (size_t)total_pages + page * p3 + (size_t)a2 * pad * pad;
+  for (int r = threadIdx.x; r < pad * pad; r += 256) {
+    const int a0 = r % pad, a1 = r / pad;
     const int a1n = a1 + 1 < pad ? a1 + 1 : pad - 1;
     int x0, y0, z0, x1, y1, z1;   // voxel of (a0, a1, a2) and of (a0, a1n, a2)
     if (p == 0) { x0 = a0; y0 = a1; z0 = a2; x1 = a0; y1 = a1n; z1 = a2; }
     else if (p == 1) { y0 = a0; z0 = a1; x0 = a2; y1 = a0; z1 = a1n; x1 = a2; }
     else { z0 = a0; x0 = a1; y0 = a2; z1 = a0; x1 = a1n; y1 = a2; }
-    const uint16_t *pg = src + page * p3;
-    const uint32_t v0 = pg[((size_t)z0 * pad + y0) * pad + x0], v1 = pg[((size_t)z1 * pad + y1) * pad + x1];
-    dst[i] = v0 | (v1 << 16);
+    const uint32_t v0 = pg[(z0 * pad + y0) * pad + x0], v1 = pg[(z1 * pad + y1) * pad + x1];
+    out[r] = v0 | (v1 << 16);
   }
 }
 
@@ -548,31 +553,32 @@ hipError_t launch1(const BatchArgs &a, int tw, int unroll, int group, int grid, 
 
 }  // namespace
 
-hipError_t launch_orient_pool(int fmt, const void *src, void *dst1, void *dst2, int pad, int n_pages, int n_cus,
+hipError_t launch_orient_pool(int fmt, const void *src, void *dst1, void *dst2, int pad, int first_page, int n_pages,
                               hipStream_t s) {
-  const size_t n = (size_t)pad * pad * pad * (size_t)n_pages;
-  const dim3 grid(n_cus * 8), blk(256);
+  if (n_pages <= 0) return hipSuccess;
+  const dim3 grid(n_pages, pad), blk(256);
   switch (fmt) {
     case HYD_FMT_U8:
       hipLaunchKernelGGL(orient_pool_kernel<uint8_t>, grid, blk, 0, s, static_cast<const uint8_t *>(src),
-                         static_cast<uint8_t *>(dst1), static_cast<uint8_t *>(dst2), pad, n);
+                         static_cast<uint8_t *>(dst1), static_cast<uint8_t *>(dst2), pad, first_page);
       break;
     case HYD_FMT_F16:
       hipLaunchKernelGGL(orient_pool_kernel<uint16_t>, grid, blk, 0, s, static_cast<const uint16_t *>(src),
-                         static_cast<uint16_t *>(dst1), static_cast<uint16_t *>(dst2), pad, n);
+                         static_cast<uint16_t *>(dst1), static_cast<uint16_t *>(dst2), pad, first_page);
       break;
     default:
       hipLaunchKernelGGL(orient_pool_kernel<uint32_t>, grid, blk, 0, s, static_cast<const uint32_t *>(src),
-                         static_cast<uint32_t *>(dst1), static_cast<uint32_t *>(dst2), pad, n);
+                         static_cast<uint32_t *>(dst1), static_cast<uint32_t *>(dst2), pad, first_page);
       break;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_orient_pairs(const void *src, void *dst, int pad, int n_pages, int n_cus, hipStream_t s) {
-  const size_t n = (size_t)pad * pad * pad * (size_t)n_pages;
-  hipLaunchKernelGGL(orient_pairs_kernel, dim3(n_cus * 8), dim3(256), 0, s, static_cast<const uint16_t *>(src),
-                     static_cast<uint32_t *>(dst), pad, n);
+hipError_t launch_orient_pairs(const void *src, void *dst, int pad, int total_pages, int first_page, int n_pages,
+                               hipStream_t s) {
+  if (n_pages <= 0) return hipSuccess;
+  hipLaunchKernelGGL(orient_pairs_kernel, dim3(n_pages, pad, 3), dim3(256), 0, s, static_cast<const uint16_t *>(src),
+                     static_cast<uint32_t *>(dst), pad, total_pages, first_page);
   return hipGetLastError();
 }
 

@@ -187,7 +187,8 @@ int32_t hyd_bind_pool_device(hyd_ctx *ctx, int32_t fmt, int32_t pad_edge, int32_
 /* Overwrite pages [first_page, first_page + n_pages) of the current pool with
  * n_pages host bricks of its format and edge: the reference's block_storage
  * upload when a decoded brick takes a slot (rt_block_paging.cc:277-280).
- * Synchronous.  Batched graphs must be captured again afterwards. */
+ * Synchronous.  The batched renderer's oriented copies of those pages are
+ * updated in place, so captured batch graphs stay valid. */
 int32_t hyd_write_pool_pages(hyd_ctx *ctx, int32_t first_page, int32_t n_pages, const void *host);
 /* Page table (BlockPaging, block_paging.hpp:38-44): vaddr grid of dims[0]*dims[1]*dims[2]
  * int32 (x fastest, -1 = not resident) and per-page BlockSamplerMapping {k, b[3]}

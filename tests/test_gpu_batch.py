@@ -307,3 +307,45 @@ def test_batch_64_bricks_vs_fp64_oracle(monkeypatch, fmt):
         assert torch.equal(out, ref)
     b.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("switch", [None, "HYD_BATCH_NO_PAIRS", "HYD_BATCH_SINGLE_LAYOUT"])
+def test_pool_page_writes_refresh_oriented_layouts(monkeypatch, switch):
+    """Pages overwritten in place (hyd_write_pool_pages) after the oriented copies were
+    made render as a fresh context with the modified pool does: eagerly, and from a graph
+    captured before the write (the copies are updated in place)."""
+    import dataclasses
+
+    import torch
+    from hydrant_amd.hip import Context
+
+    if switch is not None:
+        monkeypatch.setenv(switch, "1")
+    wl = _small(L.FMT_F16, n_views=4)
+    pad = wl.bs + 2 * wl.apron
+    cfg = batch_config(8, 4, 8)
+    ctx = Context(0)
+    b = _setup(ctx, wl)
+    out = b.new_output()
+    b.capture(cfg, wl.density, wl.ert, out)
+    first = b.render(cfg, wl.density, wl.ert)[0].clone()
+    new_pool = np.ascontiguousarray(wl.pool).copy()
+    flat = new_pool.reshape(new_pool.shape[0], -1)
+    for p in (1, 3, 4, 5):
+        flat[p] = flat[p][::-1].copy()
+    ctx.write_pool_pages(1, new_pool[1:2])
+    ctx.write_pool_pages(3, new_pool[3:6])
+    eager = b.render(cfg, wl.density, wl.ert)[0].clone()
+    out.fill_(-1.0)
+    b.replay()
+    ctx2 = Context(0)
+    b2 = _setup(ctx2, dataclasses.replace(wl, pool=new_pool))
+    ref = b2.render(cfg, wl.density, wl.ert)[0]
+    torch.cuda.synchronize()
+    assert not torch.equal(eager, first)
+    assert torch.equal(eager, ref)
+    assert torch.equal(out, ref)
+    b.close()
+    b2.close()
+    ctx.close()
+    ctx2.close()
