@@ -162,6 +162,58 @@ def pack(vol: np.ndarray, bs: int, apron: int, fmt_out: int, bricks: np.ndarray 
     return out
 
 
+@dataclass
+class LowestLevel:
+    """Fallback pages of a brick grid: one page per level-0 brick holding the block of the
+    coarse level that covers it, with the mapping into that block."""
+    pages: np.ndarray      # (n_bricks0, pad, pad, pad), level-0 brick order (x fastest)
+    page_k: np.ndarray     # float32 (n_bricks0,)
+    page_b: np.ndarray     # float32 (n_bricks0, 3)
+    scale: int             # level-0 bricks per coarse block along an axis (2^level)
+
+
+def lowest_level(vol: np.ndarray, bs: int, apron: int, fmt_out: int, level: int,
+                 normalize: bool = True) -> LowestLevel:
+    """The reference's lowest-sample-level fallback (RtBlockPagingServerImpl::
+    unarchive_lowest_level, src/slave/paging/rt_block_paging.cc:119-196).
+
+    The volume at level `level` is the level-0 volume averaged over 2^level boxes (edge
+    voxels repeated to a multiple of 2^level): this build's filter, the archive's own
+    being outside it.  It is packed into bricks of the same edge and apron; level-0 brick
+    (x, y, z) gets a copy of the coarse block (x, y, z) // s, s = 2^level, with the
+    reference's mapping k = bs / pad / s, b = apron / pad + (x, y, z) mod s * k
+    (rt_block_paging.cc:143-144,189), which the page table carries per page.
+    u8 bricks hold the rounded box average; FP16 / FP32 bricks hold it as float (divided
+    by 255 for a u8 volume when `normalize`, as the level-0 pack stores it)."""
+    s = 1 << int(level)
+    v = np.asarray(vol, np.float64)
+    nz, ny, nx = v.shape
+    v = np.pad(v, ((0, (-nz) % s), (0, (-ny) % s), (0, (-nx) % s)), mode="edge")
+    low = v.reshape(v.shape[0] // s, s, v.shape[1] // s, s, v.shape[2] // s, s).mean(axis=(1, 3, 5))
+    if fmt_out == L.FMT_U8:
+        blocks = pack(np.floor(low + 0.5).astype(np.uint8), bs, apron, L.FMT_U8)
+    else:
+        vals = low / 255.0 if (normalize and np.asarray(vol).dtype == np.uint8) else low
+        blocks = pack(vals.astype(np.float32), bs, apron, fmt_out, normalize=False)
+    gl = [(d + bs - 1) // bs for d in low.shape[::-1]]          # coarse grid (x, y, z)
+    g0 = [(d + bs - 1) // bs for d in (nx, ny, nz)]            # level-0 grid (x, y, z)
+    pad = bs + 2 * apron
+    k = np.float32(np.float32(bs) / np.float32(pad)) / np.float32(s)
+    b0 = np.float32(apron) / np.float32(pad)
+    n0 = g0[0] * g0[1] * g0[2]
+    pages = np.empty((n0,) + blocks.shape[1:], blocks.dtype)
+    page_k = np.full(n0, k, np.float32)
+    page_b = np.empty((n0, 3), np.float32)
+    i = 0
+    for z in range(g0[2]):
+        for y in range(g0[1]):
+            for x in range(g0[0]):
+                pages[i] = blocks[((z // s) * gl[1] + y // s) * gl[0] + x // s]
+                page_b[i] = [b0 + np.float32(c % s) * k for c in (x, y, z)]
+                i += 1
+    return LowestLevel(pages, page_k, page_b, s)
+
+
 def make_bricked(vol: np.ndarray, bs: int, apron: int, fmt_out: int, empty_fn=None, resident: int | float = 1.0,
                  seed: int = 7, normalize: bool = True) -> Bricked:
     """Brick ``vol`` (shape (z, y, x)); ``empty_fn(bmin, bmax) -> bool array`` marks empty bricks.

@@ -71,3 +71,47 @@ def test_partial_residency_matches_the_oracle(gpu_ctx):
           f"samples gpu {int(out['samples'].item())} oracle {r['samples']}", flush=True)
     assert err <= 1e-4
     assert int(out["samples"].item()) == r["samples"]
+
+
+def test_lowest_level_fallback_matches_the_oracle(gpu_ctx):
+    """Non-resident bricks fall back to their lowest-level page (a coarse level-1 block
+    with the reference's per-brick mapping) instead of stopping the ray: the GPU render of
+    that pool and page table matches the oracle's, and composites more samples than the
+    same residency without the fallback."""
+    from hydrant_amd import dataset as D
+
+    sc = _scene((25.0, 65.0, 2.2))
+    b = sc.bricked
+    low = D.lowest_level(sc.extra["volume"], b.bs, b.apron, b.fmt, level=1)
+    n0, mp = low.pages.shape[0], 200
+    pool = np.concatenate([low.pages, np.zeros((mp,) + b.pool.shape[1:], b.pool.dtype)])
+    gpu_ctx.upload_pool(b.fmt, b.pad, pool)
+    basic = np.arange(n0, dtype=np.int32).reshape(b.cheb.shape)   # brick i falls back to page i
+    res = Residency(b.cheb.shape, mp, first_slot=n0, basic_vaddr=basic)
+    for i in res.update(sc.view(), b.cheb):
+        s = res.commit(int(i))
+        assert s >= n0
+        pool[s] = b.pool[b.vaddr.ravel()[i]]
+        gpu_ctx.write_pool_pages(s, pool[s:s + 1])
+    vaddr = res.vaddr()
+    res.close()
+    page_k = np.concatenate([low.page_k, np.full(mp, b.page_k[0], np.float32)])
+    page_b = np.concatenate([low.page_b, np.tile(b.page_b[0], (mp, 1)).astype(np.float32)])
+    gpu_ctx.set_page_table(vaddr, page_k, page_b)
+    gpu_ctx.set_empty_grid(b.cheb)
+    gpu_ctx.set_mean_grid(b.mean)
+    gpu_ctx.set_tf(sc.tf)
+    out = gpu_ctx.render_volume(sc.volume_params(), sc.view(), sc.W, sc.H, film=False, count=True)
+    gpu_ctx.synchronize()
+    rb = dataclasses.replace(b, pool=pool, vaddr=vaddr, page_k=page_k, page_b=page_b)
+    r = oracle_volume(dataclasses.replace(sc, bricked=rb)).render(oracle_view(sc), 32, film=False)
+    no_lod = np.where(vaddr >= n0, vaddr, -1).astype(np.int32)
+    r0 = oracle_volume(dataclasses.replace(sc, bricked=dataclasses.replace(rb, vaddr=no_lod))).render(
+        oracle_view(sc), 32, film=False)
+    err = max_abs(out["rgba"].cpu().numpy(), r["rgba"].astype(np.float32))
+    n = int(out["samples"].item())
+    print(f"lowest-level fallback: max-abs {err:.3g}, samples gpu {n} oracle {r['samples']} "
+          f"(without fallback {r0['samples']})", flush=True)
+    assert err <= 1e-4
+    assert n == r["samples"]
+    assert n > r0["samples"]

@@ -191,3 +191,55 @@ def test_residency_rejects_bad_arguments():
     with pytest.raises(ValueError):
         res.update(_view(ARMS[0], 4), np.zeros((4, 4, 5), np.int32))
     res.close()
+
+
+def _sample(page: np.ndarray, k: float, b, x):
+    """Sampler of one padded page at brick-local point x: u = k x + b, then the
+    normalised-coordinate linear filter with clamped taps."""
+    pad = page.shape[0]
+    w, i, j = [], [], []
+    for c in range(3):
+        f = pad * (k * x[c] + b[c]) - 0.5
+        fl = np.floor(f)
+        w.append(f - fl)
+        i.append(int(np.clip(fl, 0, pad - 1)))
+        j.append(int(np.clip(np.ceil(f), 0, pad - 1)))
+    v = lambda xx, yy, zz: float(page[zz, yy, xx])
+    x00 = v(i[0], i[1], i[2]) * (1 - w[0]) + v(j[0], i[1], i[2]) * w[0]
+    x10 = v(i[0], j[1], i[2]) * (1 - w[0]) + v(j[0], j[1], i[2]) * w[0]
+    x01 = v(i[0], i[1], j[2]) * (1 - w[0]) + v(j[0], i[1], j[2]) * w[0]
+    x11 = v(i[0], j[1], j[2]) * (1 - w[0]) + v(j[0], j[1], j[2]) * w[0]
+    y0 = x00 * (1 - w[1]) + x10 * w[1]
+    y1 = x01 * (1 - w[1]) + x11 * w[1]
+    return y0 * (1 - w[2]) + y1 * w[2]
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_lowest_level_mapping_reproduces_a_linear_field(level):
+    """A box-averaged coarse level of a linear field, read through each level-0 brick's
+    fallback page and mapping, gives the field's value at the level-0 position (the
+    average of a linear function is its value at the box centre), away from the clamped
+    volume border; a wrong scale, offset or block index would not."""
+    n, bs, apron = 128, 8, 1
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    coef = (0.5, 1.25, 2.0)
+    vol = (coef[0] * x + coef[1] * y + coef[2] * z).astype(np.float32)
+    low = D.lowest_level(vol, bs, apron, L.FMT_F32, level)
+    s = 1 << level
+    g = n // bs
+    assert low.pages.shape[0] == g ** 3 and low.scale == s
+    rng = np.random.default_rng(2)
+    margin = s   # bricks of coarse blocks off the volume border (whose aprons are not clamped)
+    checked = 0
+    for bz in range(margin, g - margin):
+        for by in range(margin, g - margin):
+            for bx in range(margin, g - margin):
+                i = (bz * g + by) * g + bx
+                for _ in range(3):
+                    p = rng.uniform(0.0, 1.0, 3)
+                    got = _sample(low.pages[i], float(low.page_k[i]), low.page_b[i], p)
+                    pos = (np.array([bx, by, bz]) + p) * bs - 0.5   # level-0 voxel coordinate
+                    want = float(np.dot(coef, pos))
+                    assert abs(got - want) < 1e-3, (bx, by, bz, got, want)
+                    checked += 1
+    assert checked > 0
