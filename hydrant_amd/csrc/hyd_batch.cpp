@@ -144,9 +144,8 @@ int32_t ensure_oriented(Ctx *c) {
                                       c->pool.n_pages, c->stream));
     pairs = true;
   } else if (reserve_oriented(c, 2 * bytes)) {
-    HYD_CHECK_HIP(launch_orient_pool(c->pool.fmt, c->pool.data, c->own_oriented,
-                                     static_cast<char *>(c->own_oriented) + bytes, c->pool.pad, 0, c->pool.n_pages,
-                                     c->stream));
+    HYD_CHECK_HIP(launch_orient_pool(c->pool.fmt, c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages, 0,
+                                     c->pool.n_pages, c->stream));
   } else {
     c->oriented_gen = c->pool_gen;   // no memory for copies: the pool alone
     return HYD_OK;
@@ -168,9 +167,8 @@ int32_t hyd::refresh_oriented_pages(Ctx *c, int32_t first, int32_t n) {
     HYD_CHECK_HIP(launch_orient_pairs(c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages, first, n,
                                       c->stream));
   } else {
-    HYD_CHECK_HIP(launch_orient_pool(c->pool.fmt, c->pool.data, c->own_oriented,
-                                     static_cast<char *>(c->own_oriented) + layout_bytes(c), c->pool.pad, first, n,
-                                     c->stream));
+    HYD_CHECK_HIP(launch_orient_pool(c->pool.fmt, c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages,
+                                     first, n, c->stream));
   }
   HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
   return HYD_OK;

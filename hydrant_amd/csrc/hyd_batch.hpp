@@ -55,9 +55,10 @@ hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int
 hipError_t launch_orient_pairs(const void *src, void *dst, int pad, int total_pages, int first_page, int n_pages,
                                hipStream_t s);
 
-// Writes pages [first_page, first_page + n_pages) of the y-fastest (dst1) and
-// z-fastest (dst2) layouts of an x-fastest pool.
-hipError_t launch_orient_pool(int fmt, const void *src, void *dst1, void *dst2, int pad, int first_page, int n_pages,
-                              hipStream_t s);
+// Writes pages [first_page, first_page + n_pages) of the y-fastest (at dst) and
+// z-fastest (at dst + pad^3 * total_pages elements) layouts of an x-fastest
+// pool of total_pages pages.
+hipError_t launch_orient_pool(int fmt, const void *src, void *dst, int pad, int total_pages, int first_page,
+                              int n_pages, hipStream_t s);
 
 }  // namespace hyd

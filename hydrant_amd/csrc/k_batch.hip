@@ -41,6 +41,8 @@
 // of four 2-byte-aligned dwords, which cost two accesses per lane each.
 #include <hip/hip_fp16.h>
 
+#include <type_traits>
+
 #include "hyd_batch.hpp"
 
 namespace hyd {
@@ -454,44 +456,68 @@ __global__ __launch_bounds__(64 * GROUP) void batch_kernel(BatchArgs a) {
   }
 }
 
-// The two other layouts of a pool (PERM 1 and 2) from the x-fastest one, for
-// pages first_page + blockIdx.x; workgroup (page, z) copies one z plane.
-template <typename T>
-__global__ __launch_bounds__(256) void orient_pool_kernel(const T *src, T *dst1, T *dst2, int pad, int first_page) {
+// The oriented copies of an x-fastest pool, for pages first_page + blockIdx.x.
+// A workgroup produces one plane of one layout (layout p of page P at dst +
+// p * pad^3 * total_pages + P * pad^3) and, when `staged` (the source planes
+// fit in LDS), first reads the source voxels it needs row by row into LDS, so
+// that both its global reads and its writes run along rows:
+//   layout 1 (y, z, x), plane a1 = z: source plane z (and z + 1 for pairs);
+//   layout 2 (z, x, y), plane a2 = y: the source rows of every z at that y.
+// Layout 0 (the x-fastest copy of the row-pair form) reads plane z directly.
+// PAIRS: 4-byte elements (v(a0, a1, a2), v(a0, min(a1 + 1, pad - 1), a2)) of a
+// 2-byte pool; otherwise single voxels of type T.
+template <typename T, bool PAIRS>
+__global__ __launch_bounds__(256) void orient_kernel(const T *src, void *dst_v, int pad, int total_pages,
+                                                     int first_page, int layout0, bool staged) {
+  using Out = typename std::conditional<PAIRS, uint32_t, T>::type;
+  extern __shared__ unsigned char s_raw[];
+  T *s_a = reinterpret_cast<T *>(s_raw), *s_b = s_a + pad * pad;
   const size_t page = (size_t)first_page + blockIdx.x;
   const size_t p3 = (size_t)pad * pad * pad;
-  const int z = blockIdx.y;
-  const T *sp = src + page * p3 + (size_t)z * pad * pad;
-  T *d1 = dst1 + page * p3, *d2 = dst2 + page * p3;
-  for (int r = threadIdx.x; r < pad * pad; r += 256) {
-    const int x = r % pad, y = r / pad;
-    const T v = sp[r];
-    d1[(x * pad + z) * pad + y] = v;
-    d2[(y * pad + x) * pad + z] = v;
-  }
-}
-
-// The three FP16 row-pair layouts of an x-fastest pool of total_pages pages
-// (layout p at dst + p * pad^3 * total_pages words): element (a0, a1, a2) =
-// (v(a0, a1, a2), v(a0, min(a1 + 1, pad - 1), a2)) in layout p's axes (p 0:
-// x, y, z; 1: y, z, x; 2: z, x, y).  Workgroup (page, a2, p) writes one plane
-// of page first_page + blockIdx.x.
-__global__ __launch_bounds__(256) void orient_pairs_kernel(const uint16_t *src, uint32_t *dst, int pad,
-                                                           int total_pages, int first_page) {
-  const size_t page = (size_t)first_page + blockIdx.x;
-  const size_t p3 = (size_t)pad * pad * pad;
-  const int a2 = blockIdx.y, p = blockIdx.z;
-  const uint16_t *pg = src + page * p3;
-  uint32_t *out = dst + (size_t)p * p3 * (size_t)total_pages + page * p3 + (size_t)a2 * pad * pad;
-  for (int r = threadIdx.x; r < pad * pad; r += 256) {
-    const int a0 = r % pad, a1 = r / pad;
-    const int a1n = a1 + 1 < pad ? a1 + 1 : pad - 1;
-    int x0, y0, z0, x1, y1, z1;   // voxel of (a0, a1, a2) and of (a0, a1n, a2)
-    if (p == 0) { x0 = a0; y0 = a1; z0 = a2; x1 = a0; y1 = a1n; z1 = a2; }
-    else if (p == 1) { y0 = a0; z0 = a1; x0 = a2; y1 = a0; z1 = a1n; x1 = a2; }
-    else { z0 = a0; x0 = a1; y0 = a2; z1 = a0; x1 = a1n; y1 = a2; }
-    const uint32_t v0 = pg[(z0 * pad + y0) * pad + x0], v1 = pg[(z1 * pad + y1) * pad + x1];
-    out[r] = v0 | (v1 << 16);
+  const int plane = blockIdx.y, p = layout0 + (int)blockIdx.z;
+  const T *pg = src + page * p3;
+  Out *out = static_cast<Out *>(dst_v) + (size_t)(p - (PAIRS ? 0 : 1)) * p3 * (size_t)total_pages + page * p3;
+  const int n2 = pad * pad;
+  const auto pack = [](T v0, T v1) -> Out {
+    if constexpr (PAIRS) return (uint32_t)v0 | ((uint32_t)v1 << 16);
+    else return v0;
+  };
+  if (p == 0) {   // (x, y, z): plane z; the pair neighbour is y + 1
+    const int z = plane;
+    for (int r = threadIdx.x; r < n2; r += 256) {
+      const int x = r % pad, y = r / pad, y1 = y + 1 < pad ? y + 1 : pad - 1;
+      out[(size_t)z * n2 + r] = pack(pg[((size_t)z * pad + y) * pad + x], pg[((size_t)z * pad + y1) * pad + x]);
+    }
+  } else if (p == 1) {   // (y, z, x): plane a1 = z of every a2 = x; the pair neighbour is z + 1
+    const int z = plane, z1 = z + 1 < pad ? z + 1 : pad - 1;
+    if (staged) {
+      for (int r = threadIdx.x; r < n2; r += 256) {
+        s_a[r] = pg[(size_t)z * n2 + r];
+        if (PAIRS) s_b[r] = pg[(size_t)z1 * n2 + r];
+      }
+      __syncthreads();
+    }
+    for (int r = threadIdx.x; r < n2; r += 256) {
+      const int y = r % pad, x = r / pad;   // consecutive threads: consecutive y (a0)
+      const T v0 = staged ? s_a[y * pad + x] : pg[((size_t)z * pad + y) * pad + x];
+      const T v1 = !PAIRS ? v0 : staged ? s_b[y * pad + x] : pg[((size_t)z1 * pad + y) * pad + x];
+      out[((size_t)x * pad + z) * pad + y] = pack(v0, v1);
+    }
+  } else {   // (z, x, y): plane a2 = y; the pair neighbour is x + 1
+    const int y = plane;
+    if (staged) {
+      for (int r = threadIdx.x; r < n2; r += 256) {
+        const int x = r % pad, z = r / pad;
+        s_a[r] = pg[((size_t)z * pad + y) * pad + x];   // s_a[z][x]
+      }
+      __syncthreads();
+    }
+    for (int r = threadIdx.x; r < n2; r += 256) {
+      const int z = r % pad, x = r / pad, x1 = x + 1 < pad ? x + 1 : pad - 1;   // consecutive z (a0)
+      const T v0 = staged ? s_a[z * pad + x] : pg[((size_t)z * pad + y) * pad + x];
+      const T v1 = !PAIRS ? v0 : staged ? s_a[z * pad + x1] : pg[((size_t)z * pad + y) * pad + x1];
+      out[(size_t)y * n2 + r] = pack(v0, v1);
+    }
   }
 }
 
@@ -553,33 +579,31 @@ hipError_t launch1(const BatchArgs &a, int tw, int unroll, int group, int grid, 
 
 }  // namespace
 
-hipError_t launch_orient_pool(int fmt, const void *src, void *dst1, void *dst2, int pad, int first_page, int n_pages,
-                              hipStream_t s) {
+// Launches orient_kernel for layouts [layout0, layout0 + n_layouts); the LDS
+// staging is used while two source planes fit in 64 KiB.
+template <typename T, bool PAIRS>
+hipError_t launch_orient(const void *src, void *dst, int pad, int total_pages, int first_page, int n_pages,
+                         int layout0, int n_layouts, hipStream_t s) {
   if (n_pages <= 0) return hipSuccess;
-  const dim3 grid(n_pages, pad), blk(256);
-  switch (fmt) {
-    case HYD_FMT_U8:
-      hipLaunchKernelGGL(orient_pool_kernel<uint8_t>, grid, blk, 0, s, static_cast<const uint8_t *>(src),
-                         static_cast<uint8_t *>(dst1), static_cast<uint8_t *>(dst2), pad, first_page);
-      break;
-    case HYD_FMT_F16:
-      hipLaunchKernelGGL(orient_pool_kernel<uint16_t>, grid, blk, 0, s, static_cast<const uint16_t *>(src),
-                         static_cast<uint16_t *>(dst1), static_cast<uint16_t *>(dst2), pad, first_page);
-      break;
-    default:
-      hipLaunchKernelGGL(orient_pool_kernel<uint32_t>, grid, blk, 0, s, static_cast<const uint32_t *>(src),
-                         static_cast<uint32_t *>(dst1), static_cast<uint32_t *>(dst2), pad, first_page);
-      break;
-  }
+  const size_t lds = 2 * (size_t)pad * pad * sizeof(T);
+  const bool staged = lds <= 65536;
+  hipLaunchKernelGGL((orient_kernel<T, PAIRS>), dim3(n_pages, pad, n_layouts), dim3(256), staged ? lds : 0, s,
+                     static_cast<const T *>(src), dst, pad, total_pages, first_page, layout0, staged);
   return hipGetLastError();
+}
+
+hipError_t launch_orient_pool(int fmt, const void *src, void *dst, int pad, int total_pages, int first_page,
+                              int n_pages, hipStream_t s) {
+  switch (fmt) {
+    case HYD_FMT_U8: return launch_orient<uint8_t, false>(src, dst, pad, total_pages, first_page, n_pages, 1, 2, s);
+    case HYD_FMT_F16: return launch_orient<uint16_t, false>(src, dst, pad, total_pages, first_page, n_pages, 1, 2, s);
+    default: return launch_orient<uint32_t, false>(src, dst, pad, total_pages, first_page, n_pages, 1, 2, s);
+  }
 }
 
 hipError_t launch_orient_pairs(const void *src, void *dst, int pad, int total_pages, int first_page, int n_pages,
                                hipStream_t s) {
-  if (n_pages <= 0) return hipSuccess;
-  hipLaunchKernelGGL(orient_pairs_kernel, dim3(n_pages, pad, 3), dim3(256), 0, s, static_cast<const uint16_t *>(src),
-                     static_cast<uint32_t *>(dst), pad, total_pages, first_page);
-  return hipGetLastError();
+  return launch_orient<uint16_t, true>(src, dst, pad, total_pages, first_page, n_pages, 0, 3, s);
 }
 
 hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int group, int grid, hipStream_t s) {
