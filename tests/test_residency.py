@@ -183,6 +183,13 @@ def test_residency_artifact_when_nothing_can_be_freed():
 
 
 def test_residency_rejects_bad_arguments():
+    cheb = np.zeros((4, 4, 4), np.int32)
+    with pytest.raises(L.HydError) as e:
+        cull_bricks(_view(ARMS[0], 4), cheb, bbox=((0, 0, 0), (5, 4, 4)))
+    assert e.value.code == L.HYD_ERR_RANGE
+    with pytest.raises(L.HydError):
+        cull_bricks(_view(ARMS[0], 4), cheb, bbox=((3, 0, 0), (2, 4, 4)))
+    assert cull_bricks(_view(ARMS[0], 4), cheb, limit=-3).size == 0
     with pytest.raises(L.HydError):
         Residency((0, 4, 4), 8)
     res = Residency((4, 4, 4), 8)
