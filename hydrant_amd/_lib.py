@@ -15,7 +15,7 @@ HYD_OK, HYD_ERR_ARG, HYD_ERR_HIP, HYD_ERR_STATE, HYD_ERR_RANGE = 0, 1, 2, 3, 4
 FMT_U8, FMT_F16, FMT_F32 = 0, 1, 2
 MODE_DEFAULT, MODE_PARTITION, MODE_PAGING = 0, 1, 2
 SCHED_REFERENCE, SCHED_LATTICE = 0, 1
-BLOCKS_VOLUME, BLOCKS_SOLID = 0, 1
+BLOCKS_VOLUME, BLOCKS_SOLID, BLOCKS_PAGING = 0, 1, 2
 ISO_COLOR, ISO_NORMAL, ISO_POSITION = 0, 1, 2
 
 f3 = C.c_float * 3

@@ -434,12 +434,15 @@ int32_t hyd_render_blocks(hyd_ctx *c, const hyd_blocks_params *p, const hyd_view
   if (st) return st;
   HYD_REQUIRE(p && view && rgba_dev && width > 0 && height > 0, HYD_ERR_ARG, "invalid blocks arguments");
   HYD_REQUIRE(p->step > 0.f, HYD_ERR_ARG, "step must be > 0");
+  HYD_REQUIRE(p->mode == HYD_BLOCKS_VOLUME || p->mode == HYD_BLOCKS_SOLID || p->mode == HYD_BLOCKS_PAGING,
+              HYD_ERR_ARG, "unknown blocks mode %d", p->mode);
   HYD_REQUIRE(c->cheb.data, HYD_ERR_STATE, "no empty-space grid: call hyd_set_empty_grid first");
-  HYD_REQUIRE(p->mode == HYD_BLOCKS_SOLID || c->mean.data, HYD_ERR_STATE,
+  HYD_REQUIRE(p->mode != HYD_BLOCKS_VOLUME || c->mean.data, HYD_ERR_STATE,
               "Blocks.Volume needs a mean grid: call hyd_set_mean_grid first");
-  HYD_REQUIRE(p->mode == HYD_BLOCKS_SOLID ||
-                  std::memcmp(c->mean.dim, c->cheb.dim, sizeof(c->cheb.dim)) == 0,
+  HYD_REQUIRE(p->mode != HYD_BLOCKS_VOLUME || std::memcmp(c->mean.dim, c->cheb.dim, sizeof(c->cheb.dim)) == 0,
               HYD_ERR_STATE, "mean grid and empty-space grid dims differ");
+  HYD_REQUIRE(p->mode != HYD_BLOCKS_PAGING || c->paging.vaddr, HYD_ERR_STATE,
+              "the paging view needs a page table: call hyd_set_page_table first");
   BlocksArgs a;
   std::memset(&a, 0, sizeof(a));
   a.bmin = f3{p->bbox_min[0], p->bbox_min[1], p->bbox_min[2]};
@@ -454,6 +457,11 @@ int32_t hyd_render_blocks(hyd_ctx *c, const hyd_blocks_params *p, const hyd_view
   a.cheb = c->cheb.data;
   a.mean = c->mean.data;
   std::memcpy(a.dim, c->cheb.dim, sizeof(a.dim));
+  if (p->mode == HYD_BLOCKS_PAGING) {
+    a.vaddr = c->paging.vaddr;
+    std::memcpy(a.vdim, c->paging.vdim, sizeof(a.vdim));
+    a.lowest = c->paging.lowest_blkcnt;
+  }
   a.rgba = reinterpret_cast<float4 *>(rgba_dev);
   HYD_CHECK_HIP(launch_blocks(a, c->stream));
   return HYD_OK;

@@ -164,6 +164,9 @@ struct BlocksArgs {
   const int32_t *cheb;
   const float *mean;
   int32_t dim[3];
+  const int32_t *vaddr;   // page table (HYD_BLOCKS_PAGING)
+  int32_t vdim[3];
+  int32_t lowest;         // pages at or above this id are level-0 pages
   float4 *rgba;
 };
 hipError_t launch_blocks(const BlocksArgs &a, hipStream_t s);

@@ -28,7 +28,8 @@ __device__ __forceinline__ int skip_nblock_steps(f3 &o, f3 d, f3 ip, int nblocks
   return (int)di;
 }
 
-// VisBlocksShaderKernel::main, blocks_shader.cu:16-50.
+// VisBlocksShaderKernel::main, blocks_shader.cu:16-50; HYD_BLOCKS_PAGING is the
+// Paging renderer's residency view, PagingShaderKernel::main (paging_shader.cu:24-61).
 __global__ __launch_bounds__(64) void blocks_kernel(BlocksArgs a) {
   const int x = blockIdx.x * kTile + (threadIdx.x % kTile);
   const int y = blockIdx.y * kTile + (threadIdx.x / kTile);
@@ -53,6 +54,16 @@ __global__ __launch_bounds__(64) void blocks_kernel(BlocksArgs a) {
       const int cd = a.cheb[gi];
       if (cd) {
         nsteps -= skip_nblock_steps(o, d, ip, cd, cdu, a.step);
+      } else if (a.mode == HYD_BLOCKS_PAGING) {
+        // PagingShaderKernel::main (paging_shader.cu:34-59): one composite per
+        // resident brick, green for a level-0 page, red for a lower-level one
+        const int pgid = a.vaddr[grid_index(ip, a.vdim)];
+        if (pgid == -1) break;   // page fault
+        const float4 col = pgid >= a.lowest ? make_float4(0.f, .5f, 0.f, .5f) : make_float4(.2f, 0.f, 0.f, .2f);
+        nsteps -= skip_nblock_steps(o, d, ip, 1, cdu, a.step);
+        const float t = 1.f - v.w;
+        v = make_float4(v.x + col.x * t, v.y + col.y * t, v.z + col.z * t, v.w + col.w * t);
+        if (v.w > opacity_threshold) break;
       } else {
         const f3 c = {gclamp(ip.x, a.bmin.x, hi.x), gclamp(ip.y, a.bmin.y, hi.y), gclamp(ip.z, a.bmin.z, hi.z)};
         const f3 rip = {c.x / ext.x, c.y / ext.y, c.z / ext.z};

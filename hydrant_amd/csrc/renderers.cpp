@@ -552,9 +552,41 @@ class IsosurfaceRenderer : public IRenderer {
   size_t iso_cap_ = 0;
 };
 
+// ------------------------------------------------------ Paging renderer ----
+// PagingRenderer (paging.cc:10-56; PagingRendererConfig paging.schema.hpp:9-13).
+// Its offline path renders nothing: offline_render_ctxed returns the film as
+// created (paging.cc:51-56), untouched pixels, which the pixel rule writes as
+// the clear colour.  The residency view its realtime path draws is
+// hyd_render_blocks with HYD_BLOCKS_PAGING over the caller's page table.
+class PagingRenderer : public IRenderer {
+ public:
+  void init(std::shared_ptr<Dataset> dataset, const RendererConfig &cfg) override {
+    IRenderer::init(std::move(dataset), cfg);
+    compute_geometry();
+    update(cfg.params);
+  }
+
+  void update(const json::Value &params) override {
+    IRenderer::update(params);
+    if (params.is_null()) return;
+    mem_limit_mb_ = num_or(params, "mem_limit_mb", mem_limit_mb_);
+    if (!(mem_limit_mb_ > 0.f)) throw Error(HYD_ERR_ARG, "mem_limit_mb must be > 0");
+  }
+
+  void offline_render(const hyd_camera &camera, uint8_t *rgb_host) override {
+    (void)view_of(camera);   // the camera is checked as by the other renderers
+    const size_t P = (size_t)width_ * height_;
+    for (size_t i = 0; i < P; ++i) std::memcpy(rgb_host + 3 * i, clear_, 3);
+  }
+
+ private:
+  float mem_limit_mb_ = 2048.f;   // used by the reference's realtime path only
+};
+
 HYD_REGISTER_RENDERER(VolumeRenderer, "Volume");
 HYD_REGISTER_RENDERER(BlocksRenderer, "Blocks");
 HYD_REGISTER_RENDERER(IsosurfaceRenderer, "Isosurface");
+HYD_REGISTER_RENDERER(PagingRenderer, "Paging");
 
 }  // namespace
 }  // namespace front

@@ -52,7 +52,9 @@ enum { HYD_MODE_DEFAULT = 0, HYD_MODE_PARTITION = 1, HYD_MODE_PAGING = 2 };
 enum { HYD_SCHED_REFERENCE = 0, HYD_SCHED_LATTICE = 1 };
 
 /* BlocksRenderMode, ui/blocks.schema.hpp:11-12 */
-enum { HYD_BLOCKS_VOLUME = 0, HYD_BLOCKS_SOLID = 1 };
+/* Blocks.Volume, Blocks.Solid, and the Paging renderer's residency view
+ * (green: a page at or above the page table's lowest_blkcnt, red: below it). */
+enum { HYD_BLOCKS_VOLUME = 0, HYD_BLOCKS_SOLID = 1, HYD_BLOCKS_PAGING = 2 };
 
 /* IsosurfaceRenderMode, ui/isosurface.schema.hpp:11-12 */
 enum { HYD_ISO_COLOR = 0, HYD_ISO_NORMAL = 1, HYD_ISO_POSITION = 2 };
@@ -254,7 +256,9 @@ int32_t hyd_trace_volume(hyd_ctx *ctx, const hyd_volume_params *p, const hyd_vie
                          int32_t height, int32_t x, int32_t y, hyd_trace_rec *out, int32_t max_recs,
                          int32_t *n_out);
 
-/* Blocks.Volume / Blocks.Solid (VisBlocksShaderKernel::main, blocks_shader.cu:16-50). */
+/* Blocks.Volume / Blocks.Solid (VisBlocksShaderKernel::main, blocks_shader.cu:16-50) and the
+ * residency view (PagingShaderKernel::main, paging_shader.cu:24-61: the page table of
+ * hyd_set_page_table over the empty-space grid; density is not used). */
 int32_t hyd_render_blocks(hyd_ctx *ctx, const hyd_blocks_params *p, const hyd_view *view, int32_t width,
                           int32_t height, float *rgba_dev);
 /* Isosurface (IsosurfaceShaderKernel::main, isosurface_shader.cu:50-125). */

@@ -6,8 +6,10 @@
  *   IRenderer::update(json params)                    -> hyd_renderer_update
  *   IRenderer::offline_render(Camera) -> RGB8 image    -> hyd_renderer_offline_render
  *   RendererFactory::list_candidates()                -> hyd_renderer_candidates
- * with the renderers "Volume", "Blocks" and "Isosurface" registered by name as
- * REGISTER_RENDERER does (volume.cc:315, blocks.cc:74, isosurface.cc:313).
+ * with the renderers "Volume", "Blocks", "Isosurface" and "Paging" registered by
+ * name as REGISTER_RENDERER does (volume.cc:315, blocks.cc:74, isosurface.cc:313,
+ * paging.cc:193).  "Paging" renders nothing offline, as the reference's
+ * offline_render_ctxed (paging.cc:51-56): every pixel is the clear colour.
  *
  * Configuration JSON follows the reference schemas:
  *   RendererConfig  {"resolution": [w, h], "renderer": "Volume", "params": {...}}
@@ -25,7 +27,8 @@
  *                   IsosurfaceRendererParams (ui/isosurface.schema.hpp:16-22): mode
  *                   Color|Normal|Position, surface_color, isovalue (0.5),
  *                   mem_limit_mb; plus "corrected" (0 = the reference's crossing
- *                   test, 1 = bracket + "bisect_steps" bisections).
+ *                   test, 1 = bracket + "bisect_steps" bisections);
+ *                   PagingRendererConfig (paging.schema.hpp:9-13): mem_limit_mb.
  *   camera          CameraConfig (core/scene.schema.hpp:10-35): "orbit" {arm,
  *                   center} or "ptu" {position, target, up}, "perspective" {fovy}.
  * device: "Cuda" or "Hip" render on this context's GPU; "Cpu" (the reference's

@@ -541,13 +541,16 @@ template <class T> struct BlocksShader {
   Box<T> bbox;
   T step;
   int max_steps;
-  int mode;          // 0 Volume, 1 Solid (ui/blocks.schema.hpp:11-12)
+  int mode;          // 0 Volume, 1 Solid (ui/blocks.schema.hpp:11-12), 2 the Paging renderer's view
   T density;
   const int *cheb; I3 dim;
   const float *mean;
+  const int *vaddr; I3 vdim;   // mode 2: page table
+  int lowest;                  // mode 2: BlockPaging::lowest_blkcnt
 };
 
-// VisBlocksShaderKernel::main, src/slave/renderers/blocks/blocks_shader.cu:16-50.
+// VisBlocksShaderKernel::main, src/slave/renderers/blocks/blocks_shader.cu:16-50;
+// mode 2: PagingShaderKernel::main, src/slave/renderers/paging/paging_shader.cu:24-61.
 template <class T> void blocks_emit(const BlocksShader<T> &sh, const Ray<T> &ray_in, VolumePixel<T> &out) {
   VolumePixel<T> px{};
   Ray<T> ray = ray_in;
@@ -568,6 +571,13 @@ template <class T> void blocks_emit(const BlocksShader<T> &sh, const Ray<T> &ray
     int cd = grid_lookup(sh.cheb, sh.dim, ip);
     if (cd) {
       nsteps -= skip_nblock_steps(ray, ip, cd, cdu, sh.step);
+    } else if (sh.mode == 2) {
+      int pgid = grid_lookup(sh.vaddr, sh.vdim, ip);
+      if (pgid == -1) break;   // page fault
+      V4<T> col = pgid >= sh.lowest ? V4<T>{T(0), T(1), T(0), T(1)} * T(.5f) : V4<T>{T(1), T(0), T(0), T(1)} * T(.2f);
+      nsteps -= skip_nblock_steps(ray, ip, 1, cdu, sh.step);
+      px.v += col * (T(1) - px.v.w);
+      if (px.v.w > opacity_threshold) { nsteps = 0; break; }
     } else {
       V3<T> lo = sh.bbox.min, hi = sh.bbox.max - T(1);
       V3<T> c{gclamp(ip.x, lo.x, hi.x), gclamp(ip.y, lo.y, hi.y), gclamp(ip.z, lo.z, hi.z)};

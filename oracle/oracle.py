@@ -41,6 +41,7 @@ class OBlocksArgs(C.Structure):
         ("W", C.c_int), ("H", C.c_int), ("bbox_min", d3), ("bbox_max", d3), ("step", C.c_double),
         ("max_steps", C.c_int), ("mode", C.c_int), ("density", C.c_double), ("dim", C.c_int * 3),
         ("cheb", C.POINTER(C.c_int)), ("mean", C.POINTER(C.c_float)),
+        ("vaddr", C.POINTER(C.c_int)), ("vdim", C.c_int * 3), ("lowest", C.c_int),
     ]
 
 
@@ -299,9 +300,11 @@ def composite(partials: np.ndarray, z_order) -> tuple[np.ndarray, np.ndarray]:
 
 
 def blocks_render(view: OView, W, H, bbox_min, bbox_max, step, cheb, mean, mode=0, density=1e-2,
-                  max_steps=4_000_000, prec=32, nthreads=0) -> np.ndarray:
+                  max_steps=4_000_000, prec=32, nthreads=0, vaddr=None, lowest=0) -> np.ndarray:
+    """Blocks.Volume (mode 0) / Blocks.Solid (1), or the Paging view (2) over page table `vaddr` (z, y, x)."""
     ch = np.ascontiguousarray(cheb, np.int32)
-    mn = np.ascontiguousarray(mean, np.float32)
+    mn = np.ascontiguousarray(mean if mean is not None else np.zeros(ch.shape), np.float32)
+    va = np.ascontiguousarray(vaddr if vaddr is not None else np.full(ch.shape, -1), np.int32)
     a = OBlocksArgs()
     a.W, a.H = W, H
     a.bbox_min[:] = [float(x) for x in bbox_min]
@@ -309,6 +312,8 @@ def blocks_render(view: OView, W, H, bbox_min, bbox_max, step, cheb, mean, mode=
     a.step, a.max_steps, a.mode, a.density = float(step), int(max_steps), int(mode), float(density)
     a.dim[:] = list(ch.shape[::-1])
     a.cheb, a.mean = _pi(ch), _pf(mn)
+    a.vaddr, a.lowest = _pi(va), int(lowest)
+    a.vdim[:] = list(va.shape[::-1])
     rgba = np.zeros((H, W, 4))
     lib().orc_blocks_render(prec, nthreads, C.byref(view), C.byref(a), _pd(rgba))
     return rgba

@@ -46,6 +46,9 @@ typedef struct {
   int dim[3];
   const int *cheb;
   const float *mean;
+  const int *vaddr;   // mode 2
+  int vdim[3];
+  int lowest;
 } orc_blocks_args;
 
 typedef struct {
@@ -182,6 +185,7 @@ template <class T> BlocksShader<T> to_blocks(const orc_blocks_args *a) {
   s.bbox = Box<T>{v3<T>(a->bbox_min), v3<T>(a->bbox_max)};
   s.step = T(a->step); s.max_steps = a->max_steps; s.mode = a->mode; s.density = T(a->density);
   s.cheb = a->cheb; s.dim = I3{a->dim[0], a->dim[1], a->dim[2]}; s.mean = a->mean;
+  s.vaddr = a->vaddr; s.vdim = I3{a->vdim[0], a->vdim[1], a->vdim[2]}; s.lowest = a->lowest;
   return s;
 }
 

@@ -26,6 +26,27 @@ def test_blocks(gpu_ctx, mode):
     assert np.any(g[..., 3] > 0)
 
 
+def test_paging_view(gpu_ctx):
+    """The Paging renderer's residency view over a seeded page table with faults
+    and two page levels, against the oracle."""
+    sc = W.blocks(n=128, W=96, H=96, bs=16)
+    b = sc.bricked
+    rng = np.random.default_rng(11)
+    vaddr = rng.integers(0, 64, size=b.cheb.shape).astype(np.int32)
+    vaddr[rng.random(b.cheb.shape) < 0.05] = -1
+    lowest = 32
+    gpu_ctx.set_empty_grid(b.cheb)
+    gpu_ctx.set_page_table(vaddr, np.ones(64, np.float32), np.zeros((64, 3), np.float32), lowest)
+    p = W.blocks_params(sc, L.BLOCKS_PAGING)
+    g = gpu_ctx.render_blocks(p, sc.view(), sc.W, sc.H).cpu().numpy()
+    o = O.blocks_render(oracle_view(sc), sc.W, sc.H, sc.bbox_min, sc.bbox_max, sc.geom.step, b.cheb, None,
+                        mode=2, vaddr=vaddr, lowest=lowest)
+    err = max_abs(g, o.astype(np.float32))
+    print(f"paging view: max-abs {err:.3g}, faults {int((vaddr < 0).sum())}", flush=True)
+    assert err <= 1e-6
+    assert np.any(g[..., 1] > 0) and np.any(g[..., 0] > 0)   # both page levels seen
+
+
 def _iso_oracle(sc, p):
     b = sc.bricked
     return O.IsoScene(sc.W, sc.H, sc.bbox_min, sc.bbox_max, p.step, p.du, np.array(p.to_world[:]),

@@ -124,6 +124,19 @@ def test_blocks_json_matches_kernel(gpu_ctx, mode):
     r.close()
 
 
+def test_paging_renderer_offline_is_the_clear_colour(gpu_ctx):
+    """The reference's Paging renderer renders nothing offline (paging.cc:51-56)."""
+    cfg = {"resolution": [40, 24], "renderer": "Paging",
+           "params": {"clear_color": [0.2, 0.4, 1.0], "mem_limit_mb": 512}}
+    r = Renderer(gpu_ctx, _blob(48), cfg, block_size=16, apron=1)
+    img = r.offline_render({"orbit": {"arm": [25, 65, 2.2]}})
+    assert img.shape == (24, 40, 3)
+    assert np.all(img == np.array([51, 102, 255], np.uint8))
+    with pytest.raises(L.HydError, match="mem_limit_mb"):
+        r.update({"mem_limit_mb": 0})
+    r.close()
+
+
 def _blob(n):
     return D.gen_blobs(n, 16, W.SEED_BLOBS, sig=(8.0, 40.0), sigma_scale=n / 256.0, amp=(0.3, 1.0))
 

@@ -365,3 +365,46 @@ def test_pixel_pass_saturate():
     rgba = np.array([[[0.5, 1.2, -0.1, 0.3], [0.2, 0.2, 0.2, 0.0]]], np.float32)
     out = O.pixel_pass(rgba, clear=(1, 2, 3))
     assert out[0, 0].tolist() == [127, 255, 0] and out[0, 1].tolist() == [1, 2, 3]
+
+
+# ---- Paging renderer's residency view (paging_shader.cu:24-61) --------------
+def _row_view(nx_origin=-0.5):
+    """A view whose centre pixel (1, 1) of a 2 x 2 image looks along +x from
+    (nx_origin, 1.5, 1.5): camera -z maps to +x, ctg 1."""
+    v = O.OView()
+    v.trans[:] = [0, 0, 1, 0, 0, 1, 0, 0, -1, 0, 0, 0, nx_origin, 1.5, 1.5, 1]
+    v.ctg_fovy_2 = 1.0
+    v.ray_o[:] = [nx_origin, 1.5, 1.5]
+    return v
+
+
+def _paging_centre(cheb, vaddr, lowest):
+    nz, ny, nx = cheb.shape
+    rgba = O.blocks_render(_row_view(), 2, 2, (0, 0, 0), (nx, ny, nz), 1.0 / 64, cheb, None, mode=2,
+                           vaddr=vaddr, lowest=lowest)
+    return rgba[1, 1]
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_paging_view_closed_forms(prec):
+    """One composite per resident brick along the ray (not per sample), green
+    (0, .5, 0, .5) for pages >= lowest_blkcnt and red (.2, 0, 0, .2) below, front
+    to back; a page fault stops the ray; empty bricks are skipped; the ray stops
+    once alpha exceeds 0.95."""
+    cheb = np.zeros((3, 3, 3), np.int32)
+    vaddr = np.full((3, 3, 3), 5, np.int32)
+    vaddr[1, 1, :] = [12, 3, 11]                      # green, red, green along the ray (lowest 10)
+    np.testing.assert_allclose(_paging_centre(cheb, vaddr, 10), [0.1, 0.7, 0.0, 0.8], atol=1e-6)
+    v2 = vaddr.copy()
+    v2[1, 1, 1] = -1                                  # page fault in the middle brick
+    np.testing.assert_allclose(_paging_centre(cheb, v2, 10), [0.0, 0.5, 0.0, 0.5], atol=1e-6)
+    c2 = cheb.copy()
+    c2[1, 1, 1] = 1                                   # empty middle brick: skipped, its page unread
+    np.testing.assert_allclose(_paging_centre(c2, v2, 10), [0.0, 0.75, 0.0, 0.75], atol=1e-6)
+    long_cheb = np.zeros((3, 3, 8), np.int32)
+    long_vaddr = np.full((3, 3, 8), 20, np.int32)     # eight green bricks: stops after the fifth
+    np.testing.assert_allclose(_paging_centre(long_cheb, long_vaddr, 10), [0.0, 0.96875, 0.0, 0.96875],
+                               atol=1e-6)
+    long_vaddr[1, 1, 5] = -1                          # a fault past the stop changes nothing
+    np.testing.assert_allclose(_paging_centre(long_cheb, long_vaddr, 10), [0.0, 0.96875, 0.0, 0.96875],
+                               atol=1e-6)

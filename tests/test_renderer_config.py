@@ -24,7 +24,7 @@ README_CONFIG = {
 
 
 def test_registered_renderers():
-    assert R.candidates() == ["Blocks", "Isosurface", "Volume"]
+    assert R.candidates() == ["Blocks", "Isosurface", "Paging", "Volume"]
 
 
 def test_readme_config_and_renderer_config():
@@ -56,7 +56,7 @@ def test_ptu_camera_and_fovy():
 
 
 @pytest.mark.parametrize("cfg, message", [
-    ({"renderer": "Paging"}, b"unknown renderer"),
+    ({"renderer": "Raytrace"}, b"unknown renderer"),
     ({"resolution": [512, 512]}, b"missing \"renderer\""),
     ({"renderer": "Volume", "resolution": [0, 512]}, b"resolution"),
     ({"renderer": "Volume", "resolution": [512]}, b"resolution"),
