@@ -403,8 +403,8 @@ def main():
                     help="profiling aid: copy a MIB-sized buffer 3 times (scales FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sort-last design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sort-last rendering is not provided by this library", file=sys.stderr)
         sys.exit(2)
 
     import torch
