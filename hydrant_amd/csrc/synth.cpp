@@ -249,7 +249,10 @@ int32_t hyd_pack_bricks(const void *vol, int32_t fmt_in, const int32_t dims[3], 
   HYD_REQUIRE(fmt_out != HYD_FMT_U8 || fmt_in == HYD_FMT_U8, HYD_ERR_ARG, "u8 bricks need a u8 volume");
   const int g[3] = {(dims[0] + bs - 1) / bs, (dims[1] + bs - 1) / bs, (dims[2] + bs - 1) / bs};
   const int total = g[0] * g[1] * g[2];
-  const int count = bricks ? n_bricks : total;
+  // NULL stands for the bricks 0 .. n_bricks - 1, so an empty list packs nothing
+  HYD_REQUIRE(n_bricks >= 0 && (bricks || n_bricks <= total), HYD_ERR_RANGE,
+              "hyd_pack_bricks: %d bricks requested from a grid of %d", n_bricks, total);
+  const int count = n_bricks;
   if (bricks)
     for (int i = 0; i < count; ++i)
       HYD_REQUIRE(bricks[i] >= 0 && bricks[i] < total, HYD_ERR_RANGE, "brick index %d out of range", bricks[i]);

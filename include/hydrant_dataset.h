@@ -55,8 +55,9 @@ float hyd_f16_to_f32(uint16_t h);
  * normalize: for u8 input and f16/f32 output, 1 stores float(v) / 255 (the
  *            reference's NormalizedFloat read), 0 stores the count v itself,
  *            which f16 holds exactly (renderers then scale by 1/255).
- * bricks: optional list of n_bricks linear brick indices to pack (page order);
- * NULL packs every brick of the grid. */
+ * bricks: list of n_bricks linear brick indices to pack (page order); NULL
+ * stands for the bricks 0 .. n_bricks - 1 (n_bricks = the grid's brick count
+ * packs every brick).  out receives n_bricks pages; n_bricks = 0 packs nothing. */
 int32_t hyd_pack_bricks(const void *vol, int32_t fmt_in, const int32_t dims[3], int32_t bs, int32_t apron,
                         int32_t fmt_out, int32_t normalize, const int32_t *bricks, int32_t n_bricks, void *out);
 

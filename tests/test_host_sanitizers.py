@@ -1,5 +1,6 @@
-"""Host code under AddressSanitizer and UBSan: the renderer-config JSON parser
-and the culling / residency bookkeeping, driven by the fuzzers in tests/fuzz/.
+"""Host code under AddressSanitizer and UBSan: the renderer-config JSON parser,
+the culling / residency bookkeeping and the dataset preparation, driven by the
+fuzzers in tests/fuzz/.
 CPU only: the drivers make no GPU call."""
 import shutil
 import subprocess
@@ -36,3 +37,13 @@ def test_residency_fuzz_asan(tmp_path):
                 str(FUZZ / "residency_fuzz.cpp"), str(ROOT / "hydrant_amd" / "csrc" / "residency.cpp"), "-o", str(exe)],
                exe, tmp_path)
     assert out.startswith("culled ")
+
+
+@pytest.mark.skipif(not Path(HIPCC).exists(), reason="hipcc not available")
+def test_dataset_fuzz_asan(tmp_path):
+    exe = tmp_path / "dataset_fuzz"
+    out = _run([HIPCC, "-std=c++17", "-O1", "-g", "-Xarch_host", "-fsanitize=address", "-Xarch_host",
+                "-fsanitize=undefined", f"-I{ROOT / 'include'}", f"-I{ROOT / 'hydrant_amd' / 'csrc'}",
+                str(FUZZ / "dataset_fuzz.cpp"), str(ROOT / "hydrant_amd" / "csrc" / "synth.cpp"), "-o", str(exe)],
+               exe, tmp_path)
+    assert out.startswith("bricks ")
