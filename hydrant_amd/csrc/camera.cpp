@@ -153,14 +153,24 @@ int32_t hyd_orbit_camera(const float arm[3], const float center[3], float fovy, 
 
 int32_t hyd_make_view(const hyd_camera *cam, const float ex_center[3], const float ex_size[3], hyd_view *out) {
   HYD_REQUIRE(cam && ex_center && ex_size && out, HYD_ERR_ARG, "hyd_make_view: null argument");
+  HYD_REQUIRE(cam->fovy > 0.f && cam->fovy < 180.f, HYD_ERR_ARG, "fovy %g outside (0, 180) degrees", cam->fovy);
   const M4 iet = exhibit_iet(load3(ex_center), load3(ex_size));
   const M4 ivt = inverse(look_at(load3(cam->position), load3(cam->target), load3(cam->up)));
   const M4 trans = matmul(iet, ivt);
-  std::memcpy(out->trans, trans.m, sizeof(trans.m));
-  out->ctg_fovy_2 = 1.f / std::tan(radians(cam->fovy / 2.f));
+  hyd_view v{};
+  std::memcpy(v.trans, trans.m, sizeof(trans.m));
+  v.ctg_fovy_2 = 1.f / std::tan(radians(cam->fovy / 2.f));
   float o4[4];
   mat4_vec4(iet.m, cam->position[0], cam->position[1], cam->position[2], 1.f, o4);
-  out->ray_o[0] = o4[0]; out->ray_o[1] = o4[1]; out->ray_o[2] = o4[2];
+  v.ray_o[0] = o4[0]; v.ray_o[1] = o4[1]; v.ray_o[2] = o4[2];
+  // the reference renders NaN rays here; a degenerate camera (position on the
+  // target, up along the view direction) is refused instead
+  bool finite = std::isfinite(v.ctg_fovy_2);
+  for (int i = 0; i < 16; ++i) finite = finite && std::isfinite(v.trans[i]);
+  for (int i = 0; i < 3; ++i) finite = finite && std::isfinite(v.ray_o[i]);
+  HYD_REQUIRE(finite, HYD_ERR_ARG,
+              "degenerate camera: position equals target, or up is parallel to the view direction");
+  *out = v;
   return HYD_OK;
 }
 

@@ -94,3 +94,18 @@ def test_json_syntax():
     assert R.parse_config(text) == ("Volume", 64, 128)
     cam = R.parse_camera('{"orbit": {"arm": [2.5e1, 6.5E1, 22e-1]}}')
     assert np.allclose(list(cam.position), [0.842649, 1.993877, -0.392934], atol=1e-5)
+
+
+@pytest.mark.parametrize("cam, message", [
+    (dict(position=(1, 1, 1), target=(1, 1, 1)), "degenerate camera"),      # position on the target
+    (dict(position=(0, 3, 0), up=(0, 1, 0)), "degenerate camera"),          # up along the view direction
+    (dict(position=(0, 0, 3), fovy=0.0), "fovy"),
+    (dict(position=(0, 0, 3), fovy=180.0), "fovy"),
+])
+def test_degenerate_views_are_refused(cam, message):
+    from hydrant_amd.hip import make_view, ptu_camera
+
+    with pytest.raises(L.HydError, match=message):
+        make_view(ptu_camera(**cam), (0.5, 0.5, 0.5), (1, 1, 1))
+    v = make_view(ptu_camera((0, 0, 3)), (0.5, 0.5, 0.5), (1, 1, 1))
+    assert np.all(np.isfinite(list(v.trans)))
