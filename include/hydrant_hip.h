@@ -167,6 +167,9 @@ int32_t hyd_synchronize(hyd_ctx *ctx);
 int32_t hyd_orbit_camera(const float arm[3], const float center[3], float fovy, hyd_camera *out);
 /* Raycaster::fill_ray_emit_args, raycaster.hpp:126-135, with Exhibit::get_iet
  * (scene.hpp:20-29) for an exhibit of the given centre and size. */
+/* A camera whose position equals its target, whose up is parallel to the view
+ * direction, or whose fovy lies outside (0, 180) is refused with HYD_ERR_ARG
+ * (the reference renders NaN rays for it). */
 int32_t hyd_make_view(const hyd_camera *cam, const float ex_center[3], const float ex_size[3], hyd_view *out);
 /* Exhibit::get_iet and glm inverse (for the isosurface to_world). */
 int32_t hyd_exhibit_iet(const float ex_center[3], const float ex_size[3], float out[16]);
