@@ -1,4 +1,5 @@
 // C ABI implementation: context, resident state and pass dispatch.
+#include <cmath>
 #include <cstdarg>
 #include <cstring>
 #include <vector>
@@ -85,6 +86,32 @@ void fill_view(const hyd_view *v, float trans[16], float &ctg, f3 &ray_o) {
 
 }  // namespace
 
+namespace {
+// linear_to_srgb (volume.cc:175-181) then saturate (glm_math.hpp:75-81), binary32
+// on the host as the reference evaluates it.
+uint8_t srgb8_host(float x) {
+  const float s = x <= 0.0031308f ? 12.92f * x : 1.055f * std::pow(x, 1.f / 2.4f) - 0.055f;
+  return (uint8_t)(int)gclamp(s * 255.f, 0.f, 255.f);
+}
+}  // namespace
+
+void srgb8_thresholds(float t[256]) {
+  // the byte is non-decreasing in x: bisect the bit patterns of [0, +inf] for
+  // the first float of each byte value (x = 1 gives 254: 1.055f - 0.055f rounds
+  // below 1)
+  t[0] = -INFINITY;
+  for (int k = 1; k < 256; ++k) {
+    uint32_t lo = 0, hi = 0x7f800000u;   // byte(lo) < k <= byte(hi)
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      float x;
+      std::memcpy(&x, &mid, 4);
+      if (srgb8_host(x) >= k) hi = mid; else lo = mid;
+    }
+    std::memcpy(&t[k], &hi, 4);
+  }
+}
+
 int32_t activate(Ctx *c) {
   HYD_REQUIRE(c, HYD_ERR_ARG, "null context");
   HYD_CHECK_HIP(hipSetDevice(c->device));
@@ -121,11 +148,14 @@ int32_t hyd_create(int32_t device, hyd_ctx **out) {
   c->stream = c->own_stream;
   // float(v) / 255.f for every u8 value: the normalisation Texture3D<u8>
   // applies on upload (texture_3d.hpp:78-86, glm_math.hpp:83-89).
-  float lut[256];
+  // Then the sRGB byte thresholds of the composite.
+  float lut[512];
   for (int i = 0; i < 256; ++i) lut[i] = float(i) / 255.f;
+  srgb8_thresholds(lut + 256);
   int32_t st = upload(c->own_lut, lut, sizeof(lut), c->stream);
   if (st != HYD_OK) { (void)hipStreamDestroy(c->own_stream); delete c; return st; }
   c->u8_lut = c->own_lut;
+  c->srgb8_t = c->own_lut + 256;
   *out = c;
   return HYD_OK;
 }
@@ -344,7 +374,44 @@ int32_t hyd_composite(hyd_ctx *c, int32_t n, const hyd_fetch_pixel *partials_dev
   HYD_CHECK_HIP(hipMallocAsync((void **)&zdev, n * sizeof(int32_t), c->stream));
   HYD_CHECK_HIP(hipMemcpyAsync(zdev, z_order, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
   hipError_t e = launch_composite(n, partials_dev, zdev, width, height, reinterpret_cast<float4 *>(rgba_dev),
-                                  rgb_dev, c->stream);
+                                  rgb_dev, c->srgb8_t, c->stream);
+  (void)hipFreeAsync(zdev, c->stream);
+  HYD_CHECK_HIP(e);
+  HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_srgb8_thresholds(float out[256]) {
+  HYD_REQUIRE(out, HYD_ERR_ARG, "null output");
+  srgb8_thresholds(out);
+  return HYD_OK;
+}
+
+int32_t hyd_iso_fetch_pass(hyd_ctx *c, const hyd_iso_pixel *px_dev, int32_t width, int32_t height,
+                           hyd_iso_fetch_pixel *out_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(px_dev && out_dev && width > 0 && height > 0, HYD_ERR_ARG, "invalid isosurface fetch arguments");
+  HYD_CHECK_HIP(launch_iso_fetch(px_dev, width, height, out_dev, c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_composite_iso(hyd_ctx *c, int32_t n, const hyd_iso_fetch_pixel *partials_dev, const int32_t *z_order,
+                          int32_t width, int32_t height, uint8_t *rgb_dev, float *depth_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(n >= 1 && partials_dev && z_order && width > 0 && height > 0 && (rgb_dev || depth_dev), HYD_ERR_ARG,
+              "invalid isosurface composite arguments");
+  std::vector<char> seen(n, 0);
+  for (int i = 0; i < n; ++i) {
+    HYD_REQUIRE(z_order[i] >= 0 && z_order[i] < n && !seen[z_order[i]], HYD_ERR_ARG,
+                "z_order must be a permutation of 0..%d", n - 1);
+    seen[z_order[i]] = 1;
+  }
+  int32_t *zdev = nullptr;
+  HYD_CHECK_HIP(hipMallocAsync((void **)&zdev, n * sizeof(int32_t), c->stream));
+  HYD_CHECK_HIP(hipMemcpyAsync(zdev, z_order, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  hipError_t e = launch_iso_composite(n, partials_dev, zdev, width, height, rgb_dev, depth_dev, c->stream);
   (void)hipFreeAsync(zdev, c->stream);
   HYD_CHECK_HIP(e);
   HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
@@ -473,6 +540,8 @@ int32_t hyd_render_iso(hyd_ctx *c, const hyd_iso_params *p, const hyd_view *view
   if (st) return st;
   HYD_REQUIRE(c->pool.data && c->paging.vaddr && c->cheb.data, HYD_ERR_STATE,
               "isosurface needs pool, page table and empty-space grid");
+  HYD_REQUIRE(c->paging.n_pages <= c->pool.n_pages, HYD_ERR_STATE,
+              "page table names %d pages but the pool holds %d", c->paging.n_pages, c->pool.n_pages);
   HYD_REQUIRE(p && view && out_dev && width > 0 && height > 0, HYD_ERR_ARG, "invalid isosurface arguments");
   HYD_REQUIRE(p->step > 0.f, HYD_ERR_ARG, "step must be > 0");
   HYD_REQUIRE(p->bisect_steps >= 0 && p->bisect_steps <= 64, HYD_ERR_ARG, "bisect_steps out of range");

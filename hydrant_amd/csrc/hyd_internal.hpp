@@ -86,7 +86,8 @@ struct Ctx {
   float *own_mean = nullptr;
   float *own_tf = nullptr;
   const float *u8_lut = nullptr;  // 256-entry float(v)/255 table
-  float *own_lut = nullptr;
+  const float *srgb8_t = nullptr; // 256 sRGB byte thresholds (srgb8_thresholds)
+  float *own_lut = nullptr;       // both tables, 512 floats
   // multi-volume set of the batched renderer
   void *own_vols = nullptr;       // BatchVolDev[n_vols]
   int32_t *own_cells = nullptr;   // concatenated cell codes
@@ -149,7 +150,13 @@ hipError_t launch_volume_trace(const VolumeArgs &a, int fmt, int x, int y, hyd_t
 hipError_t launch_pixel_pass(const float4 *rgba, int W, int H, uchar3 clear, uint8_t *rgb, hipStream_t s);
 hipError_t launch_fetch_pass(const hyd_volume_pixel *film, int W, int H, hyd_fetch_pixel *out, hipStream_t s);
 hipError_t launch_composite(int n, const hyd_fetch_pixel *partials, const int32_t *z_order_dev, int W, int H,
-                            float4 *rgba, uint8_t *rgb, hipStream_t s);
+                            float4 *rgba, uint8_t *rgb, const float *srgb_t, hipStream_t s);
+hipError_t launch_iso_fetch(const hyd_iso_pixel *px, int W, int H, hyd_iso_fetch_pixel *out, hipStream_t s);
+hipError_t launch_iso_composite(int n, const hyd_iso_fetch_pixel *partials, const int32_t *z_order_dev, int W, int H,
+                                uint8_t *rgb, float *depth, hipStream_t s);
+// t[0] = -inf, t[k] (k = 1..255) = least float whose saturated sRGB byte is >= k
+// (volume.cc:175-181,284-288 evaluated on the host).
+void srgb8_thresholds(float t[256]);
 
 struct BlocksArgs {
   f3 bmin, bmax;

@@ -187,7 +187,9 @@ int32_t hyd_residency_commit(hyd_residency *res, int32_t idx, int32_t *slot) {
   if (res->n_slots < res->max_pages) {
     s = res->first_slot + res->n_slots++;
   } else if (!res->redundant.empty()) {
-    const int32_t out = res->redundant.back();   // swap out the last redundant brick
+    // swap out the last redundant brick; the reference's std::set<vol::Idx> is
+    // assumed to order bricks by linear index (z, then y, then x)
+    const int32_t out = res->redundant.back();
     res->redundant.pop_back();
     res->present.erase(std::lower_bound(res->present.begin(), res->present.end(), out));
     s = res->vaddr[out];

@@ -60,6 +60,13 @@ class DiffRenderer:
         if tuple(grad_rgba.shape) != (self.n_views, self.H, self.W, 4):
             raise ValueError(f"grad_rgba must have shape {(self.n_views, self.H, self.W, 4)}; "
                              f"got {tuple(grad_rgba.shape)}")
+        import torch
+
+        if grad_rgba.dtype != torch.float32:
+            raise ValueError(f"grad_rgba must be float32 (the adjoint reads float4 pixels); got {grad_rgba.dtype}")
+        if any(x.device != grad_rgba.device for x in (volume, tf, ck)):
+            raise ValueError("volume, tf, checkpoints and grad_rgba must live on one device; got "
+                             f"{volume.device}, {tf.device}, {ck.device}, {grad_rgba.device}")
         t = self.ctx.torch
         self.ctx.sync_stream()
         gv = t.empty_like(volume)   # both outputs are overwritten by the adjoint

@@ -155,8 +155,8 @@ class Context:
 
     def resume_volume(self, params: L.VolumeParams, film, count: bool = False):
         t = self.torch
+        H, W = check_image(film, 16, "film", self.dev)
         self.sync_stream()
-        H, W = film.shape[0], film.shape[1]
         rgba_t = t.empty((H, W, 4), dtype=t.float32, device=self.dev)
         cnt = t.zeros(1, dtype=t.int64, device=self.dev) if count else None
         L.call("hyd_resume_volume", self.h, C.byref(params), W, H, C.c_void_p(film.data_ptr()),
@@ -165,8 +165,8 @@ class Context:
 
     def pixel_pass(self, rgba, clear=(0, 0, 0)):
         t = self.torch
+        H, W = check_image(rgba, 4, "rgba", self.dev)
         self.sync_stream()
-        H, W = rgba.shape[0], rgba.shape[1]
         out = t.empty((H, W, 3), dtype=t.uint8, device=self.dev)
         cc = (C.c_uint8 * 3)(*clear)
         L.call("hyd_pixel_pass", self.h, C.c_void_p(rgba.data_ptr()), W, H, cc, C.c_void_p(out.data_ptr()))
@@ -174,8 +174,8 @@ class Context:
 
     def fetch_pass(self, film):
         t = self.torch
+        H, W = check_image(film, 16, "film", self.dev)
         self.sync_stream()
-        H, W = film.shape[0], film.shape[1]
         out = t.empty((H, W, 8), dtype=t.float32, device=self.dev)
         L.call("hyd_fetch_pass", self.h, C.c_void_p(film.data_ptr()), W, H, C.c_void_p(out.data_ptr()))
         return out
@@ -183,14 +183,43 @@ class Context:
     def composite(self, partials, z_order):
         """partials: (N, H, W, 8) fetch pixels; returns (rgba, rgb8)."""
         t = self.torch
+        if partials.dim() != 4:
+            raise ValueError(f"partials must have shape (N, H, W, 8); got {tuple(partials.shape)}")
+        n = partials.shape[0]
+        H, W = check_image(partials, 8, "partials", self.dev, lead=n)
         self.sync_stream()
-        n, H, W = partials.shape[0], partials.shape[1], partials.shape[2]
         z = np.ascontiguousarray(z_order, np.int32)
         rgba = t.empty((H, W, 4), dtype=t.float32, device=self.dev)
         rgb = t.empty((H, W, 3), dtype=t.uint8, device=self.dev)
         L.call("hyd_composite", self.h, n, C.c_void_p(partials.data_ptr()), L.ptr(z, C.c_int32), W, H,
                C.c_void_p(rgba.data_ptr()), C.c_void_p(rgb.data_ptr()))
         return rgba, rgb
+
+    def iso_fetch_pass(self, px):
+        """IsosurfaceShaderKernel::fetch of an (H, W, 6) isosurface output: (H, W, 8) uint8
+        records (3 colour bytes, 1 pad byte, float32 depth bytes)."""
+        t = self.torch
+        H, W = check_image(px, 6, "isosurface pixels", self.dev)
+        self.sync_stream()
+        out = t.empty((H, W, 8), dtype=t.uint8, device=self.dev)
+        L.call("hyd_iso_fetch_pass", self.h, C.c_void_p(px.data_ptr()), W, H, C.c_void_p(out.data_ptr()))
+        return out
+
+    def composite_iso(self, partials, z_order):
+        """partials: (N, H, W, 8) uint8 isosurface fetch records; returns (rgb8, depth)."""
+        t = self.torch
+        if partials.dim() != 4 or partials.shape[-1] != 8 or partials.dtype != t.uint8 or \
+                not partials.is_contiguous() or partials.device != self.dev:
+            raise ValueError(f"partials must be a contiguous uint8 (N, H, W, 8) tensor on {self.dev}; "
+                             f"got {partials.dtype} {tuple(partials.shape)} on {partials.device}")
+        n, H, W = partials.shape[0], partials.shape[1], partials.shape[2]
+        self.sync_stream()
+        z = np.ascontiguousarray(z_order, np.int32)
+        rgb = t.empty((H, W, 3), dtype=t.uint8, device=self.dev)
+        depth = t.empty((H, W), dtype=t.float32, device=self.dev)
+        L.call("hyd_composite_iso", self.h, n, C.c_void_p(partials.data_ptr()), L.ptr(z, C.c_int32), W, H,
+               C.c_void_p(rgb.data_ptr()), C.c_void_p(depth.data_ptr()))
+        return rgb, depth
 
     def render_sort_last(self, params: L.VolumeParams, view: L.View, W: int, H: int, batches, grid_dims,
                          count: bool = False):
@@ -278,6 +307,31 @@ def check_output(out, n_floats: int, what: str):
                          f"got {out.dtype} {tuple(out.shape)}")
     if not out.is_cuda:
         raise ValueError(f"{what} must be a device tensor")
+
+
+def srgb8_thresholds() -> np.ndarray:
+    """The 256 sRGB byte thresholds hyd_composite uses (host computation)."""
+    t = np.zeros(256, np.float32)
+    L.call("hyd_srgb8_thresholds", L.ptr(t, C.c_float))
+    return t
+
+
+def check_image(x, per_px: int, what: str, dev, lead: int = 1):
+    """Kernels read `x` as `lead` images of H x W pixels of `per_px` floats: refuse a tensor
+    of another type, device, layout or size before it reaches one.  Returns (H, W)."""
+    import torch
+
+    shape = tuple(x.shape)
+    if x.dim() < 3 or shape[-1] != per_px:
+        raise ValueError(f"{what} must have shape (..., H, W, {per_px}); got {shape}")
+    H, W = int(shape[-3]), int(shape[-2])
+    if x.dtype != torch.float32 or not x.is_contiguous():
+        raise ValueError(f"{what} must be a contiguous float32 tensor; got {x.dtype} {shape}")
+    if not x.is_cuda or x.device != torch.device(dev):
+        raise ValueError(f"{what} must live on device {dev}; got {x.device}")
+    if H <= 0 or W <= 0 or x.numel() != lead * H * W * per_px:
+        raise ValueError(f"{what} must hold {lead} x {H} x {W} x {per_px} floats; got {shape}")
+    return H, W
 
 
 class Batch:

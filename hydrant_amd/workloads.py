@@ -39,6 +39,7 @@ class Scene:
     schedule: int = L.SCHED_REFERENCE
     ert: float = 0.0
     mode: int = L.MODE_DEFAULT
+    rank: float = 0.0          # Partition mode colour parameter
     extra: dict = field(default_factory=dict)
 
     @property
@@ -60,7 +61,7 @@ class Scene:
     def volume_params(self):
         from .hip import volume_params
         return volume_params(self.bbox_min, self.bbox_max, self.geom.step, self.geom.du, self.max_steps,
-                             self.mode, 0.0, self.density, self.schedule, self.ert)
+                             self.mode, self.rank, self.density, self.schedule, self.ert)
 
 
 def plain_volume(n: int = 256, W: int = 512, H: int = 512, fmt: int = L.FMT_F32, density: float = 1e-2,

@@ -150,6 +150,14 @@ typedef struct {
   int32_t hit;
 } hyd_iso_pixel;
 
+/* IsosurfaceFetchPixel, isosurface_shader.hpp:17-21 (8 bytes: uchar3, padding,
+ * float depth). */
+typedef struct {
+  uint8_t val[3];
+  uint8_t pad;
+  float depth;
+} hyd_iso_fetch_pixel;
+
 /* ---------------------------------------------------------------- core -- */
 const char *hyd_last_error(void);
 int32_t hyd_version(void);
@@ -233,6 +241,22 @@ int32_t hyd_fetch_pass(hyd_ctx *ctx, const hyd_volume_pixel *film_dev, int32_t w
  * sRGB + saturate (volume.cc:267-292).  rgba_dev / rgb_dev may be NULL. */
 int32_t hyd_composite(hyd_ctx *ctx, int32_t n, const hyd_fetch_pixel *partials_dev, const int32_t *z_order,
                       int32_t width, int32_t height, float *rgba_dev, uint8_t *rgb_dev);
+/* The bytes hyd_composite writes: t[0] = -inf and, for k = 1..255, t[k] = the
+ * least float whose saturate(linear_to_srgb(x)) byte (volume.cc:175-181,284-288,
+ * binary32 on the host) is k or more; the composite writes the number of t[k]
+ * at or below each channel.  Host only. */
+int32_t hyd_srgb8_thresholds(float out[256]);
+/* Isosurface fetch pass (IsosurfaceShaderKernel::fetch, isosurface_shader.cu:42-49):
+ * bytes saturate(v.rgb) and the depth of every pixel. */
+int32_t hyd_iso_fetch_pass(hyd_ctx *ctx, const hyd_iso_pixel *px_dev, int32_t width, int32_t height,
+                           hyd_iso_fetch_pixel *out_dev);
+/* In-GPU isosurface sort-last composite (isosurface.cc:272-289) of n partial
+ * images (n images of width*height back to back) in z_order (host array of n
+ * indices): starting from image z_order[0], each later image replaces a pixel
+ * whose depth is strictly greater.  rgb_dev (3 bytes per pixel) and depth_dev
+ * (float per pixel) may each be NULL, not both.  Synchronous. */
+int32_t hyd_composite_iso(hyd_ctx *ctx, int32_t n, const hyd_iso_fetch_pixel *partials_dev, const int32_t *z_order,
+                          int32_t width, int32_t height, uint8_t *rgb_dev, float *depth_dev);
 /* Sort-last rendering over residency batches.  The reference renders a volume
  * larger than the pool in batches by pausing rays at non-resident bricks and
  * resuming them (volume.cc:95-134), which needs the bricks in ray order; here

@@ -66,7 +66,9 @@ int32_t hyd_residency_update(hyd_residency *res, const hyd_view *view, const int
  * 249-291) for brick `idx`: *slot receives the pool slot its voxels go to, a
  * new slot while the pool has room, else the slot of the last redundant brick
  * (whose page-table entry reverts to its basic entry), or -1 when the brick is
- * already present ("abandoned") or no slot can be freed ("artifact"). */
+ * already present ("abandoned") or no slot can be freed ("artifact").  "Last"
+ * assumes the reference's std::set<vol::Idx> orders bricks by linear index
+ * (z, then y, then x): varch's Idx ordering is not available to check. */
 int32_t hyd_residency_commit(hyd_residency *res, int32_t idx, int32_t *slot);
 
 /* The current page table (dims-sized, x fastest): slot of each resident brick,

@@ -299,6 +299,49 @@ def composite(partials: np.ndarray, z_order) -> tuple[np.ndarray, np.ndarray]:
     return rgba, rgb
 
 
+def srgb8_check(thresholds: np.ndarray, nthreads=0) -> tuple[int, int]:
+    """Mismatches of a 256-entry sRGB byte threshold table against the oracle's bytes
+    over every float in [0, 2] plus special values; (count, bits of the first)."""
+    L = lib()
+    if not getattr(L, "_srgb_typed", False):
+        L.orc_srgb8_check.argtypes = [C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_uint)]
+        L.orc_srgb8_check.restype = C.c_longlong
+        L._srgb_typed = True
+    t = np.ascontiguousarray(thresholds, np.float32)
+    fb = C.c_uint(0)
+    n = L.orc_srgb8_check(_pf(t), int(nthreads), C.byref(fb))
+    return int(n), int(fb.value)
+
+
+def iso_fetch(px: np.ndarray):
+    """IsosurfaceShaderKernel::fetch of (H, W, 6) isosurface pixels: (bytes (H, W, 3), depth (H, W))."""
+    px = np.ascontiguousarray(px, np.float64)
+    H, W = px.shape[:2]
+    val = np.zeros((H, W, 3), np.uint8)
+    depth = np.zeros((H, W), np.float32)
+    L = lib()
+    L.orc_iso_fetch.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_ubyte), C.POINTER(C.c_float)]
+    L.orc_iso_fetch(H * W, _pd(px), val.ctypes.data_as(C.POINTER(C.c_ubyte)), _pf(depth))
+    return val, depth
+
+
+def composite_iso(val: np.ndarray, depth: np.ndarray, z_order):
+    """Depth-min sort-last composite of N isosurface partials: val (N, H, W, 3) bytes,
+    depth (N, H, W); returns (rgb (H, W, 3), depth (H, W))."""
+    val = np.ascontiguousarray(val, np.uint8)
+    depth = np.ascontiguousarray(depth, np.float32)
+    N, H, W = depth.shape
+    z = np.ascontiguousarray(z_order, np.int32)
+    rgb = np.zeros((H, W, 3), np.uint8)
+    dout = np.zeros((H, W), np.float32)
+    L = lib()
+    L.orc_composite_iso.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_ubyte), C.POINTER(C.c_float),
+                                    C.POINTER(C.c_int), C.POINTER(C.c_ubyte), C.POINTER(C.c_float)]
+    L.orc_composite_iso(N, W, H, val.ctypes.data_as(C.POINTER(C.c_ubyte)), _pf(depth), _pi(z),
+                        rgb.ctypes.data_as(C.POINTER(C.c_ubyte)), _pf(dout))
+    return rgb, dout
+
+
 def blocks_render(view: OView, W, H, bbox_min, bbox_max, step, cheb, mean, mode=0, density=1e-2,
                   max_steps=4_000_000, prec=32, nthreads=0, vaddr=None, lowest=0) -> np.ndarray:
     """Blocks.Volume (mode 0) / Blocks.Solid (1), or the Paging view (2) over page table `vaddr` (z, y, x)."""

@@ -401,6 +401,84 @@ int orc_composite(int N, int W, int H, const float *partials, const int *z_order
   return 0;
 }
 
+// Checks a table t[0..255] of sRGB byte thresholds against saturate(linear_to_srgb(x))
+// (volume.cc:175-181,284-288): for every float x in [0, 2] and a sample of negative,
+// large and special values, the byte equals the number of t[1..255] at or below x.
+// Returns the number of mismatches; *first_bad gets the bits of the first one.
+long long orc_srgb8_check(const float *t, int nthreads, unsigned *first_bad) {
+  auto byte_ref = [](float x) { return (unsigned)(unsigned char)gclamp(srgb_f(x) * 255.f, 0.f, 255.f); };
+  auto byte_tab = [&](float x) {
+    unsigned k = 0;
+    for (int i = 1; i < 256; ++i) k += t[i] <= x;
+    return k;
+  };
+  const int n = resolve_threads(nthreads);
+  const uint32_t end = 0x40000000u;   // 2.0f
+  std::vector<long long> bad(n, 0);
+  std::vector<unsigned> first(n, ~0u);
+  std::vector<std::thread> th;
+  for (int w = 0; w < n; ++w)
+    th.emplace_back([&, w] {
+      // thresholds are increasing: walk the range keeping the table byte incrementally
+      const uint32_t lo = (uint32_t)((uint64_t)end * w / n), hi = (uint32_t)((uint64_t)end * (w + 1) / n);
+      float x0;
+      std::memcpy(&x0, &lo, 4);
+      unsigned k = byte_tab(x0);
+      for (uint32_t u = lo; u < hi; ++u) {
+        float x;
+        std::memcpy(&x, &u, 4);
+        while (k < 255 && t[k + 1] <= x) ++k;
+        if (byte_ref(x) != k) {
+          if (!bad[w]) first[w] = u;
+          ++bad[w];
+        }
+      }
+    });
+  for (auto &x : th) x.join();
+  long long total = 0;
+  unsigned fb = ~0u;
+  for (int w = 0; w < n; ++w) {
+    total += bad[w];
+    if (bad[w] && fb == ~0u) fb = first[w];
+  }
+  const float extra[] = {-0.0f, -1e-30f, -0.5f, -1.f, -1e30f, 2.5f, 10.f, 1e30f, INFINITY, -INFINITY};
+  for (float x : extra)
+    if (byte_ref(x) != byte_tab(x)) {
+      if (!total) std::memcpy(&fb, &x, 4);
+      ++total;
+    }
+  if (first_bad) *first_bad = fb;
+  return total;
+}
+
+// IsosurfaceShaderKernel::fetch (isosurface_shader.cu:42-49): saturate(v) bytes
+// and depth of each isosurface pixel (6 doubles: rgba, depth, hit).
+int orc_iso_fetch(int n, const double *px, unsigned char *val, float *depth) {
+  for (int i = 0; i < n; ++i) {
+    for (int c = 0; c < 3; ++c) val[3 * i + c] = (unsigned char)gclamp(float(px[6 * i + c]) * 255.f, 0.f, 255.f);
+    depth[i] = float(px[6 * i + 4]);
+  }
+  return 0;
+}
+
+// Isosurface sort-last composite (isosurface.cc:272-289): the front is the
+// partial of z_order[0]; a later partial in z order replaces a front pixel when
+// its depth is strictly smaller.  val: N*P*3 bytes, depth: N*P floats.
+int orc_composite_iso(int N, int W, int H, const unsigned char *val, const float *depth, const int *z_order,
+                      unsigned char *out_rgb, float *out_depth) {
+  const size_t P = (size_t)W * H;
+  for (size_t i = 0; i < P; ++i) {
+    size_t f = (size_t)z_order[0] * P + i;
+    for (int k = 1; k < N; ++k) {
+      const size_t b = (size_t)z_order[k] * P + i;
+      if (depth[b] < depth[f]) f = b;
+    }
+    for (int c = 0; c < 3; ++c) out_rgb[3 * i + c] = val[3 * f + c];
+    out_depth[i] = depth[f];
+  }
+  return 0;
+}
+
 int orc_blocks_render(int prec, int nthreads, const orc_view *view, const orc_blocks_args *a, double *rgba) {
   auto run = [&](auto tag) {
     using T = decltype(tag);

@@ -13,7 +13,7 @@ def oracle_volume(sc: Scene, schedule=None, vaddr=None, pool_values=None, exact_
     return O.VolumeScene(sc.W, sc.H, sc.bbox_min, sc.bbox_max, sc.geom.step, sc.geom.du, sc.tf, b.cheb,
                          b.vaddr if vaddr is None else vaddr, b.page_k, b.page_b,
                          b.pool_values() if pool_values is None else pool_values, b.pad,
-                         max_steps=sc.max_steps, mode=sc.mode, density=sc.density,
+                         max_steps=sc.max_steps, mode=sc.mode, rank=sc.rank, density=sc.density,
                          schedule=sc.schedule if schedule is None else schedule, ert=sc.ert,
                          lowest_blkcnt=b.lowest_blkcnt, map_bs=b.bs if exact_mapping else 0)
 
@@ -43,7 +43,8 @@ def error_pixels(W: int, H: int) -> np.ndarray:
 
 
 def oracle_view(sc: Scene) -> O.OView:
-    """The product's binary32 view handed to the oracle (the camera parity test pins that both agree)."""
+    """The product's binary32 view handed to the oracle (tests/test_camera_parity.py checks the
+    library's camera against the oracle's, bit for bit)."""
     return O.view_from_dict(hip.view_to_dict(sc.view()))
 
 

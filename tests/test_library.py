@@ -72,6 +72,10 @@ def test_diff_wrapper_rejects_mismatched_tensors():
         r.backward_raw(vol, tf, ck[:, :, :8], g)
     with pytest.raises(ValueError, match="grad_rgba"):
         r.backward_raw(vol, tf, ck, g[:1])
+    with pytest.raises(ValueError, match="float32"):
+        r.backward_raw(vol, tf, ck, g.half())
+    with pytest.raises(ValueError, match="float32"):
+        r.backward_raw(vol, tf, ck, g.double())
 
 
 def test_output_check():
@@ -87,3 +91,22 @@ def test_output_check():
         check_output(torch.zeros(100, 4, dtype=torch.float64), 400, "batched output")
     with pytest.raises(ValueError, match="device"):
         check_output(torch.zeros(100, 4), 400, "batched output")
+
+
+def test_image_input_check():
+    """The film, RGBA and partial-image inputs of the resume, pixel, fetch and composite
+    passes are checked for type, layout, device and size before they reach a kernel."""
+    import torch
+
+    from hydrant_amd.hip import check_image
+
+    with pytest.raises(ValueError, match="shape"):
+        check_image(torch.zeros(8, 16, 15), 16, "film", "cuda:0")
+    with pytest.raises(ValueError, match="float32"):
+        check_image(torch.zeros(8, 16, 16, dtype=torch.float16), 16, "film", "cuda:0")
+    with pytest.raises(ValueError, match="contiguous"):
+        check_image(torch.zeros(16, 8, 16).transpose(0, 1), 16, "film", "cuda:0")
+    with pytest.raises(ValueError, match="device"):
+        check_image(torch.zeros(8, 16, 16), 16, "film", "cuda:0")
+    with pytest.raises(ValueError, match="shape"):
+        check_image(torch.zeros(8, 16), 4, "rgba", "cuda:0")
