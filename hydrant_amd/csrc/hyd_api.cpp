@@ -73,6 +73,7 @@ int32_t fill_volume_args(const Ctx *c, const hyd_volume_params *p, VolumeArgs &a
   a.lowest_blkcnt = c->paging.lowest_blkcnt;
   a.page = c->paging.page;
   a.pool = c->pool.data;
+  a.pool_elems = (uint64_t)c->pool.n_pages * c->pool.pad * c->pool.pad * c->pool.pad;
   a.pad = c->pool.pad;
   a.u8_lut = c->u8_lut;
   return HYD_OK;
@@ -198,7 +199,7 @@ int32_t hyd_upload_pool(hyd_ctx *c, int32_t fmt, int32_t pad_edge, int32_t n_pag
   int32_t st = activate(c);
   if (st) return st;
   HYD_REQUIRE(fmt_bytes(fmt) > 0, HYD_ERR_ARG, "unknown brick format %d", fmt);
-  HYD_REQUIRE(pad_edge > 0 && pad_edge <= 1024, HYD_ERR_ARG, "pad_edge %d out of range", pad_edge);
+  HYD_REQUIRE(pad_edge >= 2 && pad_edge <= 1024, HYD_ERR_ARG, "pad_edge %d out of range [2, 1024]", pad_edge);
   HYD_REQUIRE(n_pages > 0 && host, HYD_ERR_ARG, "empty pool");
   const size_t bytes = (size_t)n_pages * pad_edge * pad_edge * pad_edge * fmt_bytes(fmt);
   st = upload(c->own_pool, host, bytes, c->stream);
@@ -213,7 +214,7 @@ int32_t hyd_bind_pool_device(hyd_ctx *c, int32_t fmt, int32_t pad_edge, int32_t 
   int32_t st = activate(c);
   if (st) return st;
   HYD_REQUIRE(fmt_bytes(fmt) > 0, HYD_ERR_ARG, "unknown brick format %d", fmt);
-  HYD_REQUIRE(pad_edge > 0 && pad_edge <= 1024 && n_pages > 0 && pool_dev, HYD_ERR_ARG, "invalid pool");
+  HYD_REQUIRE(pad_edge >= 2 && pad_edge <= 1024 && n_pages > 0 && pool_dev, HYD_ERR_ARG, "invalid pool");
   free_dev(c->own_pool);
   c->pool = Pool{pool_dev, fmt, pad_edge, n_pages, false};
   ++c->pool_gen;

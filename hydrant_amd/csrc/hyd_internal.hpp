@@ -128,6 +128,7 @@ struct VolumeArgs {
   int32_t lowest_blkcnt;
   const float4 *page;
   const void *pool;
+  uint64_t pool_elems;    // voxels in the pool (n_pages * pad^3)
   int32_t pad;
   const float *u8_lut;
   // outputs

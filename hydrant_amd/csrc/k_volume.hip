@@ -12,6 +12,9 @@
 // and the u8 normalisation table are staged in LDS per workgroup.
 #include <hip/hip_fp16.h>
 
+#include <cstdlib>
+#include <type_traits>
+
 #include "hyd_internal.hpp"
 
 namespace hyd {
@@ -133,8 +136,10 @@ struct MarchState {
 };
 
 // Shade one resolved sample and composite it front to back
-// (volume_shader.cu:52-70, plus the density scale on the TF alpha).
-template <bool WEIGHTED = false>
+// (volume_shader.cu:52-70, plus the density scale on the TF alpha).  theta and
+// phi are carried only for a film (the fetch pass exports them); without one
+// they are never observed and THETA = false leaves them out.
+template <bool WEIGHTED = false, bool THETA = true>
 __device__ __forceinline__ void shade_composite_impl(const VolumeArgs &a, const float *tf, float s, int pgid,
                                                      MarchState &st, float w) {
   float4 ub = tf_sample(tf, a.tf_n, s);
@@ -149,10 +154,12 @@ __device__ __forceinline__ void shade_composite_impl(const VolumeArgs &a, const 
   }
   const float aw = ub.w;
   ub.x = ub.x * aw; ub.y = ub.y * aw; ub.z = ub.z * aw;
-  st.theta.x = st.theta.x + ub.x * st.phi;
-  st.theta.y = st.theta.y + ub.y * st.phi;
-  st.theta.z = st.theta.z + ub.z * st.phi;
-  st.phi = st.phi * (1.f - ub.w);
+  if (THETA) {
+    st.theta.x = st.theta.x + ub.x * st.phi;
+    st.theta.y = st.theta.y + ub.y * st.phi;
+    st.theta.z = st.theta.z + ub.z * st.phi;
+    st.phi = st.phi * (1.f - ub.w);
+  }
   const float t = 1.f - st.v.w;
   st.v.x = st.v.x + ub.x * t;
   st.v.y = st.v.y + ub.y * t;
@@ -245,13 +252,256 @@ __device__ void march_reference(const VolumeArgs &a, const float *tf, const floa
   st.step_size = step_size;
 }
 
+// ---------------------------------------------------------------------------
+// The reference schedule with the tap loads of up to U consecutive samples in
+// flight together.  A ray is one long chain of dependent samples, and a frame
+// holds only a few waves per SIMD (one lane per pixel), so issuing one
+// sample's loads and waiting for them before the next leaves the memory
+// latency exposed on every sample.  Inside one brick the next sample positions
+// follow from the current step size alone (o += (d * step) * step_size, the
+// same binary32 operations one at a time), so a group of U samples resolves
+// its positions, tap offsets and weights first, issues the loads of every
+// sample whose cell differs from its predecessor's, and then composites the
+// samples in order.  Escalation changes the step size; a group stops at the
+// sample that escalates (the ones after it were placed with the old step and
+// are dropped), so positions, taps, values and step budget are those of the
+// one-sample-at-a-time march, bit for bit.  Samples are also cut at the
+// brick's boundary and at the end of the step budget.
+//
+// Voxels are addressed by element offsets from the pool base: 32-bit when the
+// pool is below 4 GiB (OFF = uint32_t, so the base stays in scalar registers),
+// else 64-bit.
+
+// Reference sampler taps of one sample: the element offsets of the four rows
+// (y, z) = (i, i), (j, i), (i, j), (j, j) of the padded brick, the x taps and the
+// weights (BlockSamplerMapping + CpuSampler Wrap/Linear, block_paging.hpp:11-36,
+// cpu_sampler.hpp:88-104,142-166).
+template <class OFF> struct RefTap {
+  OFF r00, r10, r01, r11;
+  int ix, jx;
+  float ax, ay, az;
+};
+
+// floor(fx) and ceil(fx) are integers, so clamping them to [0, pad - 1] as floats
+// (one med3) and converting gives the reference's clamped int taps.  Row
+// offsets below 2^24 voxels per page use 24-bit multiplies (pad <= 1024, so
+// z * pad + y < 2^20 and a row start < 2^30).
+template <class OFF>
+__device__ __forceinline__ RefTap<OFF> ref_tap(f3 x, float4 pg, int pad, OFF page_base) {
+  const float fd = float(pad);
+  const float hi = fd - 1.f;
+  const float u[3] = {x.x * pg.x + pg.y, x.y * pg.x + pg.z, x.z * pg.x + pg.w};
+  float a[3];
+  uint32_t i[3], j[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float fx = fd * u[k];
+    fx = wrap_mod(fx, fd);   // Wrap: glm mod(fx, fd)
+    fx = fx - 0.5f;
+    const float flr = floorf(fx);
+    a[k] = fx - flr;
+    i[k] = (uint32_t)__builtin_amdgcn_fmed3f(flr, 0.f, hi);
+    j[k] = (uint32_t)__builtin_amdgcn_fmed3f(ceilf(fx), 0.f, hi);
+  }
+  RefTap<OFF> t;
+  const uint32_t p = (uint32_t)pad;
+  t.r00 = page_base + (OFF)__umul24(__umul24(i[2], p) + i[1], p);
+  t.r10 = page_base + (OFF)__umul24(__umul24(i[2], p) + j[1], p);
+  t.r01 = page_base + (OFF)__umul24(__umul24(j[2], p) + i[1], p);
+  t.r11 = page_base + (OFF)__umul24(__umul24(j[2], p) + j[1], p);
+  t.ix = (int)i[0];
+  t.jx = (int)j[0];
+  t.ax = a[0];
+  t.ay = a[1];
+  t.az = a[2];
+  return t;
+}
+
+// Raw taps as loaded.  FP32 and FP16 rows are read as x pairs: the two voxels
+// at m = min(ix, pad - 2) and m + 1 in one load (8 bytes at 4-byte alignment
+// for FP32, 4 bytes at 2-byte alignment for FP16), which the vector memory
+// unit serves at the per-lane cost of one of the two single loads; the taps
+// are then v(ix) = ix == m ? lo : hi and v(jx) = jx == m + 1 ? hi : lo, which
+// also covers the clamped and integral cases (ix == jx).  u8 taps are single
+// bytes, one 32-bit register each (bytes packed into a register would make
+// each load wait for the previous ones).  Values are converted to float once,
+// when a sample uses them.
+template <int FMT> struct RawVox;
+template <> struct RawVox<HYD_FMT_F32> { using T = float; static constexpr int N = 8; };
+template <> struct RawVox<HYD_FMT_F16> { using T = uint16_t; static constexpr int N = 4; };
+template <> struct RawVox<HYD_FMT_U8> { using T = uint8_t; static constexpr int N = 8; };
+
+template <int FMT> struct RawTaps { uint32_t v[RawVox<FMT>::N]; };
+
+// Byte offsets are formed in OFF, so with 32-bit offsets a load is the pool's
+// scalar base plus one 32-bit register.
+template <int FMT, class OFF>
+__device__ __forceinline__ void ref_fetch(const void *pool, const RefTap<OFF> &t, int pad, RawTaps<FMT> &r) {
+  using T = typename RawVox<FMT>::T;
+  const char *b = static_cast<const char *>(pool);
+  constexpr OFF sz = (OFF)sizeof(T);
+  const OFF rows[4] = {t.r00 * sz, t.r10 * sz, t.r01 * sz, t.r11 * sz};
+  if constexpr (FMT == HYD_FMT_U8) {
+    const OFF xi = (OFF)t.ix, xj = (OFF)t.jx;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      r.v[2 * k] = *reinterpret_cast<const uint8_t *>(b + (rows[k] + xi));
+      r.v[2 * k + 1] = *reinterpret_cast<const uint8_t *>(b + (rows[k] + xj));
+    }
+  } else {
+    const OFF xm = (OFF)min(t.ix, pad - 2) * sz;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if constexpr (FMT == HYD_FMT_F32) __builtin_memcpy(&r.v[2 * k], b + (rows[k] + xm), 8);
+      else __builtin_memcpy(&r.v[k], b + (rows[k] + xm), 4);
+    }
+  }
+}
+
+// The eight tap values of a fresh sample, x-y-z order.
+template <int FMT, class OFF>
+__device__ __forceinline__ void ref_values(const RawTaps<FMT> &r, const RefTap<OFF> &t, int pad, const float *lut,
+                                           float (&v)[8]) {
+  if constexpr (FMT == HYD_FMT_U8) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = lut[r.v[k]];
+  } else {
+    const int m = min(t.ix, pad - 2);
+    const bool i_lo = t.ix == m, j_hi = t.jx == m + 1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float lo, hi;
+      if constexpr (FMT == HYD_FMT_F32) {
+        lo = __uint_as_float(r.v[2 * k]);
+        hi = __uint_as_float(r.v[2 * k + 1]);
+      } else {
+        lo = __half2float(__ushort_as_half((unsigned short)(r.v[k] & 0xffffu)));
+        hi = __half2float(__ushort_as_half((unsigned short)(r.v[k] >> 16)));
+      }
+      v[2 * k] = i_lo ? lo : hi;
+      v[2 * k + 1] = j_hi ? hi : lo;
+    }
+  }
+}
+
+// x -> y -> z lerps with x * (1 - t) + y * t (cpu_sampler.hpp:126-129).
+template <class OFF>
+__device__ __forceinline__ float ref_interp(const float (&v)[8], const RefTap<OFF> &t) {
+  const float bx = 1.f - t.ax;
+  const float x00 = v[0] * bx + v[1] * t.ax;
+  const float x10 = v[2] * bx + v[3] * t.ax;
+  const float x01 = v[4] * bx + v[5] * t.ax;
+  const float x11 = v[6] * bx + v[7] * t.ax;
+  const float by = 1.f - t.ay;
+  const float y0 = x00 * by + x10 * t.ay;
+  const float y1 = x01 * by + x11 * t.ay;
+  const float bz = 1.f - t.az;
+  return y0 * bz + y1 * t.az;
+}
+
+template <int FMT, int U, bool COUNT, bool THETA, class OFF>
+__device__ void march_reference_grouped(const VolumeArgs &a, const float *tf, const float *lut, MarchState &st,
+                                        unsigned &count) {
+  const float cdu = 1.f / comp_max(abs3(st.d));
+  const f3 d = st.d;
+  f3 o = st.o;
+  int nsteps = st.nsteps;
+  int step_size = st.step_size;
+  const OFF pad3 = (OFF)a.pad * (OFF)a.pad * (OFF)a.pad;
+  // the last composited sample's taps: first and last tap offset, values
+  OFF c0 = ~(OFF)0, c7 = ~(OFF)0;
+  float cur[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int cpg = -1;
+  float4 pgd = make_float4(0.f, 0.f, 0.f, 0.f);
+  BrickLookup bl;
+  while (nsteps > 0) {
+    const f3 ip = floor3(o);
+    bl.update(a, ip);
+    if (bl.cd) {
+      // skip_nblock_steps (volume_shader.cu:5-14) with step * step_size
+      const float sstep = a.step * float(step_size);
+      float tn, tfar;
+      intersect_box(o, d, ip, f3{ip.x + 1.f, ip.y + 1.f, ip.z + 1.f}, tn, tfar);
+      const float di = ceilf((tfar + float(bl.cd - 1) * cdu) / sstep);
+      o = add(o, muls(muls(d, di), sstep));
+      nsteps -= (int)di;
+      o = add(o, muls(muls(d, a.step), float(step_size)));
+      nsteps -= step_size;
+      continue;
+    }
+    const int pgid = bl.pg;
+    if (pgid == -1) break;
+    if (pgid != cpg) {
+      pgd = a.page[pgid];
+      cpg = pgid;
+    }
+    const OFF page_base = (OFF)pgid * pad3;
+    // positions of the group: the same additions the one-sample march makes
+    const f3 inc = muls(muls(d, a.step), float(step_size));
+    f3 p[U];
+    bool ok[U];
+    p[0] = o;
+    ok[0] = true;
+#pragma unroll
+    for (int j = 1; j < U; ++j) {
+      p[j] = add(p[j - 1], inc);
+      const f3 q = floor3(p[j]);
+      ok[j] = ok[j - 1] && nsteps - j * step_size > 0 && q.x == ip.x && q.y == ip.y && q.z == ip.z;
+    }
+    // taps, and the loads of every sample whose cell is not its predecessor's
+    RefTap<OFF> t[U];
+    RawTaps<FMT> raw[U];
+    bool fresh[U];
+    OFF p0 = c0, p7 = c7;
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      t[j] = ref_tap<OFF>(sub(p[j], ip), pgd, a.pad, page_base);
+      const OFF i0 = t[j].r00 + (OFF)t[j].ix, i7 = t[j].r11 + (OFF)t[j].jx;
+      fresh[j] = i0 != p0 || i7 != p7;
+      p0 = i0;
+      p7 = i7;
+      if (ok[j] && fresh[j]) ref_fetch<FMT, OFF>(a.pool, t[j], a.pad, raw[j]);
+    }
+    // composite in order; stop at the sample that escalates
+    int last = 0, last_ss = step_size;
+    f3 plast = p[0];
+    bool live = true;
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (ok[j] && live) {
+        if (fresh[j]) ref_values<FMT, OFF>(raw[j], t[j], a.pad, lut, cur);
+        const float s = ref_interp<OFF>(cur, t[j]);
+        shade_composite_impl<false, THETA>(a, tf, s, pgid, st, 1.f);
+        if (COUNT) ++count;
+        c0 = t[j].r00 + (OFF)t[j].ix;
+        c7 = t[j].r11 + (OFF)t[j].jx;
+        last = j;
+        plast = p[j];
+        if (st.v.w >= 0.93f) {   // v.w > 0.93 (double), see march_reference
+          last_ss = 16;
+        } else if (st.v.w >= 0.85f) {
+          last_ss = 4;
+        }
+        if (last_ss != step_size) live = false;
+      }
+    }
+    // samples 0 .. last-1 advanced by the group's step, the last by its own
+    o = add(plast, muls(muls(d, a.step), float(last_ss)));
+    nsteps -= last * step_size + last_ss;
+    step_size = last_ss;
+  }
+  st.o = o;
+  st.nsteps = nsteps;
+  st.step_size = step_size;
+}
+
 // Lattice schedule: sample k covers w = min(1, L - k) steps (L the ray extent
 // in steps, k < ceil(L)), sits at the covered segment's midpoint
 // o0 + d * ((k + w/2) * step) and its opacity is scaled by w; empty regions are
 // skipped to the next lattice point; early
 // termination once v.w > ert.  Film: o = o0, nsteps slot = bits of L (float),
 // step_size = next lattice index.
-template <int FMT, bool COUNT>
+template <int FMT, bool COUNT, bool THETA = true>
 __device__ void march_lattice(const VolumeArgs &a, const float *tf, const float *lut, MarchState &st,
                               unsigned &count) {
   const f3 o0 = st.o, d = st.d;
@@ -285,7 +535,7 @@ __device__ void march_lattice(const VolumeArgs &a, const float *tf, const float 
       cpg = pgid;
     }
     const float s = brick_sample_reuse<FMT>(a.pool, a.pad, pgid, pgd, sub(o, ip), lut, nullptr, tc);
-    shade_composite(a, tf, s, pgid, st, w);
+    shade_composite_impl<true, THETA>(a, tf, s, pgid, st, w);
     if (COUNT) ++count;
     ++k;
     if (a.ert > 0.f && st.v.w > a.ert) { k = kend; break; }
@@ -344,8 +594,32 @@ __device__ __forceinline__ void load_pixel(const VolumeArgs &a, size_t idx, Marc
   st.phi = f3v.w;
 }
 
-template <int FMT, bool COUNT, bool RESUME>
+// Samples whose tap loads a reference-schedule group issues together, per
+// brick format.  Measured on the 512^2 frames: FP32 (256^3 single brick) takes
+// 0.54 ms with 2 (105 registers, 4 waves per SIMD hold the whole frame) and
+// 0.64 ms with 4 (3 waves per SIMD); u8 (1024^3 paged, whose few long rays set
+// the frame) 0.58 ms with 8 and 0.68 ms with 2.
+#ifndef HYD_REF_GROUP_U8
+#define HYD_REF_GROUP_U8 8
+#endif
+#ifndef HYD_REF_GROUP_F16
+#define HYD_REF_GROUP_F16 2
+#endif
+#ifndef HYD_REF_GROUP_F32
+#define HYD_REF_GROUP_F32 2
+#endif
+template <int FMT> constexpr int ref_group() {
+  return FMT == HYD_FMT_U8 ? HYD_REF_GROUP_U8 : FMT == HYD_FMT_F16 ? HYD_REF_GROUP_F16 : HYD_REF_GROUP_F32;
+}
+
+// Render modes of volume_kernel: emit into an RGBA image only, emit into a film
+// (and optionally RGBA), or resume a film.
+enum { kEmitRgba = 0, kEmitFilm = 1, kResume = 2 };
+
+template <int FMT, bool COUNT, int MODE, bool BIG>
 __global__ __launch_bounds__(64) void volume_kernel(VolumeArgs a) {
+  constexpr bool RESUME = MODE == kResume;
+  constexpr bool THETA = MODE != kEmitRgba;
   __shared__ float s_tf[4 * kMaxTf];
   __shared__ float s_lut[FMT == HYD_FMT_U8 ? 256 : 1];
   __shared__ unsigned long long s_count;
@@ -369,11 +643,11 @@ __global__ __launch_bounds__(64) void volume_kernel(VolumeArgs a) {
       live = emit(a, x, y, st);
     }
     if (live) {
-      int nt = 0;
       if (a.schedule == HYD_SCHED_LATTICE)
-        march_lattice<FMT, COUNT>(a, s_tf, s_lut, st, count);
+        march_lattice<FMT, COUNT, THETA>(a, s_tf, s_lut, st, count);
       else
-        march_reference<FMT, COUNT, false>(a, s_tf, s_lut, st, count, nullptr, 0, nt);
+        march_reference_grouped<FMT, ref_group<FMT>(), COUNT, THETA, std::conditional_t<BIG, uint64_t, uint32_t>>(
+            a, s_tf, s_lut, st, count);
     }
     store_pixel(a, idx, st);
   }
@@ -510,18 +784,38 @@ hipError_t launch_segments_fmt(const VolumeArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int FMT>
-hipError_t launch_fmt(const VolumeArgs &a, bool resume, hipStream_t s) {
+template <int FMT, int MODE, bool BIG>
+void launch_mode(const VolumeArgs &a, hipStream_t s) {
   dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile);
   dim3 block(kTile * kTile);
-  const bool count = a.samples != nullptr;
-  if (resume) {
-    if (count) hipLaunchKernelGGL((volume_kernel<FMT, true, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((volume_kernel<FMT, false, true>), grid, block, 0, s, a);
-  } else {
-    if (count) hipLaunchKernelGGL((volume_kernel<FMT, true, false>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((volume_kernel<FMT, false, false>), grid, block, 0, s, a);
-  }
+  if (a.samples) hipLaunchKernelGGL((volume_kernel<FMT, true, MODE, BIG>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((volume_kernel<FMT, false, MODE, BIG>), grid, block, 0, s, a);
+}
+
+template <int FMT, bool BIG>
+void launch_fmt_off(const VolumeArgs &a, bool resume, hipStream_t s) {
+  if (resume) launch_mode<FMT, kResume, BIG>(a, s);
+  else if (a.film) launch_mode<FMT, kEmitFilm, BIG>(a, s);
+  else launch_mode<FMT, kEmitRgba, BIG>(a, s);
+}
+
+// HYD_VOLUME_OFFSETS64=1 selects the 64-bit offsets for any pool (they render the
+// same image; the switch lets a small pool exercise that variant).
+bool force_offsets64() {
+  static const bool on = [] {
+    const char *e = std::getenv("HYD_VOLUME_OFFSETS64");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
+template <int FMT>
+hipError_t launch_fmt(const VolumeArgs &a, bool resume, hipStream_t s) {
+  // 32-bit byte offsets while every byte of the pool is below 4 GiB
+  const uint64_t bytes = a.pool_elems * (uint64_t)sizeof(typename RawVox<FMT>::T);
+  // the x-pair loads need two voxels per row (hyd_upload_pool refuses smaller bricks)
+  if (bytes <= (1ull << 32) && !force_offsets64()) launch_fmt_off<FMT, false>(a, resume, s);
+  else launch_fmt_off<FMT, true>(a, resume, s);
   return hipGetLastError();
 }
 

@@ -184,8 +184,8 @@ int32_t hyd_exhibit_iet(const float ex_center[3], const float ex_size[3], float 
 int32_t hyd_mat4_inverse(const float m[16], float out[16]);
 
 /* --------------------------------------------------- resident state ---- */
-/* Brick pool: n_pages padded bricks of pad_edge^3 voxels, x fastest, brick
- * after brick (LosslessBlockPagingServer block_storage,
+/* Brick pool: n_pages padded bricks of pad_edge^3 voxels (2 <= pad_edge <= 1024),
+ * x fastest, brick after brick (LosslessBlockPagingServer block_storage,
  * src/slave/paging/lossless_block_paging.cc:15-83).  Copied to the device. */
 int32_t hyd_upload_pool(hyd_ctx *ctx, int32_t fmt, int32_t pad_edge, int32_t n_pages, const void *host);
 /* Same, but the pool already lives on the device; the context keeps the pointer
