@@ -570,6 +570,7 @@ int32_t hyd_render_iso(hyd_ctx *c, const hyd_iso_params *p, const hyd_view *view
   std::memcpy(a.vdim, c->paging.vdim, sizeof(a.vdim));
   a.page = c->paging.page;
   a.pool = c->pool.data;
+  a.pool_elems = (uint64_t)c->pool.n_pages * c->pool.pad * c->pool.pad * c->pool.pad;
   a.pad = c->pool.pad;
   a.u8_lut = c->u8_lut;
   a.out = out_dev;

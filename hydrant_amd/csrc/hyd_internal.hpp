@@ -198,6 +198,7 @@ struct IsoArgs {
   int32_t vdim[3];
   const float4 *page;
   const void *pool;
+  uint64_t pool_elems;    // voxels in the pool (n_pages * pad^3)
   int32_t pad;
   const float *u8_lut;
   hyd_iso_pixel *out;

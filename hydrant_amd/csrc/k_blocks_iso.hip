@@ -3,6 +3,10 @@
 #include <hip/hip_fp16.h>
 
 #include "hyd_internal.hpp"
+#include "hyd_sampler.hpp"
+
+#include <cstdlib>
+#include <type_traits>
 
 namespace hyd {
 namespace {
@@ -128,63 +132,6 @@ __device__ float iso_sample(const IsoArgs &a, const float *lut, int pgid, f3 x) 
   return y0 * bz + y1 * w[2];
 }
 
-// The last march sample's page descriptor and taps (first and last tap index,
-// which fix all eight, and their values).
-struct IsoTaps {
-  int pgid = -1;
-  float4 pg;
-  size_t id0 = ~(size_t)0, id7 = ~(size_t)0;
-  float v[8];
-};
-
-// iso_sample for consecutive march samples of one ray: the page descriptor and
-// the eight taps are loaded only when they change (at a quarter-voxel step
-// most samples share their predecessor's cell).  Same operations, so the value
-// is bit-identical to iso_sample's.
-template <int FMT>
-__device__ float iso_sample_reuse(const IsoArgs &a, const float *lut, int pgid, f3 x, IsoTaps &c) {
-  if (pgid != c.pgid) {
-    c.pg = a.page[pgid];
-    c.pgid = pgid;
-  }
-  const float4 pg = c.pg;
-  const int pad = a.pad;
-  const float fd = float(pad);
-  float u[3] = {x.x * pg.x + pg.y, x.y * pg.x + pg.z, x.z * pg.x + pg.w};
-  float w[3];
-  int i[3], j[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float fx = fd * u[k];
-    fx = wrap_mod(fx, fd);   // Wrap: glm mod(fx, fd)
-    fx = fx - 0.5f;
-    float flr = floorf(fx);
-    w[k] = fx - flr;
-    i[k] = iclamp((int)flr, 0, pad - 1);
-    j[k] = iclamp((int)ceilf(fx), 0, pad - 1);
-  }
-  const size_t base = (size_t)pgid * pad * pad * pad;
-  const size_t id0 = base + ((size_t)i[2] * pad + i[1]) * pad + i[0];
-  const size_t id7 = base + ((size_t)j[2] * pad + j[1]) * pad + j[0];
-  if (id0 != c.id0 || id7 != c.id7) {
-    auto at = [&](int xx, int yy, int zz) { return vox<FMT>(a.pool, base + ((size_t)zz * pad + yy) * pad + xx, lut); };
-    c.v[0] = at(i[0], i[1], i[2]); c.v[1] = at(j[0], i[1], i[2]);
-    c.v[2] = at(i[0], j[1], i[2]); c.v[3] = at(j[0], j[1], i[2]);
-    c.v[4] = at(i[0], i[1], j[2]); c.v[5] = at(j[0], i[1], j[2]);
-    c.v[6] = at(i[0], j[1], j[2]); c.v[7] = at(j[0], j[1], j[2]);
-    c.id0 = id0;
-    c.id7 = id7;
-  }
-  const float bx = 1.f - w[0], by = 1.f - w[1], bz = 1.f - w[2];
-  float x00 = c.v[0] * bx + c.v[1] * w[0];
-  float x10 = c.v[2] * bx + c.v[3] * w[0];
-  float x01 = c.v[4] * bx + c.v[5] * w[0];
-  float x11 = c.v[6] * bx + c.v[7] * w[0];
-  float y0 = x00 * by + x10 * w[1];
-  float y1 = x01 * by + x11 * w[1];
-  return y0 * bz + y1 * w[2];
-}
-
 __device__ __forceinline__ float gsign(float x) { return float((0.f < x) - (x < 0.f)); }
 
 // IsosurfaceShaderKernel::linear_to_srgb: double arithmetic on a float input
@@ -205,8 +152,108 @@ __device__ __forceinline__ float iso_srgb(float linear) {
 // latency of the tap loads.
 constexpr int32_t kIsoPending = -1;
 
-template <int FMT>
-__global__ __launch_bounds__(64) void iso_march_kernel(IsoArgs a) {
+// The march of IsosurfaceShaderKernel::main up to the first crossing, with the
+// tap loads of up to U consecutive samples in flight together (the step is
+// fixed, so the next positions inside a brick follow from the same additions
+// the one-step march makes).  The group is evaluated, then scanned in order for
+// the first crossing; values, positions, counts and the pending pixel are those
+// of the one-step march.
+template <int FMT, int U, class OFF>
+__device__ bool iso_march_grouped(const IsoArgs &a, const float *lut, f3 &o, f3 d, int nsteps, float cdu,
+                                  hyd_iso_pixel &px, unsigned &count) {
+  const float fpad = float(a.pad);
+  const OFF pad3 = (OFF)a.pad * (OFF)a.pad * (OFF)a.pad;
+  float prev_value = 0.f;
+  bool have_prev = false;
+  OFF c0 = ~(OFF)0, c7 = ~(OFF)0;   // taps of the last evaluated sample
+  float cur[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  f3 cip = {-INFINITY, -INFINITY, -INFINITY};
+  int ccd = 0, cpg = -1, dpg = -1;
+  float4 pgd = make_float4(0.f, 0.f, 0.f, 0.f);
+  (void)fpad;
+  while (nsteps > 0) {
+    const f3 ip = floor3(o);
+    if (ip.x != cip.x || ip.y != cip.y || ip.z != cip.z) {
+      cip = ip;
+      ccd = a.cheb[grid_index(ip, a.cdim)];
+      if (ccd == 0) cpg = a.vaddr[grid_index(ip, a.vdim)];
+    }
+    if (ccd) {
+      nsteps -= skip_nblock_steps(o, d, ip, ccd, cdu, a.step);
+      have_prev = false;
+      o = add(o, muls(d, a.step));
+      nsteps -= 1;
+      continue;
+    }
+    const int pgid = cpg;
+    if (pgid == -1) break;
+    if (pgid != dpg) {
+      pgd = a.page[pgid];
+      dpg = pgid;
+    }
+    const OFF page_base = (OFF)pgid * pad3;
+    const f3 inc = muls(d, a.step);
+    f3 p[U];
+    bool ok[U];
+    p[0] = o;
+    ok[0] = true;
+#pragma unroll
+    for (int j = 1; j < U; ++j) {
+      p[j] = add(p[j - 1], inc);
+      const f3 q = floor3(p[j]);
+      ok[j] = ok[j - 1] && nsteps - j > 0 && q.x == ip.x && q.y == ip.y && q.z == ip.z;
+    }
+    RefTap<OFF> t[U];
+    RawTaps<FMT> raw[U];
+    bool fresh[U];
+    OFF p0 = c0, p7 = c7;
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      t[j] = ref_tap<OFF>(sub(p[j], ip), pgd, a.pad, page_base);
+      const OFF i0 = t[j].r00 + (OFF)t[j].ix, i7 = t[j].r11 + (OFF)t[j].jx;
+      fresh[j] = i0 != p0 || i7 != p7;
+      p0 = i0;
+      p7 = i7;
+      if (ok[j] && fresh[j]) ref_fetch<FMT, OFF>(a.pool, t[j], a.pad, raw[j]);
+    }
+    int last = 0;
+    f3 plast = p[0];
+    bool live = true;
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (ok[j] && live) {
+        if (fresh[j]) ref_values<FMT, OFF>(raw[j], t[j], a.pad, lut, cur);
+        const float value = ref_interp<OFF>(cur, t[j]);
+        c0 = t[j].r00 + (OFF)t[j].ix;
+        c7 = t[j].r11 + (OFF)t[j].jx;
+        ++count;
+        bool crossing = gsign(value - a.isovalue) != gsign(prev_value - a.isovalue);
+        if (a.corrected) crossing = crossing && have_prev;
+        if (crossing) {
+          px.v[0] = p[j].x; px.v[1] = p[j].y; px.v[2] = p[j].z; px.v[3] = prev_value;
+          px.depth = value;
+          px.hit = kIsoPending;
+          live = false;
+        } else {
+          if (a.corrected) { prev_value = value; have_prev = true; }
+          last = j;
+          plast = p[j];
+        }
+      }
+    }
+    if (!live) return true;
+    o = add(plast, inc);
+    nsteps -= last + 1;
+  }
+  return false;
+}
+
+#ifndef HYD_ISO_GROUP
+#define HYD_ISO_GROUP 4
+#endif
+
+template <int FMT, bool BIG>
+__global__ __launch_bounds__(64) void iso_march_grouped_kernel(IsoArgs a) {
   __shared__ float s_lut[FMT == HYD_FMT_U8 ? 256 : 1];
   if (FMT == HYD_FMT_U8)
     for (int i = threadIdx.x; i < 256; i += blockDim.x) s_lut[i] = a.u8_lut[i];
@@ -227,44 +274,10 @@ __global__ __launch_bounds__(64) void iso_march_kernel(IsoArgs a) {
     if (tfar < 0) tfar = 0;
     o = add(o, muls(d, tnear));
     const int n = (int)((tfar - tnear) / a.step);
-    int nsteps = a.max_steps < n ? a.max_steps : n;
+    const int nsteps = a.max_steps < n ? a.max_steps : n;
     const float cdu = 1.f / comp_max(abs3(d));
-    float prev_value = 0.f;
-    bool have_prev = false;
-    IsoTaps taps;
-    // the brick of the last step and its chebyshev / page-table entries: the
-    // steps inside one brick read them once
-    f3 cip = {-INFINITY, -INFINITY, -INFINITY};
-    int ccd = 0, cpg = -1;
-    while (nsteps > 0) {
-      const f3 ip = floor3(o);
-      if (ip.x != cip.x || ip.y != cip.y || ip.z != cip.z) {
-        cip = ip;
-        ccd = a.cheb[grid_index(ip, a.cdim)];
-        if (ccd == 0) cpg = a.vaddr[grid_index(ip, a.vdim)];
-      }
-      const int cd = ccd;
-      if (cd) {
-        nsteps -= skip_nblock_steps(o, d, ip, cd, cdu, a.step);
-        have_prev = false;
-      } else {
-        const int pgid = cpg;
-        if (pgid == -1) break;
-        const float value = iso_sample_reuse<FMT>(a, s_lut, pgid, sub(o, ip), taps);
-        ++count;
-        bool crossing = gsign(value - a.isovalue) != gsign(prev_value - a.isovalue);
-        if (a.corrected) crossing = crossing && have_prev;
-        if (crossing) {
-          px.v[0] = o.x; px.v[1] = o.y; px.v[2] = o.z; px.v[3] = prev_value;
-          px.depth = value;
-          px.hit = kIsoPending;
-          break;
-        }
-        if (a.corrected) { prev_value = value; have_prev = true; }
-      }
-      o = add(o, muls(d, a.step));
-      nsteps -= 1;
-    }
+    iso_march_grouped<FMT, HYD_ISO_GROUP, std::conditional_t<BIG, uint64_t, uint32_t>>(a, s_lut, o, d, nsteps, cdu,
+                                                                                        px, count);
   }
   a.out[(size_t)y * a.W + x] = px;
   if (a.samples && count) atomicAdd(a.samples, (unsigned long long)count);
@@ -370,20 +383,33 @@ hipError_t launch_blocks(const BlocksArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_iso(const IsoArgs &a, int fmt, hipStream_t s) {
+namespace {
+// HYD_VOLUME_OFFSETS64=1 selects the 64-bit offsets for any pool (same image).
+bool iso_force_offsets64() {
+  static const bool on = [] {
+    const char *e = std::getenv("HYD_VOLUME_OFFSETS64");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
+template <int FMT>
+void launch_iso_fmt(const IsoArgs &a, hipStream_t s) {
   dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile);
+  const uint64_t bytes = a.pool_elems * (uint64_t)sizeof(typename RawVox<FMT>::T);
+  if (bytes <= (1ull << 32) && !iso_force_offsets64())
+    hipLaunchKernelGGL((iso_march_grouped_kernel<FMT, false>), grid, dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((iso_march_grouped_kernel<FMT, true>), grid, dim3(64), 0, s, a);
+  hipLaunchKernelGGL(iso_hit_kernel<FMT>, grid, dim3(64), 0, s, a);
+}
+}  // namespace
+
+hipError_t launch_iso(const IsoArgs &a, int fmt, hipStream_t s) {
   switch (fmt) {
-    case HYD_FMT_U8:
-      hipLaunchKernelGGL(iso_march_kernel<HYD_FMT_U8>, grid, dim3(64), 0, s, a);
-      hipLaunchKernelGGL(iso_hit_kernel<HYD_FMT_U8>, grid, dim3(64), 0, s, a);
-      break;
-    case HYD_FMT_F16:
-      hipLaunchKernelGGL(iso_march_kernel<HYD_FMT_F16>, grid, dim3(64), 0, s, a);
-      hipLaunchKernelGGL(iso_hit_kernel<HYD_FMT_F16>, grid, dim3(64), 0, s, a);
-      break;
-    default:
-      hipLaunchKernelGGL(iso_march_kernel<HYD_FMT_F32>, grid, dim3(64), 0, s, a);
-      hipLaunchKernelGGL(iso_hit_kernel<HYD_FMT_F32>, grid, dim3(64), 0, s, a);
+    case HYD_FMT_U8: launch_iso_fmt<HYD_FMT_U8>(a, s); break;
+    case HYD_FMT_F16: launch_iso_fmt<HYD_FMT_F16>(a, s); break;
+    default: launch_iso_fmt<HYD_FMT_F32>(a, s);
   }
   return hipGetLastError();
 }
