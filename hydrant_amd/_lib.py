@@ -129,6 +129,7 @@ SIGNATURES = {
     "hyd_pixel_pass": (I32, [P, P, I32, I32, C.POINTER(C.c_uint8), P]),
     "hyd_fetch_pass": (I32, [P, P, I32, I32, P]),
     "hyd_composite": (I32, [P, I32, P, PI32, I32, I32, P, P]),
+    "hyd_composite_noalloc": (I32, [P, I32, P, PI32, I32, I32, P, P]),
     "hyd_srgb8_thresholds": (I32, [PF]),
     "hyd_iso_fetch_pass": (I32, [P, P, I32, I32, P]),
     "hyd_composite_iso": (I32, [P, I32, P, PI32, I32, I32, P, P]),

@@ -2,6 +2,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "hyd_internal.hpp"
@@ -88,6 +89,35 @@ void fill_view(const hyd_view *v, float trans[16], float &ctg, f3 &ray_o) {
 }  // namespace
 
 namespace {
+// Device staging of the z order for hyd_composite_noalloc: a file-scope device
+// array (each device holds its own copy for the life of the process) whose
+// address is resolved once per device at context creation.  In host code the
+// array's name is not a device address; only the resolved pointer is used.
+constexpr int32_t kNoAllocMaxPartials = 4096;
+static __device__ __align__(256) int32_t scratch[kNoAllocMaxPartials];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t *g_scratch_ptr[kMaxDevices] = {};
+
+// Resolves this device's scratch address (the device must be current).
+int32_t resolve_scratch(int device) {
+  HYD_REQUIRE(device >= 0 && device < kMaxDevices, HYD_ERR_RANGE, "device %d beyond the %d the library tracks",
+              device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return HYD_OK;
+  void *ptr = nullptr;
+  HYD_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  HYD_REQUIRE(ptr, HYD_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+  g_scratch_ptr[device] = static_cast<int32_t *>(ptr);
+  return HYD_OK;
+}
+
+int32_t *scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+
 // linear_to_srgb (volume.cc:175-181) then saturate (glm_math.hpp:75-81), binary32
 // on the host as the reference evaluates it.
 uint8_t srgb8_host(float x) {
@@ -157,6 +187,12 @@ int32_t hyd_create(int32_t device, hyd_ctx **out) {
   if (st != HYD_OK) { (void)hipStreamDestroy(c->own_stream); delete c; return st; }
   c->u8_lut = c->own_lut;
   c->srgb8_t = c->own_lut + 256;
+  if ((st = resolve_scratch(device)) != HYD_OK) {
+    (void)hipFree(c->own_lut);
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return st;
+  }
   *out = c;
   return HYD_OK;
 }
@@ -378,6 +414,31 @@ int32_t hyd_composite(hyd_ctx *c, int32_t n, const hyd_fetch_pixel *partials_dev
                                   rgb_dev, c->srgb8_t, c->stream);
   (void)hipFreeAsync(zdev, c->stream);
   HYD_CHECK_HIP(e);
+  HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return HYD_OK;
+}
+
+int32_t hyd_composite_noalloc(hyd_ctx *c, int32_t n, const hyd_fetch_pixel *partials_dev, const int32_t *z_order,
+                              int32_t width, int32_t height, float *rgba_dev, uint8_t *rgb_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(n >= 1 && partials_dev && z_order && width > 0 && height > 0, HYD_ERR_ARG,
+              "invalid composite arguments");
+  HYD_REQUIRE(n <= kNoAllocMaxPartials, HYD_ERR_RANGE,
+              "hyd_composite_noalloc composites at most %d partial images (got %d)", kNoAllocMaxPartials, n);
+  // a permutation check on the host, in a fixed bitset (no allocation)
+  uint64_t seen[kNoAllocMaxPartials / 64] = {};
+  for (int i = 0; i < n; ++i) {
+    const int32_t z = z_order[i];
+    HYD_REQUIRE(z >= 0 && z < n && !(seen[z >> 6] >> (z & 63) & 1), HYD_ERR_ARG,
+                "z_order must be a permutation of 0..%d", n - 1);
+    seen[z >> 6] |= 1ull << (z & 63);
+  }
+  int32_t *zdev = scratch_ptr(c->device);
+  HYD_REQUIRE(zdev, HYD_ERR_STATE, "no scratch address resolved for device %d", c->device);
+  HYD_CHECK_HIP(hipMemcpyAsync(zdev, z_order, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  HYD_CHECK_HIP(launch_composite(n, partials_dev, zdev, width, height, reinterpret_cast<float4 *>(rgba_dev), rgb_dev,
+                                 c->srgb8_t, c->stream));
   HYD_CHECK_HIP(hipStreamSynchronize(c->stream));
   return HYD_OK;
 }

@@ -241,6 +241,15 @@ int32_t hyd_fetch_pass(hyd_ctx *ctx, const hyd_volume_pixel *film_dev, int32_t w
  * sRGB + saturate (volume.cc:267-292).  rgba_dev / rgb_dev may be NULL. */
 int32_t hyd_composite(hyd_ctx *ctx, int32_t n, const hyd_fetch_pixel *partials_dev, const int32_t *z_order,
                       int32_t width, int32_t height, float *rgba_dev, uint8_t *rgb_dev);
+/* hyd_composite without any device allocation (no counterpart in the reference):
+ * for callers that cannot allocate, such as code under HIP graph capture or a
+ * serving loop that composites every frame.  The z order is staged in a fixed
+ * device array of the library (one per device, resolved by hyd_create); n is at
+ * most 4096 (HYD_ERR_RANGE beyond).  Every call on one device uses that same
+ * array, so calls that overlap on a device (other streams, other contexts,
+ * other threads) race.  Same results as hyd_composite.  Synchronous. */
+int32_t hyd_composite_noalloc(hyd_ctx *ctx, int32_t n, const hyd_fetch_pixel *partials_dev, const int32_t *z_order,
+                              int32_t width, int32_t height, float *rgba_dev, uint8_t *rgb_dev);
 /* The bytes hyd_composite writes: t[0] = -inf and, for k = 1..255, t[k] = the
  * least float whose saturate(linear_to_srgb(x)) byte (volume.cc:175-181,284-288,
  * binary32 on the host) is k or more; the composite writes the number of t[k]
