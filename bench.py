@@ -156,6 +156,66 @@ def bench_paged(torch, args):
             "volume": 1024, "res": sc.W, "gen_s": round(gen_s, 1), "scene": sc, "rgba": rgba}
 
 
+def bench_paged_sort_last(torch, args):
+    """Paged Volume as configured: 1024^3 u8 Perlin in 64^3 + 1 bricks, rendered over
+    residency batches of 40% of the bricks (1638 pages each) until every non-empty brick
+    has been resident, each batch writing its bricks' per-ray partials, then the in-GPU
+    sort-last composite (lattice schedule, no early termination).  Every batch's pool is
+    in HBM before the timed region; a step binds each batch's pool, sets its page table,
+    runs its segment pass, then composites."""
+    from hydrant_amd import _lib as L
+    from hydrant_amd import dataset as D
+    from hydrant_amd import workloads as W
+    from hydrant_amd.hip import Context
+
+    t0 = time.perf_counter()
+    sc = W.paged_volume(resident=1.0, schedule=L.SCHED_LATTICE)
+    sc.ert = 0.0
+    b = sc.bricked
+    per = int(round(0.4 * b.vaddr.size))
+    host_batches = D.residency_batches(sc.extra["volume"], b, per)
+    gen_s = time.perf_counter() - t0
+    ctx = Context(0)
+    ctx.set_tf(sc.tf)
+    ctx.set_empty_grid(b.cheb)
+    dev_batches = [(fmt, pad, torch.from_numpy(pool).cuda(), vaddr, pk, pb)
+                   for fmt, pad, pool, vaddr, pk, pb in host_batches]
+    p, view = sc.volume_params(), sc.view()
+    max_slots = int(sum(b.grid))
+    slots = torch.empty((max_slots, sc.H, sc.W, 4), dtype=torch.float32, device="cuda")
+    nslots = torch.zeros((sc.H, sc.W), dtype=torch.int32, device="cuda")
+    rgba = torch.empty((sc.H, sc.W, 4), dtype=torch.float32, device="cuda")
+    _, cnt = ctx.render_sort_last_resident(p, view, sc.W, sc.H, dev_batches, b.grid, slots, nslots, rgba, count=True)
+    torch.cuda.synchronize()
+    n_samples = int(cnt.item())
+
+    def step():
+        ctx.render_sort_last_resident(p, view, sc.W, sc.H, dev_batches, b.grid, slots, nslots, rgba)
+
+    wall, ev = time_kernel(torch, step, args.steps, args.warmup)
+    ms = wall * 1e3 / args.steps
+    res = {"ms_per_frame": round(ms, 4), "samples_per_frame": n_samples,
+           "msamples_per_s": round(n_samples / (ms * 1e-3) / 1e6, 3),
+           "batches": len(dev_batches), "pages_per_batch": per, "bricks": int(b.vaddr.size),
+           "nonempty_bricks": int(b.n_nonempty), "max_slots": max_slots, "volume": 1024, "res": sc.W,
+           "gen_s": round(gen_s, 1), "scene": sc, "rgba": rgba}
+    ctx.close()
+    return res
+
+
+def sort_last_check(res):
+    """The sort-last frame against the FP32 oracle's single lattice march over every
+    brick, on every 7th pixel."""
+    from tests.helpers import oracle_view, oracle_volume
+
+    sc = res["scene"]
+    idx = np.arange(0, sc.W * sc.H, 7)
+    xy = np.stack([idx % sc.W, idx // sc.W], 1).astype(np.int32)
+    ref, _ = oracle_volume(sc).render_pixels(oracle_view(sc), xy, prec=32, nthreads=cpu_threads())
+    got = res["rgba"].cpu().numpy()[xy[:, 1], xy[:, 0]].astype(np.float64)
+    return {"sort_last_max_abs_vs_oracle": float(np.max(np.abs(got - ref))), "sort_last_pixels_checked": int(len(idx))}
+
+
 def bench_iso(torch, args):
     """Isosurface: 512^3 FP16 SDF of 64 seeded spheres (smooth union), 64^3 + 1 bricks,
     1024^2, iso 0, bracket on the sign change then 8 bisections, Phong colour."""
@@ -394,7 +454,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg and the FP64 error check")
     ap.add_argument("--err-views", type=int, default=6, help="batched views checked against the FP64 oracle")
     ap.add_argument("--no-paged-iso", action="store_true", help="skip the paged and isosurface legs")
-    ap.add_argument("--only", choices=["plain", "batched", "diff", "paged", "iso"],
+    ap.add_argument("--only", choices=["plain", "batched", "diff", "paged", "sortlast", "iso"],
                     help="profiling aid: run only this workload's timed launches and print its line")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-form leg")
     ap.add_argument("--diff-fit-steps", type=int, default=200, help="Adam steps of the differentiable fit")
@@ -426,8 +486,8 @@ def main():
         print(json.dumps(bench_diff(torch, args)), flush=True)
         return
     if args.only:
-        res = {"plain": bench_plain, "batched": bench_batched, "paged": bench_paged, "iso": bench_iso}[args.only](
-            torch, args)
+        res = {"plain": bench_plain, "batched": bench_batched, "paged": bench_paged, "iso": bench_iso,
+               "sortlast": bench_paged_sort_last}[args.only](torch, args)
         print(json.dumps({k: v for k, v in res.items() if isinstance(v, (int, float, str, dict))}), flush=True)
         return
 
@@ -443,11 +503,14 @@ def main():
         print(f"# diff: fwd {diff['fwd_ms']:.3f} ms, fwd+adjoint {diff['fwd_adjoint_ms']:.3f} ms "
               f"(x{diff['ratio_fwd_adjoint_vs_fwd']:.2f})", file=sys.stderr, flush=True)
 
-    paged = iso = None
+    paged = iso = sortlast = None
     if not args.no_paged_iso:
         paged = bench_paged(torch, args)
         print(f"# paged u8: {paged['ms_per_frame']:.3f} ms/frame, {paged['msamples_per_s']:.1f} Msamples/s",
               file=sys.stderr, flush=True)
+        sortlast = bench_paged_sort_last(torch, args)
+        print(f"# paged u8 sort-last ({sortlast['batches']} residency batches): {sortlast['ms_per_frame']:.3f} "
+              f"ms/frame, {sortlast['msamples_per_s']:.1f} Msamples/s", file=sys.stderr, flush=True)
         iso = bench_iso(torch, args)
         print(f"# isosurface fp16: {iso['ms_per_frame']:.3f} ms/frame, {iso['msamples_per_s']:.1f} Msamples/s",
               file=sys.stderr, flush=True)
@@ -491,6 +554,8 @@ def main():
         "batched_fp16": batched_info,
         "diff": diff,
         "paged_u8": None if paged is None else {k: v for k, v in paged.items() if k not in ("scene", "rgba")},
+        "paged_sort_last_u8": None if sortlast is None else {k: v for k, v in sortlast.items()
+                                                             if k not in ("scene", "rgba")},
         "isosurface_f16": None if iso is None else {k: v for k, v in iso.items()
                                                     if k not in ("scene", "params", "px")},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 2), "peak": HBM_PEAK_GBS,
@@ -501,6 +566,8 @@ def main():
                      "alg_bytes_rule": f"samples x 8 taps x {bytes_per_voxel} B + pixels x 16 B"},
     }
     if not args.no_cpu:
+        if sortlast is not None:
+            out["paged_sort_last_u8"].update(sort_last_check(sortlast))
         if paged is not None or iso is not None:
             chk = paged_iso_checks(paged, iso)
             if paged is not None:

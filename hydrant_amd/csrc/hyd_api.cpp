@@ -39,6 +39,40 @@ template <class T> int32_t upload(T *&dst, const void *src, size_t bytes, hipStr
   return HYD_OK;
 }
 
+// upload() into the existing allocation when it holds `bytes` (the copy is
+// ordered after the kernels already queued on the stream that read it).
+template <class T> int32_t upload_reuse(T *&dst, size_t &cap, const void *src, size_t bytes, hipStream_t s) {
+  if (!dst || cap < bytes) {
+    cap = 0;
+    int32_t st = upload(dst, src, bytes, s);
+    if (st == HYD_OK) cap = bytes;
+    return st;
+  }
+  HYD_CHECK_HIP(hipMemcpyAsync((void *)dst, src, bytes, hipMemcpyHostToDevice, s));
+  HYD_CHECK_HIP(hipStreamSynchronize(s));
+  return HYD_OK;
+}
+
+// One code per brick from the host copies of the page table and chebyshev grid
+// (when both are set with equal dims): the marches then read one entry per
+// brick instead of two dependent ones.
+int32_t rebuild_codes(Ctx *c) {
+  c->codes_valid = false;
+  if (c->host_vaddr.empty() || c->host_cheb.empty() ||
+      std::memcmp(c->paging.vdim, c->cheb.dim, sizeof(c->cheb.dim)) != 0)
+    return HYD_OK;
+  const size_t n = c->host_vaddr.size();
+  std::vector<int32_t> codes(n);
+  for (size_t i = 0; i < n; ++i) {
+    const int32_t cd = c->host_cheb[i];
+    codes[i] = cd > 0 ? -1 - cd : c->host_vaddr[i];
+  }
+  int32_t st = upload_reuse(c->own_codes, c->codes_cap, codes.data(), n * sizeof(int32_t), c->stream);
+  if (st) return st;
+  c->codes_valid = true;
+  return HYD_OK;
+}
+
 int32_t check_volume_state(const Ctx *c) {
   HYD_REQUIRE(c->pool.data, HYD_ERR_STATE, "no brick pool: call hyd_upload_pool first");
   HYD_REQUIRE(c->paging.vaddr, HYD_ERR_STATE, "no page table: call hyd_set_page_table first");
@@ -71,6 +105,7 @@ int32_t fill_volume_args(const Ctx *c, const hyd_volume_params *p, VolumeArgs &a
   std::memcpy(a.cdim, c->cheb.dim, sizeof(a.cdim));
   a.vaddr = c->paging.vaddr;
   std::memcpy(a.vdim, c->paging.vdim, sizeof(a.vdim));
+  a.codes = c->codes_valid ? c->own_codes : nullptr;
   a.lowest_blkcnt = c->paging.lowest_blkcnt;
   a.page = c->paging.page;
   a.pool = c->pool.data;
@@ -205,6 +240,7 @@ int32_t hyd_destroy(hyd_ctx *c) {
   free_dev(c->own_oriented);
   free_dev(c->own_vaddr);
   free_dev(c->own_page);
+  free_dev(c->own_codes);
   free_dev(c->own_cheb);
   free_dev(c->own_mean);
   free_dev(c->own_tf);
@@ -287,16 +323,17 @@ int32_t hyd_set_page_table(hyd_ctx *c, const int32_t dims[3], const int32_t *vad
   std::vector<float4> pages(n_pages);
   for (int i = 0; i < n_pages; ++i)
     pages[i] = make_float4(page_k[i], page_b[3 * i], page_b[3 * i + 1], page_b[3 * i + 2]);
-  st = upload(c->own_vaddr, vaddr, cells * sizeof(int32_t), c->stream);
+  st = upload_reuse(c->own_vaddr, c->vaddr_cap, vaddr, cells * sizeof(int32_t), c->stream);
   if (st) return st;
-  st = upload(c->own_page, pages.data(), pages.size() * sizeof(float4), c->stream);
+  st = upload_reuse(c->own_page, c->page_cap, pages.data(), pages.size() * sizeof(float4), c->stream);
   if (st) return st;
   c->paging.vaddr = c->own_vaddr;
   std::memcpy(c->paging.vdim, dims, 3 * sizeof(int32_t));
   c->paging.page = c->own_page;
   c->paging.n_pages = n_pages;
   c->paging.lowest_blkcnt = lowest_blkcnt;
-  return HYD_OK;
+  c->host_vaddr.assign(vaddr, vaddr + cells);
+  return rebuild_codes(c);
 }
 
 int32_t hyd_set_empty_grid(hyd_ctx *c, const int32_t dims[3], const int32_t *chebyshev) {
@@ -310,7 +347,8 @@ int32_t hyd_set_empty_grid(hyd_ctx *c, const int32_t dims[3], const int32_t *che
   if (st) return st;
   c->cheb.data = c->own_cheb;
   std::memcpy(c->cheb.dim, dims, 3 * sizeof(int32_t));
-  return HYD_OK;
+  c->host_cheb.assign(chebyshev, chebyshev + cells);
+  return rebuild_codes(c);
 }
 
 int32_t hyd_set_mean_grid(hyd_ctx *c, const int32_t dims[3], const float *mean) {
@@ -629,6 +667,7 @@ int32_t hyd_render_iso(hyd_ctx *c, const hyd_iso_params *p, const hyd_view *view
   std::memcpy(a.cdim, c->cheb.dim, sizeof(a.cdim));
   a.vaddr = c->paging.vaddr;
   std::memcpy(a.vdim, c->paging.vdim, sizeof(a.vdim));
+  a.codes = c->codes_valid ? c->own_codes : nullptr;
   a.page = c->paging.page;
   a.pool = c->pool.data;
   a.pool_elems = (uint64_t)c->pool.n_pages * c->pool.pad * c->pool.pad * c->pool.pad;

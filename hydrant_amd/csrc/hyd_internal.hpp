@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/hydrant_hip.h"
 #include "hyd_math.hpp"
@@ -82,6 +83,13 @@ struct Ctx {
   uint64_t oriented_gen = ~0ull;  // pool_gen the oriented layouts were made from
   int32_t *own_vaddr = nullptr;
   float4 *own_page = nullptr;
+  size_t vaddr_cap = 0, page_cap = 0;   // bytes of own_vaddr / own_page
+  // host copies of the page table and chebyshev grid, combined into one code per
+  // brick (see VolumeArgs::codes) while their dims agree
+  std::vector<int32_t> host_vaddr, host_cheb;
+  int32_t *own_codes = nullptr;
+  size_t codes_cap = 0;
+  bool codes_valid = false;
   int32_t *own_cheb = nullptr;
   float *own_mean = nullptr;
   float *own_tf = nullptr;
@@ -125,6 +133,9 @@ struct VolumeArgs {
   int32_t cdim[3];
   const int32_t *vaddr;
   int32_t vdim[3];
+  // one code per brick when the chebyshev and page-table grids have equal dims
+  // (else null): page id >= 0, -1 not resident, <= -2 empty at distance -1 - code
+  const int32_t *codes;
   int32_t lowest_blkcnt;
   const float4 *page;
   const void *pool;
@@ -196,6 +207,7 @@ struct IsoArgs {
   int32_t cdim[3];
   const int32_t *vaddr;
   int32_t vdim[3];
+  const int32_t *codes;   // as VolumeArgs::codes
   const float4 *page;
   const void *pool;
   uint64_t pool_elems;    // voxels in the pool (n_pages * pad^3)

@@ -60,6 +60,19 @@ HD bool intersect_box(f3 o, f3 d, f3 bmin, f3 bmax, float &tnear, float &tfar) {
   return tfar > tnear;
 }
 
+// intersect_box with the ray's reciprocal direction invr = 1 / d (componentwise,
+// correctly rounded) computed once per ray: the same operations on the same
+// values, so the same result.
+HD void intersect_box_inv(f3 o, f3 invr, f3 bmin, f3 bmax, float &tnear, float &tfar) {
+  f3 tbot = mul(invr, sub(bmin, o));
+  f3 ttop = mul(invr, sub(bmax, o));
+  f3 tmin = {gmin(ttop.x, tbot.x), gmin(ttop.y, tbot.y), gmin(ttop.z, tbot.z)};
+  f3 tmax = {gmax(ttop.x, tbot.x), gmax(ttop.y, tbot.y), gmax(ttop.z, tbot.z)};
+  tnear = comp_max(tmin);
+  tfar = comp_min(tmax);
+}
+HD f3 recip3(f3 d) { return {1.f / d.x, 1.f / d.y, 1.f / d.z}; }
+
 // Column-major 4x4, m[4*col + row].
 // glm mat4 * vec4: (m0*x + m1*y) + (m2*z + m3*w).
 HD void mat4_vec4(const float *m, float x, float y, float z, float w, float out[4]) {

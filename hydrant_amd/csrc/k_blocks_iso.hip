@@ -32,6 +32,16 @@ __device__ __forceinline__ int skip_nblock_steps(f3 &o, f3 d, f3 ip, int nblocks
   return (int)di;
 }
 
+// skip_nblock_steps with the ray's reciprocal direction computed once (same result).
+__device__ __forceinline__ int skip_nblock_steps_inv(f3 &o, f3 invr, f3 d, f3 ip, int nblocks, float cdu,
+                                                     float step) {
+  float tn, tfar;
+  intersect_box_inv(o, invr, ip, f3{ip.x + 1.f, ip.y + 1.f, ip.z + 1.f}, tn, tfar);
+  const float di = ceilf((tfar + float(nblocks - 1) * cdu) / step);
+  o = add(o, muls(muls(d, di), step));
+  return (int)di;
+}
+
 // VisBlocksShaderKernel::main, blocks_shader.cu:16-50; HYD_BLOCKS_PAGING is the
 // Paging renderer's residency view, PagingShaderKernel::main (paging_shader.cu:24-61).
 __global__ __launch_bounds__(64) void blocks_kernel(BlocksArgs a) {
@@ -161,7 +171,7 @@ constexpr int32_t kIsoPending = -1;
 template <int FMT, int U, class OFF>
 __device__ bool iso_march_grouped(const IsoArgs &a, const float *lut, f3 &o, f3 d, int nsteps, float cdu,
                                   hyd_iso_pixel &px, unsigned &count) {
-  const float fpad = float(a.pad);
+  const f3 invr = recip3(d);
   const OFF pad3 = (OFF)a.pad * (OFF)a.pad * (OFF)a.pad;
   float prev_value = 0.f;
   bool have_prev = false;
@@ -170,16 +180,21 @@ __device__ bool iso_march_grouped(const IsoArgs &a, const float *lut, f3 &o, f3 
   f3 cip = {-INFINITY, -INFINITY, -INFINITY};
   int ccd = 0, cpg = -1, dpg = -1;
   float4 pgd = make_float4(0.f, 0.f, 0.f, 0.f);
-  (void)fpad;
   while (nsteps > 0) {
     const f3 ip = floor3(o);
     if (ip.x != cip.x || ip.y != cip.y || ip.z != cip.z) {
       cip = ip;
-      ccd = a.cheb[grid_index(ip, a.cdim)];
-      if (ccd == 0) cpg = a.vaddr[grid_index(ip, a.vdim)];
+      if (a.codes) {
+        const int code = a.codes[grid_index(ip, a.vdim)];
+        ccd = code <= -2 ? -1 - code : 0;
+        cpg = code;
+      } else {
+        ccd = a.cheb[grid_index(ip, a.cdim)];
+        if (ccd == 0) cpg = a.vaddr[grid_index(ip, a.vdim)];
+      }
     }
     if (ccd) {
-      nsteps -= skip_nblock_steps(o, d, ip, ccd, cdu, a.step);
+      nsteps -= skip_nblock_steps_inv(o, invr, d, ip, ccd, cdu, a.step);
       have_prev = false;
       o = add(o, muls(d, a.step));
       nsteps -= 1;

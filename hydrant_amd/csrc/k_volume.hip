@@ -180,12 +180,20 @@ __device__ __forceinline__ void shade_composite(const VolumeArgs &a, const float
 // The chebyshev and page-table entries of the brick the last sample fell in:
 // consecutive samples inside one brick read them once (the page entry only
 // for a non-empty brick).
+// With the combined per-brick code (equal grid dims) this is one load.
 struct BrickLookup {
   f3 ip = {-INFINITY, -INFINITY, -INFINITY};
   int cd = 0, gi = 0, pg = -1;
   __device__ __forceinline__ void update(const VolumeArgs &a, f3 p) {
     if (p.x == ip.x && p.y == ip.y && p.z == ip.z) return;
     ip = p;
+    if (a.codes) {
+      gi = grid_index(p, a.vdim);
+      const int code = a.codes[gi];
+      cd = code <= -2 ? -1 - code : 0;
+      pg = code;
+      return;
+    }
     cd = a.cheb[grid_index(p, a.cdim)];
     if (cd == 0) {
       gi = grid_index(p, a.vdim);
@@ -193,6 +201,13 @@ struct BrickLookup {
     }
   }
 };
+
+// Cell code of brick ip: page id, -1 not resident, or -1 - chebyshev distance.
+__device__ __forceinline__ int cell_code(const VolumeArgs &a, f3 ip) {
+  if (a.codes) return a.codes[grid_index(ip, a.vdim)];
+  const int cd = a.cheb[grid_index(ip, a.cdim)];
+  return cd ? -1 - cd : a.vaddr[grid_index(ip, a.vdim)];
+}
 
 // VolumeShaderKernel::main (volume_shader.cu:32-81), reference step schedule.
 // The escalation tests compare float alpha with the double literals 0.93 and
@@ -278,6 +293,7 @@ __device__ void march_reference_grouped(const VolumeArgs &a, const float *tf, co
                                         unsigned &count) {
   const float cdu = 1.f / comp_max(abs3(st.d));
   const f3 d = st.d;
+  const f3 invr = recip3(d);
   f3 o = st.o;
   int nsteps = st.nsteps;
   int step_size = st.step_size;
@@ -295,7 +311,7 @@ __device__ void march_reference_grouped(const VolumeArgs &a, const float *tf, co
       // skip_nblock_steps (volume_shader.cu:5-14) with step * step_size
       const float sstep = a.step * float(step_size);
       float tn, tfar;
-      intersect_box(o, d, ip, f3{ip.x + 1.f, ip.y + 1.f, ip.z + 1.f}, tn, tfar);
+      intersect_box_inv(o, invr, ip, f3{ip.x + 1.f, ip.y + 1.f, ip.z + 1.f}, tn, tfar);
       const float di = ceilf((tfar + float(bl.cd - 1) * cdu) / sstep);
       o = add(o, muls(muls(d, di), sstep));
       nsteps -= (int)di;
@@ -558,8 +574,34 @@ __global__ __launch_bounds__(64) void volume_trace_kernel(VolumeArgs a, int x, i
 // slot; cells not resident now are left for a later pass.  The final fold
 // composites the slots in order and stops at the first slot no pass filled
 // (the reference stops a ray at its first non-resident brick).
-template <int FMT, bool COUNT>
+//
+// A cell's samples are a run of lattice indices [k, kend_cell) (positions are
+// monotone in t, so a cell is entered once); the run's end is estimated from
+// the cell's exit distance and then moved until it agrees with floor() of the
+// sample positions, evaluated exactly as the walk evaluates them, so every
+// pass sees the same cells and samples as the single march.  A cell not
+// resident in this pass is passed over in one step; a resident one composites
+// its run with the tap loads of up to U samples in flight together.
+#ifndef HYD_SEG_GROUP
+#define HYD_SEG_GROUP 4
+#endif
+
+// Sample k of the lattice walk: position o0 + d * ((k + w/2) * step), w = min(1, L - k).
+__device__ __forceinline__ f3 lattice_pos(f3 o0, f3 d, float lf, float step, int k, float &w) {
+  w = gmin(1.f, lf - float(k));
+  const float t = (float(k) + 0.5f * w) * step;
+  return add(o0, muls(d, t));
+}
+
+__device__ __forceinline__ bool same_cell(f3 p, f3 ip) {
+  const f3 q = floor3(p);
+  return q.x == ip.x && q.y == ip.y && q.z == ip.z;
+}
+
+template <int FMT, bool COUNT, bool BIG>
 __global__ __launch_bounds__(64) void segment_kernel(VolumeArgs a) {
+  using OFF = std::conditional_t<BIG, uint64_t, uint32_t>;
+  constexpr int U = HYD_SEG_GROUP;
   __shared__ float s_tf[4 * kMaxTf];
   __shared__ float s_lut[FMT == HYD_FMT_U8 ? 256 : 1];
   __shared__ unsigned long long s_count;
@@ -580,17 +622,21 @@ __global__ __launch_bounds__(64) void segment_kernel(VolumeArgs a) {
       const float lf = __int_as_float(st.nsteps);
       const int kend = (int)ceilf(lf);
       const float cdu = 1.f / comp_max(abs3(d));
+      const f3 invr = recip3(d);
+      const OFF pad3 = (OFF)a.pad * (OFF)a.pad * (OFF)a.pad;
+      OFF c0 = ~(OFF)0, c7 = ~(OFF)0;   // taps of the last composited sample
+      float cur[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       int k = 0;
-      VolTaps tc;   // taps of the last sample
       while (k < kend && ord < a.max_slots) {
-        float w = gmin(1.f, lf - float(k));
-        float t = (float(k) + 0.5f * w) * a.step;
-        f3 o = add(o0, muls(d, t));
+        float w;
+        const f3 o = lattice_pos(o0, d, lf, a.step, k, w);
+        const float t = (float(k) + 0.5f * w) * a.step;
         const f3 ip = floor3(o);
-        const int cd = a.cheb[grid_index(ip, a.cdim)];
+        const int code = cell_code(a, ip);
+        const int cd = code <= -2 ? -1 - code : 0;
+        float tn, tfar;
+        intersect_box_inv(o, invr, ip, f3{ip.x + 1.f, ip.y + 1.f, ip.z + 1.f}, tn, tfar);
         if (cd) {   // empty cell: same skip as the single lattice march
-          float tn, tfar;
-          intersect_box(o, d, ip, f3{ip.x + 1.f, ip.y + 1.f, ip.z + 1.f}, tn, tfar);
           const float t_exit = t + tfar + float(cd - 1) * cdu;
           const int kn = (int)ceilf(t_exit / a.step - 0.5f);
           k = kn > k + 1 ? kn : k + 1;
@@ -598,27 +644,52 @@ __global__ __launch_bounds__(64) void segment_kernel(VolumeArgs a) {
           ++ord;
           continue;
         }
-        const int pgid = a.vaddr[grid_index(ip, a.vdim)];
-        MarchState seg;
-        seg.v = make_float4(0.f, 0.f, 0.f, 0.f);
-        seg.theta = f3{0.f, 0.f, 0.f};
-        seg.phi = 1.f;
-        const float4 pg = pgid >= 0 ? a.page[pgid] : make_float4(0.f, 0.f, 0.f, 0.f);
-        while (true) {
-          if (pgid >= 0) {
-            const float s = brick_sample_reuse<FMT>(a.pool, a.pad, pgid, pg, sub(o, ip), s_lut, nullptr, tc);
-            shade_composite(a, s_tf, s, pgid, seg, w);
-            if (COUNT) ++count;
+        // the cell's run [k, kc): estimate from the exit distance, then agree with floor()
+        int kc = (int)ceilf((t + tfar) / a.step - 0.5f);
+        kc = kc > k + 1 ? kc : k + 1;
+        kc = kc < kend ? kc : kend;
+        float wq;
+        while (kc < kend && same_cell(lattice_pos(o0, d, lf, a.step, kc, wq), ip)) ++kc;
+        while (kc - 1 > k && !same_cell(lattice_pos(o0, d, lf, a.step, kc - 1, wq), ip)) --kc;
+        const int pgid = code;
+        if (pgid >= 0) {
+          const float4 pgd = a.page[pgid];
+          const OFF page_base = (OFF)pgid * pad3;
+          MarchState seg;
+          seg.v = make_float4(0.f, 0.f, 0.f, 0.f);
+          for (int kk = k; kk < kc; kk += U) {
+            RefTap<OFF> tp[U];
+            RawTaps<FMT> raw[U];
+            bool ok[U], fresh[U];
+            float wj[U];
+            OFF p0 = c0, p7 = c7;
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+              ok[j] = kk + j < kc;
+              const f3 pj = lattice_pos(o0, d, lf, a.step, ok[j] ? kk + j : kk, wj[j]);
+              tp[j] = ref_tap<OFF>(sub(pj, ip), pgd, a.pad, page_base);
+              const OFF i0 = tp[j].r00 + (OFF)tp[j].ix, i7 = tp[j].r11 + (OFF)tp[j].jx;
+              fresh[j] = i0 != p0 || i7 != p7;
+              p0 = i0;
+              p7 = i7;
+              if (ok[j] && fresh[j]) ref_fetch<FMT, OFF>(a.pool, tp[j], a.pad, raw[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+              if (ok[j]) {
+                if (fresh[j]) ref_values<FMT, OFF>(raw[j], tp[j], a.pad, s_lut, cur);
+                const float s = ref_interp<OFF>(cur, tp[j]);
+                shade_composite_impl<true, false>(a, s_tf, s, pgid, seg, wj[j]);
+                c0 = tp[j].r00 + (OFF)tp[j].ix;
+                c7 = tp[j].r11 + (OFF)tp[j].jx;
+                if (COUNT) ++count;
+              }
+            }
           }
-          if (++k >= kend) break;
-          w = gmin(1.f, lf - float(k));
-          t = (float(k) + 0.5f * w) * a.step;
-          o = add(o0, muls(d, t));
-          const f3 nip = floor3(o);
-          if (nip.x != ip.x || nip.y != ip.y || nip.z != ip.z) break;
+          a.slots[(size_t)ord * P + ray] = seg.v;
         }
-        if (pgid >= 0) a.slots[(size_t)ord * P + ray] = seg.v;
         ++ord;
+        k = kc;
       }
     }
     a.nslots[ray] = ord;
@@ -651,12 +722,7 @@ __global__ __launch_bounds__(256) void segment_fold_kernel(const float4 *slots, 
 }
 
 template <int FMT>
-hipError_t launch_segments_fmt(const VolumeArgs &a, hipStream_t s) {
-  dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile);
-  if (a.samples) hipLaunchKernelGGL((segment_kernel<FMT, true>), grid, dim3(64), 0, s, a);
-  else hipLaunchKernelGGL((segment_kernel<FMT, false>), grid, dim3(64), 0, s, a);
-  return hipGetLastError();
-}
+hipError_t launch_segments_fmt(const VolumeArgs &a, hipStream_t s);
 
 template <int FMT, int MODE, bool BIG>
 void launch_mode(const VolumeArgs &a, hipStream_t s) {
@@ -681,6 +747,21 @@ bool force_offsets64() {
     return e && e[0] == '1';
   }();
   return on;
+}
+
+template <int FMT>
+hipError_t launch_segments_fmt(const VolumeArgs &a, hipStream_t s) {
+  dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile);
+  const uint64_t bytes = a.pool_elems * (uint64_t)sizeof(typename RawVox<FMT>::T);
+  const bool big = bytes > (1ull << 32) || force_offsets64();
+  if (a.samples) {
+    if (big) hipLaunchKernelGGL((segment_kernel<FMT, true, true>), grid, dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((segment_kernel<FMT, true, false>), grid, dim3(64), 0, s, a);
+  } else {
+    if (big) hipLaunchKernelGGL((segment_kernel<FMT, false, true>), grid, dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((segment_kernel<FMT, false, false>), grid, dim3(64), 0, s, a);
+  }
+  return hipGetLastError();
 }
 
 template <int FMT>

@@ -245,6 +245,37 @@ class Context:
                C.c_void_p(nslots.data_ptr()), C.c_void_p(rgba.data_ptr()))
         return rgba, cnt
 
+    def render_sort_last_resident(self, params: L.VolumeParams, view: L.View, W: int, H: int, batches, grid_dims,
+                                  slots=None, nslots=None, rgba=None, count: bool = False):
+        """render_sort_last with every batch's pool already in device memory: batches is a
+        list of (fmt, pad, pool_tensor, vaddr, page_k, page_b) with pool_tensor a device
+        tensor of the batch's bricks.  Per batch the pool is bound (no copy) and its page
+        table set; slots / nslots / rgba may be passed in to reuse their memory."""
+        t = self.torch
+        self.sync_stream()
+        max_slots = int(sum(grid_dims))
+        if slots is None:
+            slots = t.empty((max_slots, H, W, 4), dtype=t.float32, device=self.dev)
+        check_output(slots, max_slots * H * W * 4, "segment slots")
+        if nslots is None:
+            nslots = t.zeros((H, W), dtype=t.int32, device=self.dev)
+        if rgba is None:
+            rgba = t.empty((H, W, 4), dtype=t.float32, device=self.dev)
+        check_output(rgba, H * W * 4, "sort-last output")
+        if nslots.dtype != t.int32 or not nslots.is_contiguous() or nslots.numel() < H * W:
+            raise ValueError("nslots must be a contiguous int32 tensor of H * W entries")
+        cnt = t.zeros(1, dtype=t.int64, device=self.dev) if count else None
+        L.call("hyd_segments_begin", self.h, W, H, max_slots, C.c_void_p(slots.data_ptr()))
+        for fmt, pad, pool_t, vaddr, page_k, page_b in batches:
+            self.bind_pool_device(fmt, pad, int(pool_t.shape[0]), pool_t.data_ptr())
+            self.set_page_table(vaddr, page_k, page_b)
+            L.call("hyd_segments_pass", self.h, C.byref(params), C.byref(view), W, H, max_slots,
+                   C.c_void_p(slots.data_ptr()), C.c_void_p(nslots.data_ptr()),
+                   C.c_void_p(cnt.data_ptr() if count else 0))
+        L.call("hyd_segments_composite", self.h, W, H, max_slots, C.c_void_p(slots.data_ptr()),
+               C.c_void_p(nslots.data_ptr()), C.c_void_p(rgba.data_ptr()))
+        return rgba, cnt
+
     def trace_volume(self, params: L.VolumeParams, view: L.View, W: int, H: int, x: int, y: int,
                      max_recs: int = 8192) -> np.ndarray:
         recs = (L.TraceRec * max_recs)()
