@@ -281,16 +281,20 @@ def paged_iso_checks(paged, iso):
     return out
 
 
-def bench_batched(torch, args, plain_ms_per_sample=None):
+MAX_ERROR = 1e-3   # max-abs per channel against the FP64 oracle
+
+
+def bench_batched(torch, args, fmt=None):
     """96 views (resolutions in {256..1024}^2) of three blob volumes (256^3, 512^3, 768^3)
-    as FP16 bricks, one persistent launch replayed from a HIP graph; launch shape from
-    the autotuner."""
+    as FP16 (or u8) bricks, one persistent launch replayed from a HIP graph; launch shape
+    and early-termination threshold from the autotuner (error budget 1e-3)."""
     from hydrant_amd import _lib as L
     from hydrant_amd import workloads as W
     from hydrant_amd.hip import Context
 
+    fmt = L.FMT_F16 if fmt is None else fmt
     t0 = time.perf_counter()
-    wl = W.batched(fmt=L.FMT_F16)
+    wl = W.batched(fmt=fmt)
     t_gen = time.perf_counter() - t0
     ctx = Context(0)
     ctx.upload_pool(wl.fmt, wl.bs + 2 * wl.apron, wl.pool)
@@ -304,7 +308,7 @@ def bench_batched(torch, args, plain_ms_per_sample=None):
         parts = [int(x) for x in args.config.split(",")]
         best, tune_ms = batch_config(*parts), None
     else:
-        best, tune_ms = b.autotune(wl.density, wl.ert, out, reps=2)
+        best, wl.ert, tune_ms = b.autotune_schedule(wl.density, MAX_ERROR, out, reps=2)
     _, cnt = b.render(best, wl.density, wl.ert, out=out, count=True)
     torch.cuda.synchronize()
     n_samples = int(cnt.item())
@@ -317,11 +321,12 @@ def bench_batched(torch, args, plain_ms_per_sample=None):
     ms_step = wall * 1e3 / args.steps
     k_ms = float(np.mean(ev))
     n_px = b.n_pixels
-    alg = n_samples * 8 * 2 + n_px * 16
+    vb = {L.FMT_U8: 1, L.FMT_F16: 2, L.FMT_F32: 4}[fmt]
+    alg = n_samples * 8 * vb + n_px * 16
     res = {"n_samples": n_samples, "ms_per_step": ms_step, "kernel_ms": k_ms,
-           "msamples_per_s": n_samples / (ms_step * 1e-3) / 1e6, "alg_bytes": alg,
+           "msamples_per_s": n_samples / (ms_step * 1e-3) / 1e6, "alg_bytes": alg, "bytes_per_voxel": vb,
            "achieved_gbs": alg / (k_ms * 1e-3) / 1e9, "config": best.as_dict(), "autotune_best_ms": tune_ms,
-           "n_pixels": n_px, "gen_s": t_gen, "wl": wl, "batch": b, "out": out, "ctx": ctx}
+           "ert": wl.ert, "n_pixels": n_px, "gen_s": t_gen, "wl": wl, "batch": b, "out": out, "ctx": ctx}
     return res
 
 
@@ -378,24 +383,35 @@ def bench_diff(torch, args):
     return res
 
 
-def batched_errors(res, n_views: int = 6):
-    """Per-view max-abs error of the batched FP16 output against the FP64 oracle (exact u8
-    source values) on the first n_views views: full frame up to 512^2, else a 64x64 crop
-    plus every 7th pixel."""
-    from tests.helpers import error_pixels, exact_pool_values, oracle_view, oracle_volume
+def batched_errors(res, stride: int = 16):
+    """Per-view max-abs error of a batched output against the FP64 oracle over every view:
+    every stride-th pixel (offset by the view index), exact u8 source values, lattice
+    schedule with early termination off, so the error includes the truncation of the
+    autotuned termination threshold."""
+    from tests.helpers import exact_pool_values, oracle_view, oracle_volume
 
     wl, b, out = res["wl"], res["batch"], res["out"]
+    pools = {}
     errs = []
-    for i in range(min(n_views, len(wl.views))):
+    t0 = time.perf_counter()
+    for i in range(len(wl.views)):
         sc = wl.scene(i)
-        xy = error_pixels(sc.W, sc.H)
-        osc = oracle_volume(sc, pool_values=exact_pool_values(sc), exact_mapping=True)
+        sc.ert = 0.0
+        vi = wl.views[i]["volume"]
+        if vi not in pools:
+            pools[vi] = exact_pool_values(sc)
+        idx = np.arange(i % stride, sc.W * sc.H, stride)
+        xy = np.stack([idx % sc.W, idx // sc.W], 1).astype(np.int32)
+        osc = oracle_volume(sc, pool_values=pools[vi], exact_mapping=True)
         ref, _ = osc.render_pixels(oracle_view(sc), xy, prec=64, nthreads=cpu_threads())
         img = b.view_image(out, i).cpu().numpy()
         got = img[xy[:, 1], xy[:, 0]].astype(np.float64)
-        errs.append({"view": i, "res": sc.W, "volume": wl.views[i]["volume"], "pixels": int(xy.shape[0]),
+        errs.append({"view": i, "res": sc.W, "volume": vi, "pixels": int(xy.shape[0]),
                      "max_abs": float(np.max(np.abs(got - ref)))})
-    return errs
+    summary = {"views": len(errs), "pixels": int(sum(e["pixels"] for e in errs)),
+               "max_abs": max(e["max_abs"] for e in errs), "stride": stride, "oracle_ert": 0.0,
+               "cpu_s": round(time.perf_counter() - t0, 1)}
+    return errs, summary
 
 
 def cpu_baseline_batched(res, budget_s: float = 12.0):
@@ -444,6 +460,66 @@ def cpu_baseline_plain(sc, budget_s: float = 12.0):
             "ms_per_frame": dt * 1e3, "rgba": r["rgba"]}
 
 
+def fp64_errors(plain, paged, sortlast, iso):
+    """Each frame against the FP64 oracle of the same schedule: plain (full frame) and paged
+    (every 7th pixel) under the reference schedule, whose escalation thresholds and integer
+    step budget can move a whole sample between FP32 and FP64 on a few rays (bounded by one
+    sample's opacity, density x max TF alpha = 1e-2); sort-last (every 7th pixel) under the
+    lattice schedule; the isosurface hit mask and depth (full frame)."""
+    from oracle import oracle as O
+    from tests.helpers import oracle_view, oracle_volume
+
+    out = {}
+    nt = cpu_threads()
+    if plain is not None:
+        sc = plain["scene"]
+        ref = oracle_volume(sc).render(oracle_view(sc), 64, nthreads=nt, film=False)["rgba"]
+        d = np.abs(plain["rgba"].cpu().numpy().astype(np.float64) - ref).max(axis=-1)
+        out["plain"] = {"max_abs": float(d.max()), "p999": float(np.percentile(d, 99.9)),
+                        "frac_within_1e-4": float(np.mean(d <= 1e-4)), "pixels": int(d.size)}
+    for name, res, sched in (("paged", paged, None), ("sort_last", sortlast, None)):
+        if res is None:
+            continue
+        sc = res["scene"]
+        idx = np.arange(0, sc.W * sc.H, 7)
+        xy = np.stack([idx % sc.W, idx // sc.W], 1).astype(np.int32)
+        ref, _ = oracle_volume(sc).render_pixels(oracle_view(sc), xy, prec=64, nthreads=nt)
+        got = res["rgba"].cpu().numpy()[xy[:, 1], xy[:, 0]].astype(np.float64)
+        d = np.abs(got - ref).max(axis=-1)
+        out[name] = {"max_abs": float(d.max()), "p999": float(np.percentile(d, 99.9)),
+                     "frac_within_1e-4": float(np.mean(d <= 1e-4)), "pixels": int(d.size)}
+    if iso is not None:
+        sc, p = iso["scene"], iso["params"]
+        b = sc.bricked
+        osc = O.IsoScene(sc.W, sc.H, sc.bbox_min, sc.bbox_max, p.step, p.du, np.array(p.to_world[:]), p.eye_pos[:],
+                         p.light_pos[:], p.surface_color[:], p.isovalue, b.cheb, b.vaddr, b.page_k, b.page_b,
+                         b.pool_values(), b.pad, mode=p.mode, corrected=p.corrected, bisect_steps=p.bisect_steps,
+                         max_steps=p.max_steps)
+        r = osc.render(oracle_view(sc), 64, nthreads=nt)["px"]
+        px = iso["px"].cpu().numpy()
+        g_hit, o_hit = px[..., 5].view(np.int32) == 1, r[..., 5] == 1
+        both = g_hit & o_hit
+        out["isosurface"] = {"hit_xor": int(np.count_nonzero(g_hit != o_hit)), "hits": int(o_hit.sum()),
+                             "max_depth_diff_voxel": float(np.max(np.abs(px[..., 4][both] - r[..., 4][both])) * b.bs)
+                             if both.any() else 0.0}
+    return out
+
+
+def batched_info(res, plain):
+    from hydrant_amd import _lib as L
+
+    name = {L.FMT_U8: "u8", L.FMT_F16: "f16", L.FMT_F32: "f32"}[res["wl"].fmt]
+    return {"msamples_per_s": round(res["msamples_per_s"], 3), "ms_per_batch": round(res["ms_per_step"], 4),
+            "samples_per_batch": res["n_samples"], "views": len(res["wl"].views), "pixels": res["n_pixels"],
+            "volumes": [256, 512, 768], "brick_format": name, "launch_config": res["config"],
+            "ert": res["ert"], "error_budget": MAX_ERROR,
+            "kernel": f"batch_kernel<{name}> (lattice schedule, HIP graph replay)",
+            "kernel_ms": round(res["kernel_ms"], 4),
+            "achieved_gbs": round(res["achieved_gbs"], 1),
+            "alg_bytes_rule": f"samples x 8 taps x {res['bytes_per_voxel']} B + pixels x 16 B",
+            "ratio_vs_plain_fp32": round(res["msamples_per_s"] / plain["msamples_per_s"], 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -451,10 +527,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="batched", choices=["batched", "plain"],
                     help="headline workload (the other one is reported alongside)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg and the FP64 error check")
-    ap.add_argument("--err-views", type=int, default=6, help="batched views checked against the FP64 oracle")
-    ap.add_argument("--no-paged-iso", action="store_true", help="skip the paged and isosurface legs")
-    ap.add_argument("--only", choices=["plain", "batched", "diff", "paged", "sortlast", "iso"],
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines and the FP64 error checks")
+    ap.add_argument("--err-stride", type=int, default=16,
+                    help="batched views: every n-th pixel of every view is checked against the FP64 oracle")
+    ap.add_argument("--no-paged-iso", action="store_true", help="skip the paged, sort-last and isosurface legs")
+    ap.add_argument("--no-u8", action="store_true", help="skip the u8 batched leg")
+    ap.add_argument("--only", choices=["plain", "batched", "batched_u8", "diff", "paged", "sortlast", "iso"],
                     help="profiling aid: run only this workload's timed launches and print its line")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-form leg")
     ap.add_argument("--diff-fit-steps", type=int, default=200, help="Adam steps of the differentiable fit")
@@ -472,6 +550,7 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    from hydrant_amd import _lib as L
 
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
@@ -486,8 +565,10 @@ def main():
         print(json.dumps(bench_diff(torch, args)), flush=True)
         return
     if args.only:
-        res = {"plain": bench_plain, "batched": bench_batched, "paged": bench_paged, "iso": bench_iso,
-               "sortlast": bench_paged_sort_last}[args.only](torch, args)
+        fns = {"plain": bench_plain, "batched": bench_batched, "paged": bench_paged, "iso": bench_iso,
+               "sortlast": bench_paged_sort_last,
+               "batched_u8": lambda t, a: bench_batched(t, a, fmt=L.FMT_U8)}
+        res = fns[args.only](torch, args)
         print(json.dumps({k: v for k, v in res.items() if isinstance(v, (int, float, str, dict))}), flush=True)
         return
 
@@ -496,7 +577,12 @@ def main():
           file=sys.stderr, flush=True)
     batched = bench_batched(torch, args)
     print(f"# batched fp16: {batched['msamples_per_s']:.1f} Msamples/s, {batched['ms_per_step']:.3f} ms/batch, "
-          f"config {batched['config']}", file=sys.stderr, flush=True)
+          f"config {batched['config']}, ert {batched['ert']}", file=sys.stderr, flush=True)
+    batched_u8 = None
+    if not args.no_u8:
+        batched_u8 = bench_batched(torch, args, fmt=L.FMT_U8)
+        print(f"# batched u8: {batched_u8['msamples_per_s']:.1f} Msamples/s, {batched_u8['ms_per_step']:.3f} "
+              f"ms/batch, config {batched_u8['config']}, ert {batched_u8['ert']}", file=sys.stderr, flush=True)
 
     diff = None if args.no_diff else bench_diff(torch, args)
     if diff is not None:
@@ -518,19 +604,14 @@ def main():
     head = batched if args.workload == "batched" else plain
     plain_info = {"msamples_per_s": round(plain["msamples_per_s"], 3), "ms_per_frame": round(plain["ms_per_step"], 4),
                   "samples_per_frame": plain["n_samples"], "kernel": "volume_kernel<f32> (reference schedule)",
-                  "kernel_ms": round(plain["kernel_ms"], 4)}
-    batched_info = {"msamples_per_s": round(batched["msamples_per_s"], 3),
-                    "ms_per_batch": round(batched["ms_per_step"], 4), "samples_per_batch": batched["n_samples"],
-                    "views": len(batched["wl"].views), "pixels": batched["n_pixels"], "volumes": [256, 512, 768],
-                    "brick_format": "f16", "launch_config": batched["config"],
-                    "kernel": "batch_kernel<f16> (lattice schedule, HIP graph replay)",
-                    "kernel_ms": round(batched["kernel_ms"], 4),
-                    "ratio_vs_plain_fp32": round(batched["msamples_per_s"] / plain["msamples_per_s"], 3)}
+                  "kernel_ms": round(plain["kernel_ms"], 4), "achieved_gbs": round(plain["achieved_gbs"], 1),
+                  "alg_bytes_rule": "samples x 8 taps x 4 B + pixels x 16 B"}
     if head is batched:
         workload = ("batched_fp16: 96 views (seeded resolutions in {256,384,512,768,1024}^2, seeded orbit "
                     "angles) of three Gaussian-blob volumes 256^3/512^3/768^3 as FP16 64^3+1 bricks, one "
-                    "persistent launch replayed from a HIP graph, lattice schedule (ERT 0.9995), README TF, "
-                    "density 1e-2, step 0.5 voxel")
+                    f"persistent launch replayed from a HIP graph, lattice schedule (early termination at "
+                    f"{batched['ert']}, autotuned within a 1e-3 error budget), README TF, density 1e-2, "
+                    "step 0.5 voxel")
         bytes_per_voxel, kern = 2, "batch_kernel<f16>"
     else:
         workload = ("plain_volume_fp32: 256^3 u8 Gaussian-blob volume as one FP32 brick, 512x512, orbit "
@@ -551,7 +632,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "samples_per_step": head["n_samples"]},
         "plain_fp32": plain_info,
-        "batched_fp16": batched_info,
+        "batched_fp16": batched_info(batched, plain),
+        "batched_u8": None if batched_u8 is None else batched_info(batched_u8, plain),
         "diff": diff,
         "paged_u8": None if paged is None else {k: v for k, v in paged.items() if k not in ("scene", "rgba")},
         "paged_sort_last_u8": None if sortlast is None else {k: v for k, v in sortlast.items()
@@ -574,15 +656,25 @@ def main():
                 out["paged_u8"].update({k: v for k, v in chk.items() if k.startswith("paged_")})
             if iso is not None:
                 out["isosurface_f16"].update({k: v for k, v in chk.items() if k.startswith("iso_")})
-        errs = batched_errors(batched, args.err_views)
-        out["batched_fp16"]["max_abs_vs_fp64"] = max(e["max_abs"] for e in errs)
-        out["batched_fp16"]["per_view_error"] = errs
+        out["fp64_error"] = fp64_errors(plain, paged, sortlast, iso)
+        print("# fp64 errors of the single-view frames done", file=sys.stderr, flush=True)
+        for key, res in (("batched_fp16", batched), ("batched_u8", batched_u8)):
+            if res is None:
+                continue
+            errs, summary = batched_errors(res, args.err_stride)
+            out[key]["max_abs_vs_fp64"] = summary["max_abs"]
+            out[key]["fp64_check"] = summary
+            out[key]["per_view_error"] = [round(e["max_abs"], 8) for e in errs]
+            print(f"# {key}: max-abs {summary['max_abs']:.3g} vs FP64 over {summary['views']} views, "
+                  f"{summary['pixels']} pixels", file=sys.stderr, flush=True)
+        cb = cpu_baseline_plain(plain["scene"])
+        plain_cpu = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        plain_cpu["value"] = round(cb["value"], 3)
         if head is batched:
             out["cpu_baseline"] = cpu_baseline_batched(batched)
+            out["cpu_baseline_plain"] = plain_cpu
         else:
-            cb = cpu_baseline_plain(plain["scene"])
-            out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
-            out["cpu_baseline"]["value"] = round(cb["value"], 3)
+            out["cpu_baseline"] = plain_cpu
     print(json.dumps(out), flush=True)
 
 

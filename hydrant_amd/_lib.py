@@ -96,6 +96,12 @@ class Dataset(C.Structure):
                 ("apron", C.c_int32)]
 
 
+class PackageInfo(C.Structure):
+    """hyd_package_info (include/hydrant_renderer.h)."""
+    _fields_ = [("raw", C.c_int32 * 3), ("dim", C.c_int32 * 3), ("block_size", C.c_int32), ("padding", C.c_int32),
+                ("fmt", C.c_int32), ("has_mean", C.c_int32)]
+
+
 class BatchConfig(C.Structure):
     _fields_ = [("tile_w", I32), ("unroll", I32), ("waves_per_cu", I32), ("group", I32)]
 
@@ -152,6 +158,8 @@ SIGNATURES = {
     "hyd_batch_replay": (I32, [P, P]),
     "hyd_batch_autotune": (I32, [P, P, C.c_float, C.c_float, P, I32, C.POINTER(BatchConfig),
                                  C.POINTER(C.c_float)]),
+    "hyd_batch_autotune_schedule": (I32, [P, P, C.c_float, C.c_float, P, I32, C.POINTER(BatchConfig),
+                                          C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     # hydrant_dataset.h
     "hyd_gen_blobs_u8": (I32, [I32, I32, I32, I32, C.c_uint64, C.c_double, C.c_double, C.c_double,
                                C.c_double, C.c_double, P]),
@@ -175,6 +183,9 @@ SIGNATURES = {
     # hydrant_renderer.h
     "hyd_renderer_candidates": (I32, [C.c_char_p, I32, C.POINTER(I32)]),
     "hyd_renderer_create": (I32, [P, C.POINTER(Dataset), C.c_char_p, C.POINTER(P)]),
+    "hyd_package_read_info": (I32, [C.c_char_p, C.POINTER(PackageInfo)]),
+    "hyd_package_read_thumbnails": (I32, [C.c_char_p, PI32, PF, C.c_int64]),
+    "hyd_renderer_create_package": (I32, [P, C.c_char_p, C.c_char_p, C.POINTER(P)]),
     "hyd_renderer_update": (I32, [P, C.c_char_p]),
     "hyd_renderer_resolution": (I32, [P, C.POINTER(I32), C.POINTER(I32)]),
     "hyd_renderer_offline_render": (I32, [P, C.c_char_p, P]),

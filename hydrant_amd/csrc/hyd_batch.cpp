@@ -389,6 +389,55 @@ int32_t hyd_batch_replay(hyd_ctx *c, hyd_batch *b) {
   return HYD_OK;
 }
 
+int32_t hyd_batch_autotune_schedule(hyd_ctx *c, hyd_batch *b, float density, float max_error, float *rgba_dev,
+                                    int32_t reps, hyd_batch_config *best, float *best_ert, float *best_ms) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(b && best && best_ert && reps >= 1, HYD_ERR_ARG, "invalid autotune arguments");
+  HYD_REQUIRE(max_error > 0.f, HYD_ERR_ARG, "max_error must be > 0");
+  // termination thresholds whose truncation (at most 1 - ert of any channel)
+  // takes at most half the error budget; 0 (no termination) is always admissible
+  std::vector<float> erts;
+  for (float ert : {0.99f, 0.995f, 0.999f, 0.9995f, 0.9999f})
+    if (1.f - ert <= 0.5f * max_error) erts.push_back(ert);
+  erts.push_back(0.f);
+  // coordinate search: launch shapes at the earliest admissible termination,
+  // then every admissible threshold at that shape
+  hyd_batch_config cfg;
+  float t_best = 0.f;
+  if ((st = hyd_batch_autotune(c, b, density, erts[0], rgba_dev, reps, &cfg, &t_best))) return st;
+  float e_best = erts[0];
+  hipEvent_t e0, e1;
+  HYD_CHECK_HIP(hipEventCreate(&e0));
+  HYD_CHECK_HIP(hipEventCreate(&e1));
+  for (size_t i = 1; i < erts.size() && st == HYD_OK; ++i) {
+    BatchArgs a;
+    int grid = 0;
+    if ((st = fill_args(c, b, &cfg, density, erts[i], rgba_dev, nullptr, a, grid))) break;
+    const int g = group_of(&cfg);
+    if (launch_batch(a, c->pool.fmt, cfg.tile_w, cfg.unroll, g, grid, c->stream) != hipSuccess) {
+      st = HYD_ERR_HIP;
+      set_error("autotune launch failed");
+      break;
+    }
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; ++r) (void)launch_batch(a, c->pool.fmt, cfg.tile_w, cfg.unroll, g, grid, c->stream);
+    (void)hipEventRecord(e1, c->stream);
+    if (hipEventSynchronize(e1) != hipSuccess) { st = HYD_ERR_HIP; set_error("autotune launch failed"); break; }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= (float)reps;
+    if (ms < t_best) { t_best = ms; e_best = erts[i]; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  *best = cfg;
+  *best_ert = e_best;
+  if (best_ms) *best_ms = t_best;
+  return HYD_OK;
+}
+
 int32_t hyd_batch_autotune(hyd_ctx *c, hyd_batch *b, float density, float ert, float *rgba_dev, int32_t reps,
                            hyd_batch_config *best, float *best_ms) {
   int32_t st = activate(c);

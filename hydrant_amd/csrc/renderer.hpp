@@ -34,8 +34,16 @@ struct Dataset {
   // per-brick statistics of normalised values (hyd_brick_stats), grid order
   std::array<int32_t, 3> grid{};
   std::vector<float> bmin, bmax, bmean;
+  // thumbnails of a dataset package (sample_levels[0].thumbnails, grid order);
+  // empty for a dataset given as raw voxels, whose grids are derived instead
+  std::vector<int32_t> cheb_thumb;
+  std::vector<float> mean_thumb;
   size_t n_bricks() const { return (size_t)grid[0] * grid[1] * grid[2]; }
 };
+
+// A dataset package (RendererFactory(dataset_path), renderer.hpp:53-59): reads
+// package_meta.json under `root` and the files it names.  Throws Error.
+std::shared_ptr<Dataset> read_package(const std::string &root, bool load_voxels);
 
 // RendererConfig (core/renderer.schema.hpp:8-13).
 struct RendererConfig {

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <numeric>
 #include <sstream>
 
@@ -382,6 +383,13 @@ class VolumeRenderer : public IRenderer {
     check(hyd_set_tf(ctx_, n, tf_.data()));
     const Dataset &ds = *dataset_;
     empty_.assign(ds.n_bricks(), 0);
+    if (!ds.cheb_thumb.empty()) {
+      // a package's chebyshev thumbnail, as VolumeRenderer::init reads it (volume.cc:47-51)
+      for (size_t i = 0; i < ds.n_bricks(); ++i) empty_[i] = ds.cheb_thumb[i] != 0;
+      check(hyd_set_empty_grid(ctx_, ds.grid.data(), ds.cheb_thumb.data()));
+      tf_set_ = true;
+      return;
+    }
     for (size_t i = 0; i < ds.n_bricks(); ++i) empty_[i] = (uint8_t)hyd_tf_alpha_zero_on(tf_.data(), n, ds.bmin[i], ds.bmax[i]);
     std::vector<int32_t> cheb(ds.n_bricks());
     check(hyd_chebyshev(ds.grid.data(), empty_.data(), cheb.data()));
@@ -409,12 +417,19 @@ class BlocksRenderer : public IRenderer {
     IRenderer::init(std::move(dataset), cfg);
     compute_geometry();
     const Dataset &ds = *dataset_;
-    std::vector<uint8_t> empty(ds.n_bricks());
-    for (size_t i = 0; i < ds.n_bricks(); ++i) empty[i] = ds.bmax[i] <= 0.f;
-    std::vector<int32_t> cheb(ds.n_bricks());
-    check(hyd_chebyshev(ds.grid.data(), empty.data(), cheb.data()));
-    check(hyd_set_empty_grid(ctx_, ds.grid.data(), cheb.data()));
-    check(hyd_set_mean_grid(ctx_, ds.grid.data(), ds.bmean.data()));
+    if (!ds.cheb_thumb.empty()) {
+      // a package's chebyshev and mean thumbnails (blocks.cc:35-48)
+      if (ds.mean_thumb.empty()) throw Error(HYD_ERR_STATE, "Blocks: the package has no \"mean\" thumbnail");
+      check(hyd_set_empty_grid(ctx_, ds.grid.data(), ds.cheb_thumb.data()));
+      check(hyd_set_mean_grid(ctx_, ds.grid.data(), ds.mean_thumb.data()));
+    } else {
+      std::vector<uint8_t> empty(ds.n_bricks());
+      for (size_t i = 0; i < ds.n_bricks(); ++i) empty[i] = ds.bmax[i] <= 0.f;
+      std::vector<int32_t> cheb(ds.n_bricks());
+      check(hyd_chebyshev(ds.grid.data(), empty.data(), cheb.data()));
+      check(hyd_set_empty_grid(ctx_, ds.grid.data(), cheb.data()));
+      check(hyd_set_mean_grid(ctx_, ds.grid.data(), ds.bmean.data()));
+    }
     update(cfg.params);
   }
 
@@ -526,17 +541,24 @@ class IsosurfaceRenderer : public IRenderer {
     const Dataset &ds = *dataset_;
     std::vector<uint8_t> empty(ds.n_bricks());
     std::vector<int32_t> bricks;
+    // a package's chebyshev thumbnail (isosurface.cc:50-53), else the iso value's
+    // brick range
     for (size_t i = 0; i < ds.n_bricks(); ++i) {
-      empty[i] = (ds.bmin[i] > isovalue_) || (ds.bmax[i] < isovalue_);
+      empty[i] = ds.cheb_thumb.empty() ? (ds.bmin[i] > isovalue_) || (ds.bmax[i] < isovalue_)
+                                       : ds.cheb_thumb[i] != 0;
       if (!empty[i]) bricks.push_back((int32_t)i);
     }
     const int32_t fmt = pool_format_of(ds);
     if (bricks.size() > pages_per_batch(ds, fmt, mem_limit_mb_))
       throw Error(HYD_ERR_RANGE, "isosurface: " + std::to_string(bricks.size()) +
                                      " non-empty bricks exceed mem_limit_mb / 4096 pages");
-    std::vector<int32_t> cheb(ds.n_bricks());
-    check(hyd_chebyshev(ds.grid.data(), empty.data(), cheb.data()));
-    check(hyd_set_empty_grid(ctx_, ds.grid.data(), cheb.data()));
+    if (!ds.cheb_thumb.empty()) {
+      check(hyd_set_empty_grid(ctx_, ds.grid.data(), ds.cheb_thumb.data()));
+    } else {
+      std::vector<int32_t> cheb(ds.n_bricks());
+      check(hyd_chebyshev(ds.grid.data(), empty.data(), cheb.data()));
+      check(hyd_set_empty_grid(ctx_, ds.grid.data(), cheb.data()));
+    }
     make_resident(ctx_, ds, fmt, 1, bricks);
     resident_ = true;
   }
@@ -589,6 +611,116 @@ HYD_REGISTER_RENDERER(IsosurfaceRenderer, "Isosurface");
 HYD_REGISTER_RENDERER(PagingRenderer, "Paging");
 
 }  // namespace
+
+// ------------------------------------------------------ dataset package ----
+// The package this library reads (varch's own layout is not available here,
+// so its field names are kept and its binary files are plain arrays):
+//   package_meta.json  {"block_size": bs, "padding": p,
+//                       "sample_levels": [{"raw": {"x","y","z"}, "dim": {"x","y","z"},
+//                         "thumbnails": {"chebyshev": file, "mean": file},
+//                         "voxels": {"path": file, "format": "u8"|"f16"|"f32"}}, ...]}
+//   level-0 voxels     decoded voxels, x fastest (the reference decodes H.264
+//                      brick archives instead, which this library leaves out)
+//   thumbnails         dim.x * dim.y * dim.z little-endian int32 (chebyshev: 0 for a
+//                      non-empty brick, else the L-inf brick distance to the nearest
+//                      non-empty one) and float32 (mean), x fastest
+// Relative file names resolve against the package directory.
+namespace {
+std::string resolve(const std::string &root, const std::string &name) {
+  if (!name.empty() && name[0] == '/') return name;
+  return root.empty() ? name : root + "/" + name;
+}
+
+// The file's bytes, after checking that it holds exactly `expect` of them; with
+// load false only the check is made.
+std::vector<uint8_t> read_file(const std::string &path, size_t expect, const char *what, bool load = true) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw Error(HYD_ERR_ARG, std::string(what) + ": cannot open " + path);
+  f.seekg(0, std::ios::end);
+  const std::streamoff n = f.tellg();
+  if (n < 0 || (size_t)n != expect)
+    throw Error(HYD_ERR_ARG, std::string(what) + ": " + path + " holds " + std::to_string((long long)n) +
+                                 " bytes, expected " + std::to_string(expect));
+  if (!load) return {};
+  std::vector<uint8_t> out(expect);
+  f.seekg(0);
+  if (expect && !f.read(reinterpret_cast<char *>(out.data()), (std::streamsize)expect))
+    throw Error(HYD_ERR_ARG, std::string(what) + ": read failed for " + path);
+  return out;
+}
+
+int32_t int_field(const json::Value &v, const char *key, const char *what) {
+  const json::Value *f = v.find(key);
+  if (!f) throw Error(HYD_ERR_ARG, std::string(what) + ": missing \"" + key + "\"");
+  const double d = f->as_number(what);
+  if (d != std::floor(d) || d < -2147483648.0 || d > 2147483647.0)
+    throw Error(HYD_ERR_ARG, std::string(what) + "." + key + ": expected an integer");
+  return (int32_t)d;
+}
+
+std::array<int32_t, 3> xyz_field(const json::Value &lvl, const char *key) {
+  const json::Value *v = lvl.find(key);
+  if (!v || !v->is_object()) throw Error(HYD_ERR_ARG, std::string("sample_levels[0].") + key + ": expected {x, y, z}");
+  return {int_field(*v, "x", key), int_field(*v, "y", key), int_field(*v, "z", key)};
+}
+}  // namespace
+
+std::shared_ptr<Dataset> read_package(const std::string &root, bool load_voxels) {
+  const std::string meta_path = resolve(root, "package_meta.json");
+  std::ifstream mf(meta_path);
+  if (!mf) throw Error(HYD_ERR_ARG, "dataset package: cannot open " + meta_path);
+  std::stringstream ss;
+  ss << mf.rdbuf();
+  const json::Value meta = json::parse(ss.str());
+  auto d = std::make_shared<Dataset>();
+  d->block_size = int_field(meta, "block_size", "package_meta");
+  d->apron = int_field(meta, "padding", "package_meta");
+  if (d->block_size < 1 || d->apron < 0 || d->apron > d->block_size || d->block_size + 2 * d->apron > 1024)
+    throw Error(HYD_ERR_ARG, "package_meta: invalid block_size / padding");
+  const json::Value *levels = meta.find("sample_levels");
+  if (!levels || !levels->is_array() || levels->items("sample_levels").empty())
+    throw Error(HYD_ERR_ARG, "package_meta: sample_levels must be a non-empty array");
+  const json::Value &lvl = levels->items("sample_levels")[0];
+  const std::array<int32_t, 3> raw = xyz_field(lvl, "raw"), dim = xyz_field(lvl, "dim");
+  for (int i = 0; i < 3; ++i) {
+    if (raw[i] < 1) throw Error(HYD_ERR_ARG, "sample_levels[0].raw must be positive");
+    if (dim[i] != (raw[i] + d->block_size - 1) / d->block_size)
+      throw Error(HYD_ERR_ARG, "sample_levels[0].dim does not match raw / block_size");
+  }
+  d->dims = raw;
+  d->grid = dim;
+  const json::Value *vox = lvl.find("voxels");
+  if (!vox || !vox->is_object()) throw Error(HYD_ERR_ARG, "sample_levels[0].voxels: expected {path, format}");
+  const json::Value *fmt = vox->find("format");
+  const std::string fs = fmt ? fmt->as_string("voxels.format") : "";
+  if (fs == "u8") d->fmt = HYD_FMT_U8;
+  else if (fs == "f16") d->fmt = HYD_FMT_F16;
+  else if (fs == "f32") d->fmt = HYD_FMT_F32;
+  else throw Error(HYD_ERR_ARG, "sample_levels[0].voxels.format must be u8, f16 or f32");
+  const json::Value *vpath = vox->find("path");
+  if (!vpath) throw Error(HYD_ERR_ARG, "sample_levels[0].voxels: missing \"path\"");
+  const size_t nvox = (size_t)raw[0] * raw[1] * raw[2];
+  const size_t esz = d->fmt == HYD_FMT_U8 ? 1 : (d->fmt == HYD_FMT_F16 ? 2 : 4);
+  d->voxels = read_file(resolve(root, vpath->as_string("voxels.path")), nvox * esz, "voxels", load_voxels);
+  const size_t nb = d->n_bricks();
+  const json::Value *th = lvl.find("thumbnails");
+  if (!th || !th->is_object()) throw Error(HYD_ERR_ARG, "sample_levels[0].thumbnails: expected an object");
+  const json::Value *cheb = th->find("chebyshev");
+  if (!cheb) throw Error(HYD_ERR_ARG, "sample_levels[0].thumbnails: missing \"chebyshev\"");
+  const std::vector<uint8_t> cb = read_file(resolve(root, cheb->as_string("thumbnails.chebyshev")), nb * 4,
+                                            "chebyshev thumbnail");
+  d->cheb_thumb.resize(nb);
+  std::memcpy(d->cheb_thumb.data(), cb.data(), nb * 4);
+  for (int32_t c : d->cheb_thumb)
+    if (c < 0) throw Error(HYD_ERR_ARG, "chebyshev thumbnail holds a negative distance");
+  if (const json::Value *mean = th->find("mean")) {
+    const std::vector<uint8_t> mb = read_file(resolve(root, mean->as_string("thumbnails.mean")), nb * 4,
+                                              "mean thumbnail");
+    d->mean_thumb.resize(nb);
+    std::memcpy(d->mean_thumb.data(), mb.data(), nb * 4);
+  }
+  return d;
+}
 }  // namespace front
 }  // namespace hyd
 
@@ -656,6 +788,59 @@ int32_t hyd_renderer_create(hyd_ctx *ctx, const hyd_dataset *ds, const char *con
     const size_t n = (size_t)ds->dims[0] * ds->dims[1] * ds->dims[2];
     d->voxels.assign(static_cast<const uint8_t *>(ds->voxels), static_cast<const uint8_t *>(ds->voxels) + n * esz);
     for (int i = 0; i < 3; ++i) d->grid[i] = (ds->dims[i] + ds->block_size - 1) / ds->block_size;
+    d->bmin.resize(d->n_bricks());
+    d->bmax.resize(d->n_bricks());
+    d->bmean.resize(d->n_bricks());
+    hyd::front::check(hyd_brick_stats(d->voxels.data(), d->fmt, d->dims.data(), d->block_size, d->apron,
+                                      d->bmin.data(), d->bmax.data(), d->bmean.data()));
+    const hyd::json::Value cfg = hyd::json::parse(config_json);
+    r->impl = hyd::front::RendererFactory(d).create(ctx, hyd::front::parse_renderer_config(cfg));
+  });
+  if (st != HYD_OK) return st;
+  *out = r.release();
+  return HYD_OK;
+}
+
+int32_t hyd_package_read_info(const char *dataset_path, hyd_package_info *out) {
+  HYD_REQUIRE(dataset_path && out, HYD_ERR_ARG, "null argument");
+  return guarded([&] {
+    const auto d = hyd::front::read_package(dataset_path, false);
+    hyd_package_info info{};
+    for (int i = 0; i < 3; ++i) {
+      info.raw[i] = d->dims[i];
+      info.dim[i] = d->grid[i];
+    }
+    info.block_size = d->block_size;
+    info.padding = d->apron;
+    info.fmt = d->fmt;
+    info.has_mean = d->mean_thumb.empty() ? 0 : 1;
+    *out = info;
+  });
+}
+
+int32_t hyd_package_read_thumbnails(const char *dataset_path, int32_t *chebyshev, float *mean, int64_t n) {
+  HYD_REQUIRE(dataset_path && chebyshev && n >= 0, HYD_ERR_ARG, "null argument");
+  return guarded([&] {
+    const auto d = hyd::front::read_package(dataset_path, false);
+    if ((int64_t)d->n_bricks() != n)
+      throw hyd::front::Error(HYD_ERR_ARG, "thumbnail buffers hold " + std::to_string((long long)n) +
+                                               " entries, the package has " + std::to_string(d->n_bricks()));
+    std::memcpy(chebyshev, d->cheb_thumb.data(), d->n_bricks() * 4);
+    if (mean) {
+      if (d->mean_thumb.empty()) throw hyd::front::Error(HYD_ERR_ARG, "the package has no \"mean\" thumbnail");
+      std::memcpy(mean, d->mean_thumb.data(), d->n_bricks() * 4);
+    }
+  });
+}
+
+int32_t hyd_renderer_create_package(hyd_ctx *ctx, const char *dataset_path, const char *config_json,
+                                    hyd_renderer **out) {
+  HYD_REQUIRE(ctx && dataset_path && config_json && out, HYD_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::unique_ptr<hyd_renderer> r(new hyd_renderer);
+  r->ctx = ctx;
+  const int32_t st = guarded([&] {
+    const auto d = hyd::front::read_package(dataset_path, true);
     d->bmin.resize(d->n_bricks());
     d->bmax.resize(d->n_bricks());
     d->bmean.resize(d->n_bricks());

@@ -143,6 +143,71 @@ def chebyshev(empty: np.ndarray) -> np.ndarray:
     return out
 
 
+# ------------------------------------------------------- dataset package ----
+_PKG_FORMATS = {np.dtype(np.uint8): "u8", np.dtype(np.float16): "f16", np.dtype(np.uint16): "f16",
+                np.dtype(np.float32): "f32"}
+
+
+def write_package(root, vol: np.ndarray, block_size: int, padding: int = 1, empty: np.ndarray | None = None) -> dict:
+    """Write a dataset package that hyd_renderer_create_package reads (the layout is in
+    include/hydrant_renderer.h): package_meta.json, the level-0 voxels and the chebyshev
+    and mean thumbnails of sample_levels[0] (the files build_dataset.sh:79-124 makes with
+    varch's tools).  A brick is empty when its apron-inclusive maximum is <= 0, the same
+    for every transfer function, unless ``empty`` (bool, (dim.z, dim.y, dim.x)) says
+    otherwise; the mean is that of its normalised interior voxels.  Returns the meta
+    dictionary."""
+    import json
+    from pathlib import Path
+
+    vol = np.ascontiguousarray(vol)
+    if vol.ndim != 3 or vol.dtype not in _PKG_FORMATS:
+        raise TypeError("vol must be a 3-D uint8, float16 (or uint16 half bits) or float32 array")
+    if vol.dtype == np.float16:
+        vol = vol.view(np.uint16)
+    root = Path(root)
+    root.mkdir(parents=True, exist_ok=True)
+    grid, _, bmax, bmean = brick_stats(vol, block_size, padding)
+    if empty is None:
+        empty = bmax <= 0
+    elif np.shape(empty) != bmax.shape:
+        raise ValueError(f"empty must have shape {bmax.shape}")
+    cheb = chebyshev(np.asarray(empty, np.uint8))
+    raw = vol.shape[::-1]
+    meta = {
+        "block_size": int(block_size),
+        "padding": int(padding),
+        "sample_levels": [{
+            "raw": {"x": int(raw[0]), "y": int(raw[1]), "z": int(raw[2])},
+            "dim": {"x": int(grid[0]), "y": int(grid[1]), "z": int(grid[2])},
+            "thumbnails": {"chebyshev": "thumb_chebyshev.i32", "mean": "thumb_mean.f32"},
+            "voxels": {"path": "voxels.raw", "format": _PKG_FORMATS[vol.dtype]},
+        }],
+    }
+    (root / "voxels.raw").write_bytes(vol.astype(vol.dtype.newbyteorder("<"), copy=False).tobytes())
+    (root / "thumb_chebyshev.i32").write_bytes(cheb.astype("<i4").tobytes())
+    (root / "thumb_mean.f32").write_bytes(bmean.astype("<f4").tobytes())
+    (root / "package_meta.json").write_text(json.dumps(meta, indent=1) + "\n")
+    return meta
+
+
+def read_package_info(root) -> L.PackageInfo:
+    """hyd_package_read_info: the package's sizes, after validating meta and thumbnails."""
+    info = L.PackageInfo()
+    L.call("hyd_package_read_info", str(root).encode(), C.byref(info))
+    return info
+
+
+def read_package_thumbnails(root) -> tuple[np.ndarray, np.ndarray | None]:
+    """(chebyshev int32, mean float32 or None), each shaped (dim.z, dim.y, dim.x)."""
+    info = read_package_info(root)
+    shape = (info.dim[2], info.dim[1], info.dim[0])
+    cheb = np.empty(shape, np.int32)
+    mean = np.empty(shape, np.float32) if info.has_mean else None
+    L.call("hyd_package_read_thumbnails", str(root).encode(), L.ptr(cheb, C.c_int32),
+           L.ptr(mean, C.c_float) if mean is not None else None, cheb.size)
+    return cheb, mean
+
+
 def pack(vol: np.ndarray, bs: int, apron: int, fmt_out: int, bricks: np.ndarray | None = None,
          normalize: bool = True) -> np.ndarray:
     vol = np.ascontiguousarray(vol)

@@ -428,6 +428,17 @@ class Batch:
                int(reps), C.byref(best), C.byref(ms))
         return best, ms.value
 
+    def autotune_schedule(self, density: float, max_error: float, out, reps: int = 2):
+        """Launch shape and early-termination threshold: the fastest within max_error."""
+        check_output(out, 4 * self.n_pixels, "batched output")
+        self.ctx.sync_stream()
+        best = L.BatchConfig()
+        ert = C.c_float(0)
+        ms = C.c_float(0)
+        L.call("hyd_batch_autotune_schedule", self.ctx.h, self.h, float(density), float(max_error),
+               C.c_void_p(out.data_ptr()), int(reps), C.byref(best), C.byref(ert), C.byref(ms))
+        return best, ert.value, ms.value
+
     def view_image(self, out, i: int):
         off, w, h = self.offsets[i]
         return out[off:off + w * h].view(h, w, 4)

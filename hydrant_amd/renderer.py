@@ -47,7 +47,25 @@ def parse_camera(cfg) -> L.Camera:
 
 
 class Renderer:
-    """One renderer over a raw volume (numpy (z, y, x) of uint8 / float16 / float32)."""
+    """One renderer over a raw volume (numpy (z, y, x) of uint8 / float16 / float32),
+    or over a dataset package on disk (``Renderer.from_package``)."""
+
+    @classmethod
+    def from_package(cls, ctx, dataset_path, config) -> "Renderer":
+        """RendererFactory(dataset_path).create(config): package_meta.json, its level-0
+        voxels and thumbnails (hydrant_amd.dataset.write_package writes one)."""
+        self = cls.__new__(cls)
+        self._vol = None
+        h = C.c_void_p()
+        L.call("hyd_renderer_create_package", ctx.h, str(dataset_path).encode(), _text(config), C.byref(h))
+        self.h = h
+        self._resolution()
+        return self
+
+    def _resolution(self) -> None:
+        w, hh = C.c_int32(0), C.c_int32(0)
+        L.call("hyd_renderer_resolution", self.h, C.byref(w), C.byref(hh))
+        self.W, self.H = w.value, hh.value
 
     def __init__(self, ctx, volume: np.ndarray, config, block_size: int = 64, apron: int = 1):
         vol = np.ascontiguousarray(volume)
@@ -62,9 +80,7 @@ class Renderer:
         h = C.c_void_p()
         L.call("hyd_renderer_create", ctx.h, C.byref(d), _text(config), C.byref(h))
         self.h = h
-        w, hh = C.c_int32(0), C.c_int32(0)
-        L.call("hyd_renderer_resolution", self.h, C.byref(w), C.byref(hh))
-        self.W, self.H = w.value, hh.value
+        self._resolution()
 
     def update(self, params) -> None:
         L.call("hyd_renderer_update", self.h, _text(params))
@@ -75,7 +91,7 @@ class Renderer:
         return out
 
     def close(self) -> None:
-        if self.h:
+        if getattr(self, "h", None):
             L.call("hyd_renderer_destroy", self.h)
             self.h = None
 

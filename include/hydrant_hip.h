@@ -367,6 +367,14 @@ int32_t hyd_batch_replay(hyd_ctx *ctx, hyd_batch *batch);
  * launches each and return the fastest; writes the output buffer. */
 int32_t hyd_batch_autotune(hyd_ctx *ctx, hyd_batch *batch, float density, float ert, float *rgba_dev, int32_t reps,
                            hyd_batch_config *best, float *best_ms);
+/* Also tune the step schedule: the early-termination threshold, over
+ * {0.99, 0.995, 0.999, 0.9995, 0.9999, off}.  Terminating once alpha exceeds ert
+ * drops at most 1 - ert of any channel, so only thresholds with
+ * 1 - ert <= max_error / 2 are timed (the other half of the budget is left to
+ * storage and accumulation rounding); returns the fastest launch shape and
+ * threshold (0 = no termination). */
+int32_t hyd_batch_autotune_schedule(hyd_ctx *ctx, hyd_batch *batch, float density, float max_error, float *rgba_dev,
+                                    int32_t reps, hyd_batch_config *best, float *best_ert, float *best_ms);
 
 /* -------------------------------------------- differentiable form ----- */
 /* A differentiable variant of the Volume march (no counterpart in the

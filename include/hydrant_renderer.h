@@ -63,6 +63,44 @@ typedef struct {
   int32_t apron;         /* padding voxels per side (e.g. 1) */
 } hyd_dataset;
 
+/* A dataset package on disk (the reference's RendererFactory(dataset_path),
+ * core/renderer.hpp:53-59, reading package_meta.json and the level-0
+ * thumbnails its renderers load, volume.cc:47-51, blocks.cc:35-44,
+ * isosurface.cc:50-53):
+ *   package_meta.json  {"block_size": bs, "padding": p, "sample_levels": [{
+ *                        "raw": {x, y, z}, "dim": {x, y, z},
+ *                        "thumbnails": {"chebyshev": file, "mean": file},
+ *                        "voxels": {"path": file, "format": "u8"|"f16"|"f32"}}]}
+ *   voxels             raw level-0 voxels, x fastest (the reference's package
+ *                      holds H.264 brick archives that it decodes; decode is
+ *                      outside this library)
+ *   chebyshev          dim.x*dim.y*dim.z little-endian int32, x fastest: 0 for a
+ *                      brick with data, else the L-inf distance in bricks to the
+ *                      nearest such brick
+ *   mean               the same count of little-endian float32 (Blocks mode)
+ * File names resolve against the package directory.  dim must equal
+ * ceil(raw / block_size); every file must hold exactly the bytes named. */
+typedef struct {
+  int32_t raw[3];        /* voxels (x, y, z) */
+  int32_t dim[3];        /* bricks (x, y, z) */
+  int32_t block_size;
+  int32_t padding;
+  int32_t fmt;           /* HYD_FMT_* of the voxel file */
+  int32_t has_mean;      /* 1 if the package has a "mean" thumbnail */
+} hyd_package_info;
+
+/* Read and validate package_meta.json and the thumbnails (not the voxels). */
+int32_t hyd_package_read_info(const char *dataset_path, hyd_package_info *out);
+/* Copy the chebyshev (and, when mean is not NULL, mean) thumbnails; n must be
+ * dim.x*dim.y*dim.z. */
+int32_t hyd_package_read_thumbnails(const char *dataset_path, int32_t *chebyshev, float *mean, int64_t n);
+/* RendererFactory(dataset_path).create + IRenderer::init: Volume, Blocks and
+ * Isosurface skip the bricks the chebyshev thumbnail marks empty (Blocks shades
+ * with the mean thumbnail), as the reference's renderers do, instead of deriving
+ * emptiness from the transfer function or iso value. */
+int32_t hyd_renderer_create_package(hyd_ctx *ctx, const char *dataset_path, const char *config_json,
+                                    hyd_renderer **out);
+
 /* Registered renderer names, comma separated, into buf (cap bytes, NUL
  * terminated); *len receives the full length. */
 int32_t hyd_renderer_candidates(char *buf, int32_t cap, int32_t *len);

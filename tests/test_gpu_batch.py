@@ -349,3 +349,24 @@ def test_pool_page_writes_refresh_oriented_layouts(monkeypatch, switch):
     b2.close()
     ctx.close()
     ctx2.close()
+
+
+def test_autotune_schedule_within_budget(gpu_ctx):
+    """The step-schedule autotuner times only termination thresholds whose truncation
+    (1 - ert) fits half the error budget, and the chosen schedule's image is within the
+    budget of the FP64 oracle run without termination."""
+    wl = _small(L.FMT_F16, n_views=4)
+    b = _setup(gpu_ctx, wl)
+    out = b.new_output()
+    for budget in (1e-3, 2e-2):
+        cfg, ert, ms = b.autotune_schedule(wl.density, budget, out, reps=1)
+        assert ert == 0.0 or 1.0 - ert <= budget / 2 + 1e-7, (budget, ert)
+        assert ms > 0.0
+        out, _ = b.render(cfg, wl.density, ert, out=out)
+        gpu_ctx.synchronize()
+        wl.ert = 0.0
+        errs = _view_errors(wl, out, b, 64, exact=True)
+        print(f"budget {budget}: ert {ert}, config {cfg.as_dict()}, max-abs vs FP64 (no termination) "
+              f"{max(errs):.3g}", flush=True)
+        assert max(errs) <= budget
+    b.close()
