@@ -43,7 +43,9 @@ int32_t nck_of(const hyd_diff_params *p) { return (p->n_steps + p->ckpt_every - 
 
 int32_t check_params(const hyd_diff_params *p) {
   HYD_REQUIRE(p, HYD_ERR_ARG, "null diff params");
-  HYD_REQUIRE(p->dims[0] > 0 && p->dims[1] > 0 && p->dims[2] > 0, HYD_ERR_ARG, "volume dims must be > 0");
+  HYD_REQUIRE(p->dims[0] >= 2 && p->dims[1] > 0 && p->dims[2] > 0, HYD_ERR_ARG,
+              "volume dims must be > 0, and >= 2 along x (taps load x-adjacent pairs)");
+  HYD_REQUIRE(p->dims[0] < (1 << 22), HYD_ERR_RANGE, "volume rows of %d voxels exceed 2^22", p->dims[0]);
   HYD_REQUIRE(p->n_steps > 0 && p->ckpt_every > 0, HYD_ERR_ARG, "n_steps and ckpt_every must be > 0");
   HYD_REQUIRE(p->sigma >= 0.f, HYD_ERR_ARG, "sigma must be >= 0");
   HYD_REQUIRE((int64_t)p->dims[0] * p->dims[1] * p->dims[2] * 4 <= 2147483647ll, HYD_ERR_RANGE,
@@ -140,8 +142,9 @@ int32_t hyd_diff_backward(hyd_ctx *c, const hyd_diff_params *p, const hyd_view *
   const size_t rays_off = (ds_floats * sizeof(float) + 255) & ~(size_t)255;
   a.view_groups = diff_view_groups(a.nx, a.ny, a.nz, n_views);
   const size_t blocks_floats = (size_t)a.view_groups * diff_bricks(a.nx, a.ny, a.nz) * kDiffBlock;
-  const size_t need =
-      rays_off + sizeof(float) * (8 * (size_t)n_rays + blocks_floats + (size_t)n_rays + kDiffBoundParts) + 64;
+  const size_t gtf_off = (rays_off + sizeof(float) * (8 * (size_t)n_rays + blocks_floats + (size_t)n_rays +
+                                                     kDiffBoundParts) + 255) & ~(size_t)255;
+  const size_t need = gtf_off + sizeof(unsigned long long) * 4 * 256 + 64;
   if (c->diff_scratch_cap < need) {
     if (c->own_diff_scratch) (void)hipFree(c->own_diff_scratch);
     c->own_diff_scratch = nullptr;
@@ -155,10 +158,15 @@ int32_t hyd_diff_backward(hyd_ctx *c, const hyd_diff_params *p, const hyd_view *
   a.blocks = reinterpret_cast<float *>(a.rays + 2 * n_rays);
   a.dsmax = a.blocks + blocks_floats;
   a.bound = a.dsmax + n_rays;
-  a.tile_counter = reinterpret_cast<int32_t *>(a.bound + kDiffBoundParts);
+  a.gtf_acc = reinterpret_cast<unsigned long long *>(base + gtf_off);
+  a.tile_counter = reinterpret_cast<int32_t *>(a.gtf_acc + 4 * 256);
+  a.dsbound = reinterpret_cast<unsigned int *>(a.tile_counter + 1);
   // fixed-point headroom: the int64 accumulators hold 2^33 terms of up to 2^30 each
   const double terms = 2.0 * (double)n_rays * (double)a.K;
   a.headroom = terms > 8589934592.0 ? (int32_t)std::ceil(std::log2(terms / 8589934592.0)) : 0;
+  // a block slot gets at most one term per sample of the launch
+  const double vterms = (double)n_rays * (double)a.K;
+  a.vheadroom = vterms > 8589934592.0 ? (int32_t)std::ceil(std::log2(vterms / 8589934592.0)) : 0;
   a.n_workers = c->n_cus > 0 ? c->n_cus * 5 : 1280;   // 4-wave workgroups at 5 waves per SIMD
   HYD_CHECK_HIP(launch_diff_backward(a, c->stream));
   return HYD_OK;

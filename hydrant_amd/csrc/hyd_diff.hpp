@@ -36,9 +36,12 @@ struct DiffArgs {
   float4 *rays;                // [view][y][x][2] setup of active rays: (dx, dy, dz, tnear), (dt, 0, 0, 0)
   float *blocks;               // [view group][brick][10^3] voxel-gradient blocks (brick + one-voxel halo)
   int32_t view_groups;         // scatter workgroups per brick (views split into contiguous groups)
-  float *bound;                // [1] bound of one sample's TF-gradient term
+  float *bound;                // [kDiffBoundParts] partial bounds of one sample's TF-gradient term
+  unsigned long long *gtf_acc; // [4][256] fixed-point TF-gradient sums of the launch
   int32_t *tile_counter;       // [1] work queue of the ray-order pass
-  int32_t headroom;            // extra right shift of the fixed-point scales for very large launches
+  int32_t headroom;            // extra right shift of the TF fixed-point scale for very large launches
+  int32_t vheadroom;           // the same for the voxel-gradient blocks
+  unsigned int *dsbound;       // [1] max |dL/ds| of the launch (float bits; non-negative floats order as uints)
   int32_t n_workers;           // workgroups of the ray-order pass
 };
 

@@ -18,8 +18,11 @@
 // flagged segment recomputes T_k forward from its checkpoint instead.
 // With the suffix colour S and suffix transmittance P (back to front):
 //   dL/dalpha_k = T_k (g . (c_k - S_{k+1}) + gA * P_{k+1}),  dL/dc_k = T_k alpha_k g.
-// TF gradients accumulate in an LDS copy per workgroup (flushed with global
-// atomics); voxel gradients use hardware float atomics.
+// TF gradients: int64 fixed-point sums (LDS per workgroup, then the launch's
+// global sums), converted to FP32 once.  Voxel gradients: dL/ds of every
+// sample is stored; a brick-order pass adds the corner terms of each brick's
+// samples in LDS, one wave per brick block, and a final pass sums the blocks
+// per voxel in a fixed order.  Both gradients are bit-identical from run to run.
 #include "hyd_diff.hpp"
 
 namespace hyd {
@@ -43,66 +46,63 @@ __device__ __forceinline__ VolumeRef volume_ref(const DiffArgs &a) {
   return VolumeRef{reinterpret_cast<const char *>(a.vol)};
 }
 
-__device__ __forceinline__ float vol_at(const VolumeRef &v, int byte_offset) {
-  return *reinterpret_cast<const float *>(v.base + (unsigned)byte_offset);
+// Two x-adjacent voxels from one 8-byte load at 4-byte alignment.
+__device__ __forceinline__ float2 vol_pair(const VolumeRef &v, unsigned byte_offset) {
+  float2 r;
+  __builtin_memcpy(&r, v.base + byte_offset, sizeof(r));
+  return r;
 }
 
-__device__ __forceinline__ void sample(const DiffArgs &a, const VolumeRef &vr, float px, float py, float pz,
-                                       Smp &sm) {
-  const float fx = float(a.nx) * px - 0.5f, fy = float(a.ny) * py - 0.5f, fz = float(a.nz) * pz - 0.5f;
-  const float flx = floorf(fx), fly = floorf(fy), flz = floorf(fz);
-  const float gx = fx - flx, gy = fy - fly, gz = fz - flz;
-  const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-  const int sx = a.nx * 4, sxy = a.nx * a.ny * 4;   // byte strides
-  const int x0 = iclamp(ix, 0, a.nx - 1) * 4, x1 = iclamp(ix + 1, 0, a.nx - 1) * 4;
-  const int y0 = iclamp(iy, 0, a.ny - 1) * sx, y1 = iclamp(iy + 1, 0, a.ny - 1) * sx;
-  const int z0 = iclamp(iz, 0, a.nz - 1) * sxy, z1 = iclamp(iz + 1, 0, a.nz - 1) * sxy;
-  const int r00 = z0 + y0, r01 = z0 + y1, r10 = z1 + y0, r11 = z1 + y1;
-  const int id[8] = {r00 + x0, r00 + x1, r01 + x0, r01 + x1, r10 + x0, r10 + x1, r11 + x0, r11 + x1};
-  const float hx = 1.f - gx, hy = 1.f - gy, hz = 1.f - gz;
-  const float w[8] = {hx * hy * hz, gx * hy * hz, hx * gy * hz, gx * gy * hz,
-                      hx * hy * gz, gx * hy * gz, hx * gy * gz, gx * gy * gz};
-  float s = 0.f;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) s = fmaf(w[q], vol_at(vr, id[q]), s);
-  sm.s = s;
-}
-
-// Taps of the last sample of sample_reuse: its first and last tap offsets
-// (which fix all eight) and their values.
+// The taps of a sample's cell: the four x-pairs of rows (y0, z0), (y1, z0),
+// (y0, z1), (y1, z1), loaded at x = clamp(ix, 0, nx - 2) (nx >= 2), and the
+// cell they belong to.
 struct TapCache {
-  int id0 = -1, id7 = -1;
-  float v[8];
+  int ix = -0x40000000, iy = 0, iz = 0;
+  float2 p[4];
 };
 
-// sample() for consecutive samples of one ray: a sample whose eight taps are
-// the previous sample's (same first and last offset) loads nothing.  The
-// weights and the sum are the same operations, so the value is bit-identical.
-__device__ __forceinline__ void sample_reuse(const DiffArgs &a, const VolumeRef &vr, float px, float py, float pz,
-                                             Smp &sm, TapCache &tc) {
+// s = trilinear(volume, n p - 0.5) with clamped taps.  Rows clamp through
+// their offsets; along x the pair at clamp(ix, 0, nx - 2) holds both taps, and
+// a cell below 0 (at or above nx - 1) takes the pair's first (second) voxel
+// alone (x weight 0 or 1).  Linear interpolations along x, then y, then z.
+// With REUSE a sample in the previous sample's cell loads nothing.
+template <bool REUSE>
+__device__ __forceinline__ void sample_taps(const DiffArgs &a, const VolumeRef &vr, float px, float py, float pz,
+                                            Smp &sm, TapCache &tc) {
   const float fx = float(a.nx) * px - 0.5f, fy = float(a.ny) * py - 0.5f, fz = float(a.nz) * pz - 0.5f;
   const float flx = floorf(fx), fly = floorf(fy), flz = floorf(fz);
   const float gx = fx - flx, gy = fy - fly, gz = fz - flz;
   const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-  const int sx = a.nx * 4, sxy = a.nx * a.ny * 4;   // byte strides
-  const int x0 = iclamp(ix, 0, a.nx - 1) * 4, x1 = iclamp(ix + 1, 0, a.nx - 1) * 4;
-  const int y0 = iclamp(iy, 0, a.ny - 1) * sx, y1 = iclamp(iy + 1, 0, a.ny - 1) * sx;
-  const int z0 = iclamp(iz, 0, a.nz - 1) * sxy, z1 = iclamp(iz + 1, 0, a.nz - 1) * sxy;
-  const int r00 = z0 + y0, r01 = z0 + y1, r10 = z1 + y0, r11 = z1 + y1;
-  const int id[8] = {r00 + x0, r00 + x1, r01 + x0, r01 + x1, r10 + x0, r10 + x1, r11 + x0, r11 + x1};
-  if (id[0] != tc.id0 || id[7] != tc.id7) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) tc.v[q] = vol_at(vr, id[q]);
-    tc.id0 = id[0];
-    tc.id7 = id[7];
+  if (!REUSE || ix != tc.ix || iy != tc.iy || iz != tc.iz) {
+    const unsigned sx = (unsigned)a.nx * 4u, sxy = (unsigned)a.nx * (unsigned)a.ny * 4u;   // byte strides
+    const unsigned xm = (unsigned)iclamp(ix, 0, a.nx - 2) * 4u;
+    const unsigned cy0 = (unsigned)iclamp(iy, 0, a.ny - 1), cy1 = (unsigned)iclamp(iy + 1, 0, a.ny - 1);
+    const unsigned cz0 = (unsigned)iclamp(iz, 0, a.nz - 1), cz1 = (unsigned)iclamp(iz + 1, 0, a.nz - 1);
+    const unsigned y0 = __umul24(cy0, sx) + xm, y1 = __umul24(cy1, sx) + xm;   // 4 nx < 2^24 (host check)
+    const unsigned z0 = cz0 * sxy, z1 = cz1 == cz0 ? z0 : z0 + sxy;
+    tc.p[0] = vol_pair(vr, z0 + y0);
+    tc.p[1] = vol_pair(vr, z0 + y1);
+    tc.p[2] = vol_pair(vr, z1 + y0);
+    tc.p[3] = vol_pair(vr, z1 + y1);
+    tc.ix = ix;
+    tc.iy = iy;
+    tc.iz = iz;
   }
-  const float hx = 1.f - gx, hy = 1.f - gy, hz = 1.f - gz;
-  const float w[8] = {hx * hy * hz, gx * hy * hz, hx * gy * hz, gx * gy * hz,
-                      hx * hy * gz, gx * hy * gz, hx * gy * gz, gx * gy * gz};
-  float s = 0.f;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) s = fmaf(w[q], tc.v[q], s);
-  sm.s = s;
+  const float wx = ix < 0 ? 0.f : (ix >= a.nx - 1 ? 1.f : gx);
+  const float c00 = fmaf(wx, tc.p[0].y - tc.p[0].x, tc.p[0].x), c10 = fmaf(wx, tc.p[1].y - tc.p[1].x, tc.p[1].x);
+  const float c01 = fmaf(wx, tc.p[2].y - tc.p[2].x, tc.p[2].x), c11 = fmaf(wx, tc.p[3].y - tc.p[3].x, tc.p[3].x);
+  const float c0 = fmaf(gy, c10 - c00, c00), c1 = fmaf(gy, c11 - c01, c01);
+  sm.s = fmaf(gz, c1 - c0, c0);
+}
+
+__device__ __forceinline__ void sample(const DiffArgs &a, const VolumeRef &vr, float px, float py, float pz, Smp &sm) {
+  TapCache tc;
+  sample_taps<false>(a, vr, px, py, pz, sm, tc);
+}
+
+__device__ __forceinline__ void sample_reuse(const DiffArgs &a, const VolumeRef &vr, float px, float py, float pz,
+                                             Smp &sm, TapCache &tc) {
+  sample_taps<true>(a, vr, px, py, pz, sm, tc);
 }
 
 struct Shade {
@@ -111,7 +111,9 @@ struct Shade {
   bool inside;
 };
 
-__device__ __forceinline__ void shade(const float4 *tf, int n, float sigma, float dt, float s, Shade &sh) {
+// (c, a) = TF(s) (Border + Linear lookup) and alpha = 1 - exp(-sigma a dt),
+// computed as 1 - 2^(nl a) with nl = -sigma dt log2(e), the ray's constant.
+__device__ __forceinline__ void shade(const float4 *tf, int n, float nl, float s, Shade &sh) {
   sh.inside = (0.f <= s) & (s <= 1.f);
   float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
   sh.j0 = sh.j1 = 0;
@@ -124,13 +126,16 @@ __device__ __forceinline__ void shade(const float4 *tf, int n, float sigma, floa
     sh.j0 = iclamp(j, 0, n - 1);
     sh.j1 = iclamp(j + 1, 0, n - 1);
     const float4 p = tf[sh.j0], q = tf[sh.j1];
-    const float h = 1.f - sh.g;
-    v = make_float4(p.x * h + q.x * sh.g, p.y * h + q.y * sh.g, p.z * h + q.z * sh.g, p.w * h + q.w * sh.g);
+    v = make_float4(fmaf(sh.g, q.x - p.x, p.x), fmaf(sh.g, q.y - p.y, p.y), fmaf(sh.g, q.z - p.z, p.z),
+                    fmaf(sh.g, q.w - p.w, p.w));
   }
   sh.c[0] = v.x; sh.c[1] = v.y; sh.c[2] = v.z;
   sh.a = v.w;
-  sh.alpha = 1.f - expf(-sigma * fmaxf(v.w, 0.f) * dt);
+  sh.alpha = 1.f - __builtin_amdgcn_exp2f(nl * fmaxf(v.w, 0.f));
 }
+
+// nl = -sigma dt log2(e), so that alpha = 1 - 2^(nl a)
+__device__ __forceinline__ float opacity_const(float sigma, float dt) { return -(sigma * dt) * 1.44269504f; }
 
 struct RaySetup {
   float ox, oy, oz, dx, dy, dz, tnear, dt;
@@ -176,6 +181,7 @@ __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs a) {
     float C0 = 0.f, C1 = 0.f, C2 = 0.f, T = 1.f;
     const int nseg = a.nck - 1;
     if (setup(a, a.views[view], x, y, r)) {
+      const float nl = opacity_const(a.sigma, r.dt);
       TapCache tcf;   // consecutive samples in one cell share their taps
       for (int seg = 0; seg < nseg; ++seg) {
         const float Tstart = T;
@@ -187,7 +193,7 @@ __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs a) {
           Smp sm;
           sample_reuse(a, vr, px, py, pz, sm, tcf);
           Shade sh;
-          shade(s_tf, a.tf_n, a.sigma, r.dt, sm.s, sh);
+          shade(s_tf, a.tf_n, nl, sm.s, sh);
           const float ta = T * sh.alpha;
           C0 = fmaf(ta, sh.c[0], C0);
           C1 = fmaf(ta, sh.c[1], C1);
@@ -235,13 +241,6 @@ __device__ __forceinline__ unsigned long long fixed_term(float v, float scale) {
   return (unsigned long long)(long long)(int)rintf(v * scale);
 }
 
-// A term already scaled (|v| < 2^30) to fixed point by truncation: one
-// conversion; the bias (under one unit of 2^-30 of the bound per term) is far
-// below the FP32 rounding of the final sums.
-__device__ __forceinline__ unsigned long long fixed_trunc(float v) {
-  return (unsigned long long)(long long)(int)v;
-}
-
 // Scale 2^e with b 2^e < 2^30 (frexp: b < 2^E), less the headroom, in [2^-100, 2^100].
 __device__ __forceinline__ int fixed_exponent(float b, int headroom) {
   int E = 0;
@@ -249,17 +248,36 @@ __device__ __forceinline__ int fixed_exponent(float b, int headroom) {
   return min(100, max(-100, 30 - E - headroom));
 }
 
+// A term already scaled (|v| < 2^30) to fixed point by truncation: one
+// conversion; the bias (under one unit, 2^-30 of the launch's bound, per term)
+// is far below the FP32 rounding of the final sums.
+__device__ __forceinline__ unsigned long long fixed_trunc(float v) {
+  return (unsigned long long)(long long)(int)v;
+}
+
+// The TF-gradient scale exponent of the launch, from the prepass's partial
+// bounds (every wave that calls this gets the same value).
+__device__ __forceinline__ int tf_exponent(const DiffArgs &a, int lane) {
+  float b = 0.f;
+  for (int i = lane; i < kDiffBoundParts; i += 64) b = fmaxf(b, a.bound[i]);
+  b = wave_max(b);
+  return fixed_exponent(b, a.headroom + 3);   // a run sums <= kRun = 8 terms
+}
+
 // Prepass: partial maxima of the bound b >= |h d_ch| of one TF-gradient term,
 //   |d_ch| = T alpha |g_ch| <= |g_ch|              (colour channels),
 //   |d_3| <= |dalpha| sigma dt, |dalpha| <= 2 max|tf_rgb| |g|_1 + |g_A|
 // (|c|, |S| <= max|tf_rgb|, T, P <= 1, dt <= sqrt(3)/K for the unit box).  Workgroup 0
-// also resets the work queue and clears grad_tf (both outputs are overwritten).
+// also resets the work queue and clears the TF-gradient sums.
 __global__ __launch_bounds__(256) void diff_bound_kernel(DiffArgs a) {
   __shared__ float s_red[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (blockIdx.x == 0) {
-    if (tid == 0) *a.tile_counter = 0;
-    for (int i = tid; i < 4 * a.tf_n; i += 256) a.grad_tf[i] = 0.f;
+    if (tid == 0) {
+      *a.tile_counter = 0;
+      *a.dsbound = 0u;
+    }
+    for (int i = tid; i < 4 * kMaxTf; i += 256) a.gtf_acc[i] = 0ull;
   }
   float mc = 0.f;
   for (int i = tid; i < a.tf_n; i += 256) {
@@ -323,6 +341,7 @@ __device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long 
   const VolumeRef vr = volume_ref(a);
   const bool active =
       in_image && (g[0] != 0.f || g[1] != 0.f || g[2] != 0.f || gA != 0.f) && setup(a, a.views[view], x, y, r);
+  const float nl = opacity_const(a.sigma, r.dt);
   const unsigned long long amask = __ballot(active);
   if (active) {   // the scatter pass reuses the ray setup
     a.rays[2 * ray] = make_float4(r.dx, r.dy, r.dz, r.tnear);
@@ -352,7 +371,7 @@ __device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long 
           Smp sm;
           sample_reuse(a, vr, px, py, pz, sm, tc);
           Shade sh;
-          shade(tf, a.tf_n, a.sigma, r.dt, sm.s, sh);
+          shade(tf, a.tf_n, nl, sm.s, sh);
           float Tk;
           if (!exact) {
             Tk = Tnext * __builtin_amdgcn_rcpf(1.f - sh.alpha);   // alpha <= 0.5 here
@@ -364,7 +383,7 @@ __device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long 
               Smp sj;
               sample(a, vr, qx, qy, qz, sj);
               Shade hj;
-              shade(tf, a.tf_n, a.sigma, r.dt, sj.s, hj);
+              shade(tf, a.tf_n, nl, sj.s, hj);
               Tk = Tk * (1.f - hj.alpha);
             }
           }
@@ -427,26 +446,24 @@ __device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long 
     if (tn > 0) flush_tf(s_gtf, tj0, tj1, T0, T1, scale);
   }
   if (in_image) a.dsmax[ray] = dmax;
+  const float wmax = wave_max(dmax);
+  if (lane == 0 && wmax > 0.f) atomicMax(a.dsbound, __float_as_uint(wmax));
 }
 
 // Ray-order adjoint: persistent workgroups of four waves; each wave claims
 // 8x8 ray tiles from the work queue.  TF gradients: register runs, then int64
-// LDS accumulators with one scale per launch (from the prepass bound), folded
-// once per workgroup into FP32 global atomics.
+// LDS accumulators with one scale per launch (from the prepass bound), added
+// once per workgroup to the launch's int64 sums (integer additions: the sums
+// do not depend on which workgroup took which tile, or in what order).
 __global__ __launch_bounds__(256) void diff_adjoint_kernel(DiffArgs a) {
   __shared__ unsigned long long s_gtf[4 * kMaxTf];   // [channel][entry]
-  __shared__ float s_b[kBoundParts];
   __shared__ float s_stage[4][64 * kStagePitch];
   __shared__ float4 s_tfl[kMaxTf];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int i = tid; i < 4 * kMaxTf; i += 256) s_gtf[i] = 0ull;
-  for (int i = tid; i < kBoundParts; i += 256) s_b[i] = a.bound[i];
   for (int i = tid; i < a.tf_n; i += 256) s_tfl[i] = reinterpret_cast<const float4 *>(a.tf)[i];
+  const int e = tf_exponent(a, lane);
   __syncthreads();
-  float b = 0.f;
-  for (int i = lane; i < kBoundParts; i += 64) b = fmaxf(b, s_b[i]);
-  b = wave_max(b);
-  const int e = fixed_exponent(b, a.headroom + 3);   // a run sums <= kRun = 8 terms
   const float scale = ldexpf(1.f, e);
   const int tiles_x = (a.W + kTile - 1) / kTile, tiles_y = (a.H + kTile - 1) / kTile;
   const int per_view = tiles_x * tiles_y, total = per_view * a.n_views;
@@ -460,11 +477,9 @@ __global__ __launch_bounds__(256) void diff_adjoint_kernel(DiffArgs a) {
     adjoint_tile(a, s_tfl, s_gtf, s_stage[wave], scale, view, tx, ty, lane);
   }
   __syncthreads();
-  const double unscale = ldexp(1.0, -e);
   for (int i = tid; i < 4 * kMaxTf; i += 256) {
-    const int ch = i / kMaxTf, j = i - ch * kMaxTf;
-    const long long v = (long long)s_gtf[i];
-    if (j < a.tf_n && v != 0) unsafeAtomicAdd(&a.grad_tf[4 * j + ch], (float)((double)v * unscale));
+    const unsigned long long v = s_gtf[i];
+    if (v != 0ull) atomicAdd(&a.gtf_acc[i], v);
   }
 }
 
@@ -493,20 +508,19 @@ __device__ __forceinline__ void slab(float o, float d, float lo, float hi, float
 // list with their sample range inside it (conservative); the list's samples
 // are split into 64 contiguous runs, one per lane; each candidate sample's
 // position and cell are recomputed with the forward's operations and samples
-// of other bricks' cells dropped.  Each corner term goes to an int64 LDS
-// accumulator (one scale per (brick, view) from the rays' max |dL/ds|),
-// folded per view into FP32 sums held in registers and stored once as the
-// brick's block in global scratch; diff_combine_kernel then adds, per voxel,
-// the block entries that map to it (own brick, the high halo of the lower
-// neighbour, and the border slots that clamp onto it) in a fixed order.
-// The views are split into groups (one workgroup per brick and group, each
-// with its own block) so that the launch has several waves per SIMD slot.
-// No float atomics: the voxel gradient is deterministic.
+// of other bricks' cells dropped.  Each corner term is added to the brick's
+// int64 fixed-point block in LDS (one scale for the launch), which is
+// converted and stored once, after the group's views, in global scratch;
+// diff_combine_kernel then adds, per voxel, the block entries that
+// map to it (own brick, the high halo of the lower neighbour, and the border
+// slots that clamp onto it) in a fixed order.  The views are split into groups
+// (one workgroup per brick and group, each with its own block) so that the
+// launch has several waves per SIMD slot.  The voxel gradient is
+// deterministic: integer sums in LDS, no global atomics.
 constexpr int kList = 64;    // rays per scatter list
 constexpr int kBlk = kBrick + 2;                    // corner coordinates -1 .. kBrick, plus one
 constexpr int kBlk3 = kBlk * kBlk * kBlk;
 static_assert(kBlk3 == kDiffBlock, "block size shared with the host");
-constexpr int kBlkPerLane = (kBlk3 + 63) / 64;
 
 struct ScatterList {
   float dx[kList], dy[kList], dz[kList], tn[kList], dt[kList];
@@ -519,77 +533,145 @@ struct BrickBox {
   int cx1, cy1, cz1;     // last owned cell
 };
 
-// Scatter the samples of the first n rays of the list (total samples `total`);
-// lane l takes the contiguous run [l per, (l+1) per) and prefetches the next
-// sample's dL/ds while it works on the current one.
+#ifndef HYD_SCATTER_U
+#define HYD_SCATTER_U 8
+#endif
+// Consecutive list items per lane in one round of the scatter (a round covers
+// 64 kScatterU items): their corner terms are summed in registers while they
+// stay in one cell, so a sum holds at most kScatterU terms; the launch's scale
+// leaves kScatterBits bits of headroom for that, so a sum stays below 2^30.
+// Measured on the workload above: 1 item per lane (fully coalesced loads, but
+// neighbouring lanes add to the same slots) 12.7 ms, 2 6.9 ms, 4 5.2 ms,
+// 8 4.9 ms, 16 5.8 ms; keeping the sums of corners shared with the next cell
+// (fewer additions, more instructions) was slower at every length.
+constexpr int kScatterU = HYD_SCATTER_U;
+constexpr int kScatterBits = kScatterU <= 4 ? 2 : (kScatterU <= 8 ? 3 : 4);
+static_assert(kScatterU >= 1 && kScatterU <= 16, "scatter run length");
+
+// First corner slot of cell (x, y, z) in the brick's block (24-bit products).
+__device__ __forceinline__ unsigned corner_slot(int x, int y, int z, const BrickBox &bb) {
+  const unsigned sxl = (unsigned)(x - bb.X0 + 1), syl = (unsigned)(y - bb.Y0 + 1), szl = (unsigned)(z - bb.Z0 + 1);
+  return __umul24(szl, (unsigned)(kBlk * kBlk)) + __umul24(syl, (unsigned)kBlk) + sxl;
+}
+
+// The eight corner sums of one cell into the brick's int64 block.
+__device__ __forceinline__ void flush_cell(unsigned long long *p, const float *A) {
+  constexpr int off[8] = {0, 1, kBlk, kBlk + 1, kBlk * kBlk, kBlk * kBlk + 1, kBlk * kBlk + kBlk,
+                          kBlk * kBlk + kBlk + 1};
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (A[q] != 0.f) atomicAdd(p + off[q], fixed_trunc(A[q]));
+}
+
+// Scatter the samples of the first n rays of the list (total samples `total`)
+// in rounds of 64 kScatterU items: lane l takes items kScatterU l .. of the
+// round, so each load instruction of the wave reads one short stretch of
+// dL/ds (the list's items of one ray are consecutive in memory) and the
+// lane's items are consecutive samples of a ray, which mostly share cells.
+// Each owned sample's corner terms (dL/ds times the launch's scale, times the
+// weight) are summed in registers per cell and added in fixed point (int32
+// by truncation, sign-extended) to the brick's int64 block in LDS.
 __device__ void scatter_list(const DiffArgs &a, const DiffView &v, const ScatterList &L, int n, int total,
                              float scale, unsigned long long *s_acc, int lane, const BrickBox &bb) {
-  const int per = (total + 63) / 64;
-  const int start = lane * per, end = min(total, start + per);
-  int j = 0;   // the last ray whose first sample is <= start (every listed ray has samples)
-  for (int step = kList / 2; step > 0; step >>= 1)
-    if (j + step < n && L.off[j + step] <= start) j += step;
-  RaySetup q;
-  q.ox = v.ray_o[0]; q.oy = v.ray_o[1]; q.oz = v.ray_o[2];
-  q.dx = L.dx[j]; q.dy = L.dy[j]; q.dz = L.dz[j]; q.tnear = L.tn[j]; q.dt = L.dt[j];
-  int next = (j + 1 < n) ? L.off[j + 1] : total;
-  long long base = (long long)L.ray[j] * a.K + L.k0[j] - L.off[j];   // ds index of list sample `it`: base + it
-  int kb = L.k0[j] - L.off[j];                                         // its sample index: kb + it
-  // one step: item it with dL/ds `cur`; issues the load of item it + 1 and
-  // returns it.  Two steps per loop iteration, alternating two registers, so a
-  // load is consumed a whole step after it is issued; the load is issued
-  // unconditionally (past the run's end it rereads the run's last item) so that
-  // no branch forces an early wait for it.
-  const auto step = [&](int it, float cur) -> float {
-    const bool act = it < end;
-    if (act && it == next) {   // next ray of the list
-      ++j;
-      q.dx = L.dx[j]; q.dy = L.dy[j]; q.dz = L.dz[j]; q.tnear = L.tn[j]; q.dt = L.dt[j];
-      next = (j + 1 < n) ? L.off[j + 1] : total;
-      base = (long long)L.ray[j] * a.K + L.k0[j] - L.off[j];
-      kb = L.k0[j] - L.off[j];
+  const float fnx = float(a.nx), fny = float(a.ny), fnz = float(a.nz);
+  for (int r0 = 0; r0 < total; r0 += 64 * kScatterU) {
+    const int i0 = r0 + kScatterU * lane;
+    if (i0 >= total) continue;
+    int j = 0;   // the ray of item i0: the last ray whose first item is <= i0
+    for (int step = kList / 2; step > 0; step >>= 1)
+      if (j + step < n && L.off[j + step] <= i0) j += step;
+    // the items' dL/ds, issued together, and their rays
+    float dv[kScatterU];
+    int ju[kScatterU];
+    {
+      int jj = j, nx = (j + 1 < n) ? L.off[j + 1] : total;
+      const float *p = a.ds + (size_t)L.ray[j] * a.K + (L.k0[j] - L.off[j]);
+#pragma unroll
+      for (int u = 0; u < kScatterU; ++u) {
+        const int it = i0 + u;
+        dv[u] = 0.f;
+        ju[u] = -1;
+        if (it < total) {
+          while (it >= nx) {
+            ++jj;
+            nx = (jj + 1 < n) ? L.off[jj + 1] : total;
+            p = a.ds + (size_t)L.ray[jj] * a.K + (L.k0[jj] - L.off[jj]);
+          }
+          dv[u] = p[it];
+          ju[u] = jj;
+        }
+      }
     }
-    const int in = min(it + 1, end - 1);   // a listed item (end >= 1)
-    const long long nb = (in == next) ? (long long)L.ray[j + 1] * a.K + L.k0[j + 1] - L.off[j + 1] : base;
-    const float nxt = a.ds[nb + in];
-    float px, py, pz;
-    position(q, kb + it, px, py, pz);
-    const float fx = float(a.nx) * px - 0.5f, fy = float(a.ny) * py - 0.5f, fz = float(a.nz) * pz - 0.5f;
-    const float flx = floorf(fx), fly = floorf(fy), flz = floorf(fz);
-    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-    if (act && ix >= bb.cx0 && ix <= bb.cx1 && iy >= bb.cy0 && iy <= bb.cy1 && iz >= bb.cz0 && iz <= bb.cz1 &&
-        cur != 0.f) {
-      const float gx = fx - flx, gy = fy - fly, gz = fz - flz;
-      const float vs = cur * scale;
-      const float hx = 1.f - gx, hy = 1.f - gy;
-      const float wz0 = (1.f - gz) * vs, wz1 = gz * vs;   // z weights times the scaled dL/ds
-      const float w00 = hx * hy, w10 = gx * hy, w01 = hx * gy, w11 = gx * gy;
-      // corner slots relative to the brick, 0 .. kBlk - 1 on each axis (24-bit products)
-      const unsigned sxl = (unsigned)(ix - bb.X0 + 1), syl = (unsigned)(iy - bb.Y0 + 1),
-                     szl = (unsigned)(iz - bb.Z0 + 1);
-      unsigned long long *p = s_acc + (__umul24(__umul24(szl, kBlk) + syl, kBlk) + sxl);
-      atomicAdd(p, fixed_trunc(w00 * wz0));
-      atomicAdd(p + 1, fixed_trunc(w10 * wz0));
-      atomicAdd(p + kBlk, fixed_trunc(w01 * wz0));
-      atomicAdd(p + kBlk + 1, fixed_trunc(w11 * wz0));
-      atomicAdd(p + kBlk * kBlk, fixed_trunc(w00 * wz1));
-      atomicAdd(p + kBlk * kBlk + 1, fixed_trunc(w10 * wz1));
-      atomicAdd(p + kBlk * kBlk + kBlk, fixed_trunc(w01 * wz1));
-      atomicAdd(p + kBlk * kBlk + kBlk + 1, fixed_trunc(w11 * wz1));
+    RaySetup q;
+    q.ox = v.ray_o[0]; q.oy = v.ray_o[1]; q.oz = v.ray_o[2];
+    int jq = -1, kb = 0;
+    float A[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int cx = 0, cy = 0, cz = 0;
+    bool held = false;   // A holds sums of cell (cx, cy, cz)
+#pragma unroll
+    for (int u = 0; u < kScatterU; ++u) {
+      if (ju[u] < 0) continue;
+      if (ju[u] != jq) {
+        jq = ju[u];
+        q.dx = L.dx[jq]; q.dy = L.dy[jq]; q.dz = L.dz[jq]; q.tnear = L.tn[jq]; q.dt = L.dt[jq];
+        kb = L.k0[jq] - L.off[jq];
+      }
+      const float d = dv[u];
+      float px, py, pz;
+      position(q, kb + i0 + u, px, py, pz);
+      const float fx = fnx * px - 0.5f, fy = fny * py - 0.5f, fz = fnz * pz - 0.5f;
+      const float flx = floorf(fx), fly = floorf(fy), flz = floorf(fz);
+      const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+      if (ix >= bb.cx0 && ix <= bb.cx1 && iy >= bb.cy0 && iy <= bb.cy1 && iz >= bb.cz0 && iz <= bb.cz1 &&
+          d != 0.f) {
+        if (held && (ix != cx || iy != cy || iz != cz)) {
+          flush_cell(s_acc + corner_slot(cx, cy, cz, bb), A);
+#pragma unroll
+          for (int k = 0; k < 8; ++k) A[k] = 0.f;
+        }
+        held = true;
+        cx = ix; cy = iy; cz = iz;
+        const float gx = fx - flx, gy = fy - fly, gz = fz - flz;
+        const float hx = 1.f - gx, hy = 1.f - gy;
+        const float vs = d * scale;
+        const float wz0 = (1.f - gz) * vs, wz1 = gz * vs;   // z weights times the scaled dL/ds
+        const float w00 = hx * hy, w10 = gx * hy, w01 = hx * gy, w11 = gx * gy;
+        A[0] = fmaf(w00, wz0, A[0]); A[1] = fmaf(w10, wz0, A[1]);
+        A[2] = fmaf(w01, wz0, A[2]); A[3] = fmaf(w11, wz0, A[3]);
+        A[4] = fmaf(w00, wz1, A[4]); A[5] = fmaf(w10, wz1, A[5]);
+        A[6] = fmaf(w01, wz1, A[6]); A[7] = fmaf(w11, wz1, A[7]);
+      }
     }
-    return nxt;
-  };
-  float dA = a.ds[base + min(start, end - 1)], dB;
-  for (int i = 0; i < per; i += 2) {
-    dB = step(start + i, dA);
-    dA = step(start + i + 1, dB);
+    if (held) flush_cell(s_acc + corner_slot(cx, cy, cz, bb), A);
   }
 }
 
-__global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
+#ifndef HYD_SCATTER_WAVES
+#define HYD_SCATTER_WAVES 1
+#endif
+// Waves per brick workgroup, one list each.  The scatter is bound by the LDS's
+// 64-bit atomics: one wave per block measured 4.95 ms for 32 views at 256^2 of
+// a 128^3 volume (K = 512), two 5.2 ms, four 7.1 ms (more waves contend for
+// the same block).
+constexpr int kScatterWaves = HYD_SCATTER_WAVES;
+
+// Orders one wave's LDS list writes and reads (the waves of the workgroup
+// build and scatter their lists independently).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One workgroup per (brick, view group): its waves share the brick's int64
+// block and take the group's views in turn, each with its own ray list.  The
+// block's integer sums do not depend on the order of the additions, and are
+// converted to FP32 once, after the group's views.
+__global__ __launch_bounds__(64 * kScatterWaves) void diff_scatter_kernel(DiffArgs a) {
   __shared__ unsigned long long s_acc[kBlk3];
-  __shared__ ScatterList L;
-  const int lane = threadIdx.x;
+  __shared__ ScatterList s_list[kScatterWaves];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  ScatterList &L = s_list[wave];
   const int nbx = (a.nx + kBrick - 1) / kBrick, nby = (a.ny + kBrick - 1) / kBrick,
             nbz = (a.nz + kBrick - 1) / kBrick;
   const int bi = blockIdx.x % (nbx * nby * nbz), grp = blockIdx.x / (nbx * nby * nbz);
@@ -603,10 +685,11 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
   bb.cx1 = min(bb.X0 + kBrick - 1, a.nx - 1); bb.cy1 = min(bb.Y0 + kBrick - 1, a.ny - 1);
   bb.cz1 = min(bb.Z0 + kBrick - 1, a.nz - 1);
   float *blk = a.blocks + ((size_t)grp * nbx * nby * nbz + bi) * kBlk3;
-  for (int i = lane; i < kBlk3; i += 64) s_acc[i] = 0ull;
-  float acc[kBlkPerLane];   // the block's FP32 sums over this group's views; entry lane + 64 i
-#pragma unroll
-  for (int i = 0; i < kBlkPerLane; ++i) acc[i] = 0.f;
+  for (int i = tid; i < kBlk3; i += 64 * kScatterWaves) s_acc[i] = 0ull;
+  // one scale for the launch: the largest |dL/ds| times 2^e below 2^30, with
+  // headroom for every sample of the launch adding to one slot
+  const int e = fixed_exponent(__uint_as_float(*a.dsbound), a.vheadroom + kScatterBits);
+  const float scale = ldexpf(1.f, e);
   // the box of the owned cells: floor(n p - 0.5) in [c0, c1]  <=>  p in [(c0 + 0.5) / n, (c1 + 1.5) / n)
   const float delta = 1e-4f;
   const float lo[3] = {(float(bb.cx0) + 0.5f) / float(a.nx) - delta, (float(bb.cy0) + 0.5f) / float(a.ny) - delta,
@@ -614,7 +697,7 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
   const float hi[3] = {(float(bb.cx1) + 1.5f) / float(a.nx) + delta, (float(bb.cy1) + 1.5f) / float(a.ny) + delta,
                        (float(bb.cz1) + 1.5f) / float(a.nz) + delta};
   __syncthreads();
-  for (int view = v_begin; view < v_end; ++view) {
+  for (int view = v_begin + wave; view < v_end; view += kScatterWaves) {
     const DiffView &v = a.views[view];
     // pixel bounding box of the projected box (whole image if a corner is not in front)
     int px0 = 0, px1 = a.W - 1, py0 = 0, py1 = a.H - 1;
@@ -645,19 +728,6 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
     const int bw = px1 - px0 + 1, bh = py1 - py0 + 1;
     if (bw <= 0 || bh <= 0) continue;
     const int nR = bw * bh;
-    float B = 0.f;
-    for (int r = lane; r < nR; r += 64) {
-      const int y = py0 + r / bw, x = px0 + r % bw;
-      B = fmaxf(B, a.dsmax[((size_t)view * a.H + y) * a.W + x]);
-    }
-    B = wave_max(B);
-    if (!(B > 0.f)) continue;
-    // terms below 2^30 each; terms per accumulator in this view <= 8 nR K, so the
-    // int64 sums hold while nR K <= 2^29
-    const double items = (double)nR * (double)a.K;
-    const int hv = items > 536870912.0 ? (int)ceil(log2(items / 536870912.0)) : 0;
-    const int e = fixed_exponent(B, hv);
-    const float scale = ldexpf(1.f, e);
     int n = 0, total = 0;   // rays and samples in the list (wave-uniform)
     for (int base = 0; base < nR; base += 64) {
       const int rr = base + lane;
@@ -693,9 +763,9 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
       const int n_new = __popcll(m);
       if (n_new == 0) continue;
       if (n + n_new > kList) {   // list full: scatter it first
-        __syncthreads();
+        wave_sync();
         scatter_list(a, v, L, n, total, scale, s_acc, lane, bb);
-        __syncthreads();
+        wave_sync();
         n = 0;
         total = 0;
       }
@@ -709,29 +779,14 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
       total += __shfl(incl, 63);
     }
     if (n > 0) {
-      __syncthreads();
+      wave_sync();
       scatter_list(a, v, L, n, total, scale, s_acc, lane, bb);
     }
-    __syncthreads();
-    const double unscale = ldexp(1.0, -e);
-#pragma unroll
-    for (int i = 0; i < kBlkPerLane; ++i) {
-      const int idx = lane + 64 * i;
-      if (idx < kBlk3) {
-        const long long val = (long long)s_acc[idx];
-        if (val != 0) {
-          acc[i] += (float)((double)val * unscale);
-          s_acc[idx] = 0ull;
-        }
-      }
-    }
-    __syncthreads();
+    wave_sync();
   }
-#pragma unroll
-  for (int i = 0; i < kBlkPerLane; ++i) {
-    const int idx = lane + 64 * i;
-    if (idx < kBlk3) blk[idx] = acc[i];
-  }
+  __syncthreads();
+  const float unscale = ldexpf(1.f, -e);
+  for (int i = tid; i < kBlk3; i += 64 * kScatterWaves) blk[i] = (float)(long long)s_acc[i] * unscale;
 }
 
 // Block entries that map to voxel coordinate v on one axis of extent n: the
@@ -750,8 +805,18 @@ __device__ __forceinline__ int axis_sources(int v, int n, int *db, int *slot) {
 }
 
 // grad_vol[v] = over the view groups, every block entry that maps to v
-// (axis_sources on each axis), summed in a fixed order.
+// (axis_sources on each axis), summed in a fixed order.  Workgroup 0 also
+// writes grad_tf from the int64 sums: one rounding (int64 -> float) and an
+// exact power-of-two scale.
 __global__ __launch_bounds__(256) void diff_combine_kernel(DiffArgs a) {
+  if (blockIdx.x == 0) {
+    const int e = tf_exponent(a, threadIdx.x & 63);
+    const float unscale = ldexpf(1.f, -e);
+    for (int i = threadIdx.x; i < 4 * a.tf_n; i += 256) {
+      const int j = i >> 2, ch = i & 3;
+      a.grad_tf[i] = (float)(long long)a.gtf_acc[ch * kMaxTf + j] * unscale;
+    }
+  }
   const size_t nvox = (size_t)a.nx * a.ny * a.nz;
   const int nbx = (a.nx + kBrick - 1) / kBrick, nby = (a.ny + kBrick - 1) / kBrick;
   const size_t nb = (size_t)nbx * nby * ((a.nz + kBrick - 1) / kBrick);
@@ -790,7 +855,7 @@ hipError_t launch_diff_backward(const DiffArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(diff_adjoint_kernel, dim3(a.n_workers), dim3(256), 0, s, a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int nb = ((a.nx + kBrick - 1) / kBrick) * ((a.ny + kBrick - 1) / kBrick) * ((a.nz + kBrick - 1) / kBrick);
-  hipLaunchKernelGGL(diff_scatter_kernel, dim3(nb * a.view_groups), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(diff_scatter_kernel, dim3(nb * a.view_groups), dim3(64 * kScatterWaves), 0, s, a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const size_t nvox = (size_t)a.nx * a.ny * a.nz;
   const int cg = (int)std::min<size_t>((nvox + 255) / 256, 4096);

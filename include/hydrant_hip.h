@@ -385,7 +385,7 @@ int32_t hyd_batch_autotune_schedule(hyd_ctx *ctx, hyd_batch *batch, float densit
  * forward keeps transmittance checkpoints every ckpt_every samples; the adjoint
  * recomputes each segment from them and returns dL/dTF and dL/dvolume. */
 typedef struct {
-  int32_t dims[3];        /* voxels (x, y, z) */
+  int32_t dims[3];        /* voxels (x, y, z); x in [2, 2^22) */
   int32_t n_steps;        /* samples per ray (512) */
   int32_t ckpt_every;     /* checkpoint interval (64) */
   float sigma;            /* extinction scale */
@@ -401,9 +401,9 @@ int32_t hyd_diff_forward(hyd_ctx *ctx, const hyd_diff_params *p, const hyd_view 
                          float *rgba_dev, float *ckpt_dev, unsigned long long *samples_dev);
 /* Adjoint: writes dL/dvolume to grad_volume_dev and dL/dTF to grad_tf_dev
  * (both overwritten) given dL/dRGBA (grad_rgba_dev).  The context keeps a
- * scratch buffer of n_views * height * width * (n_steps + 1) floats (dL/ds of
- * every sample), grown on demand; n_views * height * width must be < 2^31.
- * Voxel gradients are deterministic (integer accumulation per brick). */
+ * scratch buffer of n_views * height * width * (n_steps + 8) floats (dL/ds of
+ * every sample, ray setups), grown on demand; n_views * height * width must be
+ * < 2^31.  Both gradients are deterministic (integer accumulation). */
 int32_t hyd_diff_backward(hyd_ctx *ctx, const hyd_diff_params *p, const hyd_view *views, int32_t n_views,
                           int32_t width, int32_t height, const float *volume_dev, const float *tf_dev, int32_t tf_n,
                           const float *ckpt_dev, const float *grad_rgba_dev, float *grad_volume_dev,

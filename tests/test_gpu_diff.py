@@ -95,6 +95,7 @@ def test_adjoint_ragged_sizes(gpu_ctx):
     gv, gt = r.backward_raw(vol, tf, ck, torch.from_numpy(grad).cuda())
     torch.cuda.synchronize()
     assert torch.equal(gv, gv1), "voxel gradients are deterministic"
+    assert torch.equal(gt, gt1), "TF gradients are deterministic"
     sc = O.DiffScene(wl.W, wl.H, vol_np, wl.hidden_tf, K=wl.n_steps, sigma=wl.sigma)
     ys, xs = np.mgrid[0:wl.H, 0:wl.W]
     xy = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.int32)
