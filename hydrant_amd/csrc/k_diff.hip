@@ -306,13 +306,10 @@ __global__ __launch_bounds__(256) void diff_bound_kernel(DiffArgs a) {
 
 constexpr int kRun = 8;      // samples per register run (TF terms, scatter corners): 3 bits of headroom
 
-__device__ __forceinline__ void flush_tf(unsigned long long *s_gtf, int j0, int j1, const float *T0, const float *T1,
-                                         float scale) {
+// The four channel sums of one TF entry into the fixed-point LDS accumulators.
+__device__ __forceinline__ void flush_tf_entry(unsigned long long *s_gtf, int j, const float *T, float scale) {
 #pragma unroll
-  for (int ch = 0; ch < 4; ++ch) {
-    atomicAdd(&s_gtf[ch * kMaxTf + j0], fixed_term(T0[ch], scale));
-    atomicAdd(&s_gtf[ch * kMaxTf + j1], fixed_term(T1[ch], scale));
-  }
+  for (int ch = 0; ch < 4; ++ch) atomicAdd(&s_gtf[ch * kMaxTf + j], fixed_term(T[ch], scale));
 }
 
 constexpr int kStage = 16;   // samples per ray staged in LDS before a store
@@ -349,7 +346,9 @@ __device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long 
   }
   float dmax = 0.f;
   // TF-gradient run: terms of consecutive samples on the same pair of TF entries
-  // are summed in registers (at most kRun of them) before they reach LDS
+  // (or on pairs one entry apart, for the entry they share) are summed in
+  // registers (at most kRun samples since the last full flush) before they
+  // reach LDS
   int tj0 = -1, tj1 = -1, tn = 0;
   float T0[4] = {0.f, 0.f, 0.f, 0.f}, T1[4] = {0.f, 0.f, 0.f, 0.f};
   if (amask != 0ull) {
@@ -401,12 +400,22 @@ __device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long 
             const float4 p = tf[sh.j0], q = tf[sh.j1];
             ds = float(a.tf_n) * ((q.x - p.x) * d[0] + (q.y - p.y) * d[1] + (q.z - p.z) * d[2] + (q.w - p.w) * d[3]);
             if (sh.j0 != tj0 || sh.j1 != tj1 || tn == kRun) {
-              if (tn > 0) flush_tf(s_gtf, tj0, tj1, T0, T1, scale);
+              // a move by one entry keeps the sums of the entry both pairs share
+              const bool up = tn < kRun && sh.j0 == tj1;     // old second entry = new first
+              const bool down = tn < kRun && sh.j1 == tj0;   // old first entry = new second
+              if (tn > 0) {
+                if (!down) flush_tf_entry(s_gtf, tj0, T0, scale);
+                if (!up) flush_tf_entry(s_gtf, tj1, T1, scale);
+              }
+#pragma unroll
+              for (int ch = 0; ch < 4; ++ch) {
+                const float n0 = up ? T1[ch] : 0.f, n1 = down ? T0[ch] : 0.f;
+                T0[ch] = n0;
+                T1[ch] = n1;
+              }
+              if (!up && !down) tn = 0;
               tj0 = sh.j0;
               tj1 = sh.j1;
-              tn = 0;
-#pragma unroll
-              for (int ch = 0; ch < 4; ++ch) T0[ch] = T1[ch] = 0.f;
             }
 #pragma unroll
             for (int ch = 0; ch < 4; ++ch) {
@@ -424,17 +433,32 @@ __device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long 
         }
         stage[lane * kStagePitch + (k % kStage)] = ds;
         if (k % kStage == 0) {
-          // samples k .. k+15 of all 64 rays are staged: store 16 rays x 4 at a time,
-          // 16 lanes per ray, one 64 B run each
+          // samples k .. k+15 of all 64 rays are staged: with K a multiple of 4,
+          // 16 rays at a time, 4 lanes per ray storing 16 B each (one 64 B run
+          // per ray); else 4 rays at a time, 16 lanes per ray, 4 B each
           __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
           __builtin_amdgcn_wave_barrier();
-          const int kk = lane % kStage, kg = k + kk;
+          if ((a.K & 3) == 0) {
+            const int kk = 4 * (lane & 3), kg = k + kk;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int rr = 16 * i + (lane >> 2);
+              if (kg < a.K && ((amask >> rr) & 1ull)) {
+                const int xr = tx * kTile + (rr % kTile), yr = ty * kTile + (rr / kTile);
+                const float *st = stage + rr * kStagePitch + kk;
+                *reinterpret_cast<float4 *>(a.ds + (((size_t)view * a.H + yr) * a.W + xr) * a.K + kg) =
+                    make_float4(st[0], st[1], st[2], st[3]);
+              }
+            }
+          } else {
+            const int kk = lane % kStage, kg = k + kk;
 #pragma unroll 1
-          for (int i = 0; i < 64 / (64 / kStage); ++i) {
-            const int rr = i * (64 / kStage) + lane / kStage;
-            if (kg < a.K && ((amask >> rr) & 1ull)) {
-              const int xr = tx * kTile + (rr % kTile), yr = ty * kTile + (rr / kTile);
-              a.ds[(((size_t)view * a.H + yr) * a.W + xr) * a.K + kg] = stage[rr * kStagePitch + kk];
+            for (int i = 0; i < 64 / (64 / kStage); ++i) {
+              const int rr = i * (64 / kStage) + lane / kStage;
+              if (kg < a.K && ((amask >> rr) & 1ull)) {
+                const int xr = tx * kTile + (rr % kTile), yr = ty * kTile + (rr / kTile);
+                a.ds[(((size_t)view * a.H + yr) * a.W + xr) * a.K + kg] = stage[rr * kStagePitch + kk];
+              }
             }
           }
           __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -443,7 +467,10 @@ __device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long 
       }
       if (active) Tnext = Tstart;   // re-anchor at the checkpoint
     }
-    if (tn > 0) flush_tf(s_gtf, tj0, tj1, T0, T1, scale);
+    if (tn > 0) {
+      flush_tf_entry(s_gtf, tj0, T0, scale);
+      flush_tf_entry(s_gtf, tj1, T1, scale);
+    }
   }
   if (in_image) a.dsmax[ray] = dmax;
   const float wmax = wave_max(dmax);
@@ -522,9 +549,10 @@ constexpr int kBlk = kBrick + 2;                    // corner coordinates -1 .. 
 constexpr int kBlk3 = kBlk * kBlk * kBlk;
 static_assert(kBlk3 == kDiffBlock, "block size shared with the host");
 
+// Rays of a scatter list: setup, candidate samples k0 .. k1, first segment.
 struct ScatterList {
   float dx[kList], dy[kList], dz[kList], tn[kList], dt[kList];
-  int k0[kList], off[kList], ray[kList];
+  int k0[kList], k1[kList], off[kList], ray[kList];
 };
 
 struct BrickBox {
@@ -533,20 +561,13 @@ struct BrickBox {
   int cx1, cy1, cz1;     // last owned cell
 };
 
-#ifndef HYD_SCATTER_U
-#define HYD_SCATTER_U 8
-#endif
-// Consecutive list items per lane in one round of the scatter (a round covers
-// 64 kScatterU items): their corner terms are summed in registers while they
-// stay in one cell, so a sum holds at most kScatterU terms; the launch's scale
-// leaves kScatterBits bits of headroom for that, so a sum stays below 2^30.
-// Measured on the workload above: 1 item per lane (fully coalesced loads, but
-// neighbouring lanes add to the same slots) 12.7 ms, 2 6.9 ms, 4 5.2 ms,
-// 8 4.9 ms, 16 5.8 ms; keeping the sums of corners shared with the next cell
-// (fewer additions, more instructions) was slower at every length.
-constexpr int kScatterU = HYD_SCATTER_U;
-constexpr int kScatterBits = kScatterU <= 4 ? 2 : (kScatterU <= 8 ? 3 : 4);
-static_assert(kScatterU >= 1 && kScatterU <= 16, "scatter run length");
+// A lane's segment has at most 8 samples, so a register sum holds at most 8
+// terms; the launch's scale leaves 3 bits of headroom for that, so a sum stays
+// below 2^30.  Measured for 32 views at 256^2 of a 128^3 volume (K = 512):
+// lanes on single samples (neighbouring lanes adding to the same slots)
+// 12.7 ms, runs of 4 samples 5.2 ms, of 8 4.9 ms, of 16 5.8 ms; keeping the
+// sums of corners shared with the next cell was slower at every length.
+constexpr int kScatterBits = 3;
 
 // First corner slot of cell (x, y, z) in the brick's block (24-bit products).
 __device__ __forceinline__ unsigned corner_slot(int x, int y, int z, const BrickBox &bb) {
@@ -563,71 +584,57 @@ __device__ __forceinline__ void flush_cell(unsigned long long *p, const float *A
     if (A[q] != 0.f) atomicAdd(p + off[q], fixed_trunc(A[q]));
 }
 
-// Scatter the samples of the first n rays of the list (total samples `total`)
-// in rounds of 64 kScatterU items: lane l takes items kScatterU l .. of the
-// round, so each load instruction of the wave reads one short stretch of
-// dL/ds (the list's items of one ray are consecutive in memory) and the
-// lane's items are consecutive samples of a ray, which mostly share cells.
-// Each owned sample's corner terms (dL/ds times the launch's scale, times the
-// weight) are summed in registers per cell and added in fixed point (int32
+// Scatter the first n rays of the list, whose candidate samples form nseg
+// segments (aligned runs of 8 sample indices, clipped to the ray's candidate
+// range), in rounds of 64 segments: lane l takes segment l of the round, so a
+// lane's samples are consecutive samples of one ray (which mostly share cells)
+// and its dL/ds is one 32-byte run (two 16-byte loads when K is a multiple of
+// 8).  Each owned sample's corner terms (dL/ds times the launch's scale, times
+// the weight) are summed in registers per cell and added in fixed point (int32
 // by truncation, sign-extended) to the brick's int64 block in LDS.
-__device__ void scatter_list(const DiffArgs &a, const DiffView &v, const ScatterList &L, int n, int total,
+__device__ void scatter_list(const DiffArgs &a, const DiffView &v, const ScatterList &L, int n, int nseg,
                              float scale, unsigned long long *s_acc, int lane, const BrickBox &bb) {
   const float fnx = float(a.nx), fny = float(a.ny), fnz = float(a.nz);
-  for (int r0 = 0; r0 < total; r0 += 64 * kScatterU) {
-    const int i0 = r0 + kScatterU * lane;
-    if (i0 >= total) continue;
-    int j = 0;   // the ray of item i0: the last ray whose first item is <= i0
+  const bool vec = (a.K & 7) == 0;
+  for (int g0 = 0; g0 < nseg; g0 += 64) {
+    const int g = g0 + lane;
+    if (g >= nseg) continue;
+    int j = 0;   // the ray of segment g: the last ray whose first segment is <= g
     for (int step = kList / 2; step > 0; step >>= 1)
-      if (j + step < n && L.off[j + step] <= i0) j += step;
-    // the items' dL/ds, issued together, and their rays
-    float dv[kScatterU];
-    int ju[kScatterU];
-    {
-      int jj = j, nx = (j + 1 < n) ? L.off[j + 1] : total;
-      const float *p = a.ds + (size_t)L.ray[j] * a.K + (L.k0[j] - L.off[j]);
+      if (j + step < n && L.off[j + step] <= g) j += step;
+    const int c8 = 8 * ((L.k0[j] >> 3) + (g - L.off[j]));
+    const int ks = max(L.k0[j], c8), ke = min(L.k1[j], c8 + 7);
+    const float *p = a.ds + (size_t)L.ray[j] * a.K + c8;
+    float dv[8];
+    if (vec) {
+      const float4 u0 = reinterpret_cast<const float4 *>(p)[0], u1 = reinterpret_cast<const float4 *>(p)[1];
+      dv[0] = u0.x; dv[1] = u0.y; dv[2] = u0.z; dv[3] = u0.w;
+      dv[4] = u1.x; dv[5] = u1.y; dv[6] = u1.z; dv[7] = u1.w;
+    } else {
 #pragma unroll
-      for (int u = 0; u < kScatterU; ++u) {
-        const int it = i0 + u;
-        dv[u] = 0.f;
-        ju[u] = -1;
-        if (it < total) {
-          while (it >= nx) {
-            ++jj;
-            nx = (jj + 1 < n) ? L.off[jj + 1] : total;
-            p = a.ds + (size_t)L.ray[jj] * a.K + (L.k0[jj] - L.off[jj]);
-          }
-          dv[u] = p[it];
-          ju[u] = jj;
-        }
-      }
+      for (int u = 0; u < 8; ++u) dv[u] = (c8 + u >= ks && c8 + u <= ke) ? p[u] : 0.f;
     }
     RaySetup q;
     q.ox = v.ray_o[0]; q.oy = v.ray_o[1]; q.oz = v.ray_o[2];
-    int jq = -1, kb = 0;
+    q.dx = L.dx[j]; q.dy = L.dy[j]; q.dz = L.dz[j]; q.tnear = L.tn[j]; q.dt = L.dt[j];
     float A[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     int cx = 0, cy = 0, cz = 0;
     bool held = false;   // A holds sums of cell (cx, cy, cz)
 #pragma unroll
-    for (int u = 0; u < kScatterU; ++u) {
-      if (ju[u] < 0) continue;
-      if (ju[u] != jq) {
-        jq = ju[u];
-        q.dx = L.dx[jq]; q.dy = L.dy[jq]; q.dz = L.dz[jq]; q.tnear = L.tn[jq]; q.dt = L.dt[jq];
-        kb = L.k0[jq] - L.off[jq];
-      }
+    for (int u = 0; u < 8; ++u) {
+      const int k = c8 + u;
       const float d = dv[u];
+      if (k < ks || k > ke || d == 0.f) continue;
       float px, py, pz;
-      position(q, kb + i0 + u, px, py, pz);
+      position(q, k, px, py, pz);
       const float fx = fnx * px - 0.5f, fy = fny * py - 0.5f, fz = fnz * pz - 0.5f;
       const float flx = floorf(fx), fly = floorf(fy), flz = floorf(fz);
       const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-      if (ix >= bb.cx0 && ix <= bb.cx1 && iy >= bb.cy0 && iy <= bb.cy1 && iz >= bb.cz0 && iz <= bb.cz1 &&
-          d != 0.f) {
+      if (ix >= bb.cx0 && ix <= bb.cx1 && iy >= bb.cy0 && iy <= bb.cy1 && iz >= bb.cz0 && iz <= bb.cz1) {
         if (held && (ix != cx || iy != cy || iz != cz)) {
           flush_cell(s_acc + corner_slot(cx, cy, cz, bb), A);
 #pragma unroll
-          for (int k = 0; k < 8; ++k) A[k] = 0.f;
+          for (int q8 = 0; q8 < 8; ++q8) A[q8] = 0.f;
         }
         held = true;
         cx = ix; cy = iy; cz = iz;
@@ -728,10 +735,10 @@ __global__ __launch_bounds__(64 * kScatterWaves) void diff_scatter_kernel(DiffAr
     const int bw = px1 - px0 + 1, bh = py1 - py0 + 1;
     if (bw <= 0 || bh <= 0) continue;
     const int nR = bw * bh;
-    int n = 0, total = 0;   // rays and samples in the list (wave-uniform)
+    int n = 0, total = 0;   // rays and segments in the list (wave-uniform)
     for (int base = 0; base < nR; base += 64) {
       const int rr = base + lane;
-      int cnt = 0, k0 = 0, ray = 0;
+      int cnt = 0, k0 = 0, k1 = -1, ray = 0;
       RaySetup rs{};
       if (rr < nR) {
         const int y = py0 + rr / bw, x = px0 + rr % bw;
@@ -749,13 +756,17 @@ __global__ __launch_bounds__(64 * kScatterWaves) void diff_scatter_kernel(DiffAr
             if (rs.dt > 0.f) {
               const float rdt = __builtin_amdgcn_rcpf(rs.dt);
               const float fa = (ta - rs.tnear) * rdt - 0.5f, fb = (tb - rs.tnear) * rdt - 0.5f;
-              const float ka = fminf(fmaxf(ceilf(fa) - 1.f, 0.f), float(a.K));
-              const float kb = fminf(fmaxf(floorf(fb) + 1.f, -1.f), float(a.K - 1));
+              // the box's margin (1e-4) is far above the rounding of ta, tb and
+              // of these quotients, so the range holds every owned sample
+              const float ka = fminf(fmaxf(ceilf(fa), 0.f), float(a.K));
+              const float kb = fminf(fmaxf(floorf(fb), -1.f), float(a.K - 1));
               k0 = (int)ka;
-              cnt = max(0, (int)kb - k0 + 1);
+              k1 = (int)kb;
             } else {
-              cnt = a.K;
+              k0 = 0;
+              k1 = a.K - 1;
             }
+            cnt = k1 >= k0 ? (k1 >> 3) - (k0 >> 3) + 1 : 0;   // segments
           }
         }
       }
@@ -773,7 +784,7 @@ __global__ __launch_bounds__(64 * kScatterWaves) void diff_scatter_kernel(DiffAr
       if (cnt > 0) {
         const int pos = n + __popcll(m & ((1ull << lane) - 1ull));
         L.dx[pos] = rs.dx; L.dy[pos] = rs.dy; L.dz[pos] = rs.dz; L.tn[pos] = rs.tnear; L.dt[pos] = rs.dt;
-        L.k0[pos] = k0; L.off[pos] = total + incl - cnt; L.ray[pos] = ray;
+        L.k0[pos] = k0; L.k1[pos] = k1; L.off[pos] = total + incl - cnt; L.ray[pos] = ray;
       }
       n += n_new;
       total += __shfl(incl, 63);

@@ -77,13 +77,16 @@ def test_adjoint_matches_fp64_gradient(gpu_ctx):
     assert ev <= 1e-3 and et <= 1e-3
 
 
-def test_adjoint_ragged_sizes(gpu_ctx):
+@pytest.mark.parametrize("n_steps", [None, 98])
+def test_adjoint_ragged_sizes(gpu_ctx, n_steps):
     """Image and volume sizes that are not multiples of the 8x8 ray tiles and 8^3
     gradient bricks (partial tiles and bricks), gradient on every pixel, two calls in a
-    row (the outputs are overwritten, not accumulated)."""
+    row (the outputs are overwritten, not accumulated); also a sample count that is not
+    a multiple of 4 or 8 (the element-wise paths of the dL/ds stores and loads) or of the
+    checkpoint interval."""
     import torch
 
-    wl = W.differentiable(n=44, n_views=2, W=60, H=37)
+    wl = W.differentiable(n=44, n_views=2, W=60, H=37, n_steps=n_steps or 512)
     vol_np = wl.hidden_volume[:, :, :41].copy()          # dims (x, y, z) = (41, 44, 44)
     r = DiffRenderer(gpu_ctx, wl.views()[:2], wl.W, wl.H, vol_np.shape[::-1], wl.n_steps, wl.ckpt_every, wl.sigma)
     vol = torch.from_numpy(vol_np).cuda()
