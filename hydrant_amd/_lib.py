@@ -96,6 +96,12 @@ class Dataset(C.Structure):
                 ("apron", C.c_int32)]
 
 
+class SegmentBatch(C.Structure):
+    """hyd_segment_batch (include/hydrant_hip.h)."""
+    _fields_ = [("fmt", C.c_int32), ("pad_edge", C.c_int32), ("n_pages", C.c_int32), ("pool_dev", C.c_void_p),
+                ("vaddr", C.POINTER(C.c_int32)), ("page_k", C.POINTER(C.c_float)), ("page_b", C.POINTER(C.c_float))]
+
+
 class PackageInfo(C.Structure):
     """hyd_package_info (include/hydrant_renderer.h)."""
     _fields_ = [("raw", C.c_int32 * 3), ("dim", C.c_int32 * 3), ("block_size", C.c_int32), ("padding", C.c_int32),
@@ -141,6 +147,9 @@ SIGNATURES = {
     "hyd_composite_iso": (I32, [P, I32, P, PI32, I32, I32, P, P]),
     "hyd_segments_begin": (I32, [P, I32, I32, I32, P]),
     "hyd_segments_pass": (I32, [P, C.POINTER(VolumeParams), C.POINTER(View), I32, I32, I32, P, P, P]),
+    "hyd_segment_set_create": (I32, [P, I32, C.POINTER(SegmentBatch), C.POINTER(P)]),
+    "hyd_segment_set_destroy": (I32, [P]),
+    "hyd_segments_pass_set": (I32, [P, P, C.POINTER(VolumeParams), C.POINTER(View), I32, I32, I32, P, P, P]),
     "hyd_segments_composite": (I32, [P, I32, I32, I32, P, P, P]),
     "hyd_trace_volume": (I32, [P, C.POINTER(VolumeParams), C.POINTER(View), I32, I32, I32, I32,
                                C.POINTER(TraceRec), I32, PI32]),

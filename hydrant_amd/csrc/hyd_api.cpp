@@ -1,7 +1,9 @@
 // C ABI implementation: context, resident state and pass dispatch.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -547,6 +549,120 @@ int32_t hyd_segments_pass(hyd_ctx *c, const hyd_volume_params *p, const hyd_view
   a.max_slots = max_slots;
   a.samples = samples_dev;
   HYD_CHECK_HIP(launch_segments_pass(a, c->pool.fmt, c->stream));
+  return HYD_OK;
+}
+
+}  // extern "C"
+
+struct hyd_segment_set {
+  int device = 0;
+  int32_t n = 0, fmt = 0, pad = 0, max_pages = 0;
+  int32_t vdim[3] = {0, 0, 0};
+  uint64_t max_pool_elems = 0;
+  const void *pool[hyd::kMaxSegBatches] = {};
+  int32_t *codes = nullptr;   // [n][cells]
+  float4 *pages = nullptr;    // [n][max_pages]
+};
+
+extern "C" {
+
+int32_t hyd_segment_set_destroy(hyd_segment_set *set) {
+  if (!set) return HYD_OK;
+  if (hipSetDevice(set->device) == hipSuccess) {
+    if (set->codes) (void)hipFree(set->codes);
+    if (set->pages) (void)hipFree(set->pages);
+  }
+  delete set;
+  return HYD_OK;
+}
+
+int32_t hyd_segment_set_create(hyd_ctx *c, int32_t n_batches, const hyd_segment_batch *b, hyd_segment_set **out) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(b && out, HYD_ERR_ARG, "null argument");
+  *out = nullptr;
+  HYD_REQUIRE(n_batches >= 1 && n_batches <= kMaxSegBatches, HYD_ERR_RANGE, "n_batches %d outside [1, %d]",
+              n_batches, kMaxSegBatches);
+  HYD_REQUIRE(!c->host_cheb.empty(), HYD_ERR_STATE, "no empty-space grid: call hyd_set_empty_grid first");
+  const size_t cells = c->host_cheb.size();
+  std::unique_ptr<hyd_segment_set> set(new hyd_segment_set);
+  set->device = c->device;
+  set->n = n_batches;
+  set->fmt = b[0].fmt;
+  set->pad = b[0].pad_edge;
+  std::memcpy(set->vdim, c->cheb.dim, sizeof(set->vdim));
+  for (int i = 0; i < n_batches; ++i) {
+    HYD_REQUIRE(fmt_bytes(b[i].fmt) > 0 && b[i].fmt == set->fmt, HYD_ERR_ARG,
+                "batch %d: format %d (every batch needs the first batch's known format)", i, b[i].fmt);
+    HYD_REQUIRE(b[i].pad_edge == set->pad && set->pad >= 2 && set->pad <= 1024, HYD_ERR_ARG,
+                "batch %d: page edge %d (every batch needs the first batch's, in [2, 1024])", i, b[i].pad_edge);
+    HYD_REQUIRE(b[i].n_pages > 0 && b[i].pool_dev && b[i].vaddr && b[i].page_k && b[i].page_b, HYD_ERR_ARG,
+                "batch %d: invalid pool or page table", i);
+    set->max_pages = std::max(set->max_pages, b[i].n_pages);
+    set->max_pool_elems =
+        std::max(set->max_pool_elems, (uint64_t)b[i].n_pages * set->pad * set->pad * set->pad);
+    set->pool[i] = b[i].pool_dev;
+  }
+  std::vector<int32_t> codes((size_t)n_batches * cells);
+  std::vector<float4> pages((size_t)n_batches * set->max_pages, make_float4(0.f, 0.f, 0.f, 0.f));
+  for (int i = 0; i < n_batches; ++i) {
+    for (size_t j = 0; j < cells; ++j) {
+      const int32_t v = b[i].vaddr[j];
+      HYD_REQUIRE(v >= -1 && v < b[i].n_pages, HYD_ERR_RANGE, "batch %d: vaddr[%zu] = %d outside [-1, %d)", i, j,
+                  v, b[i].n_pages);
+      const int32_t cd = c->host_cheb[j];
+      codes[(size_t)i * cells + j] = cd > 0 ? -1 - cd : v;
+    }
+    for (int k = 0; k < b[i].n_pages; ++k)
+      pages[(size_t)i * set->max_pages + k] =
+          make_float4(b[i].page_k[k], b[i].page_b[3 * k], b[i].page_b[3 * k + 1], b[i].page_b[3 * k + 2]);
+  }
+  HYD_CHECK_HIP(hipMalloc((void **)&set->codes, codes.size() * sizeof(int32_t)));
+  HYD_CHECK_HIP(hipMalloc((void **)&set->pages, pages.size() * sizeof(float4)));
+  HYD_CHECK_HIP(hipMemcpy(set->codes, codes.data(), codes.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HYD_CHECK_HIP(hipMemcpy(set->pages, pages.data(), pages.size() * sizeof(float4), hipMemcpyHostToDevice));
+  *out = set.release();
+  return HYD_OK;
+}
+
+int32_t hyd_segments_pass_set(hyd_ctx *c, const hyd_segment_set *set, const hyd_volume_params *p,
+                              const hyd_view *view, int32_t width, int32_t height, int32_t max_slots,
+                              float *slots_dev, int32_t *nslots_dev, unsigned long long *samples_dev) {
+  int32_t st = activate(c);
+  if (st) return st;
+  HYD_REQUIRE(set && view && slots_dev && nslots_dev && width > 0 && height > 0, HYD_ERR_ARG,
+              "invalid segment pass");
+  HYD_REQUIRE(set->device == c->device, HYD_ERR_ARG, "the set was made on device %d, the context uses %d",
+              set->device, c->device);
+  HYD_REQUIRE(c->tf, HYD_ERR_STATE, "no transfer function: call hyd_set_tf first");
+  HYD_REQUIRE(c->cheb.data, HYD_ERR_STATE, "no empty-space grid: call hyd_set_empty_grid first");
+  const int32_t need = set->vdim[0] + set->vdim[1] + set->vdim[2];
+  HYD_REQUIRE(max_slots >= need, HYD_ERR_RANGE, "max_slots %d < %d (sum of the brick grid dims)", max_slots, need);
+  VolumeArgs a;
+  if ((st = fill_volume_args(c, p, a))) return st;
+  HYD_REQUIRE(a.schedule == HYD_SCHED_LATTICE, HYD_ERR_ARG, "segments need the lattice schedule");
+  std::memcpy(a.vdim, set->vdim, sizeof(a.vdim));
+  a.pool = set->pool[0];
+  a.codes = set->codes;
+  a.page = set->pages;
+  a.pool_elems = set->max_pool_elems;
+  a.pad = set->pad;
+  fill_view(view, a.trans, a.ctg, a.ray_o);
+  a.W = width;
+  a.H = height;
+  a.slots = reinterpret_cast<float4 *>(slots_dev);
+  a.nslots = nslots_dev;
+  a.max_slots = max_slots;
+  a.samples = samples_dev;
+  SegBatchTable tb{};
+  tb.n = set->n;
+  const size_t cells = (size_t)set->vdim[0] * set->vdim[1] * set->vdim[2];
+  for (int i = 0; i < set->n; ++i) {
+    tb.pool[i] = set->pool[i];
+    tb.codes[i] = set->codes + (size_t)i * cells;
+    tb.page[i] = set->pages + (size_t)i * set->max_pages;
+  }
+  HYD_CHECK_HIP(launch_segments_batches(a, tb, set->fmt, c->stream));
   return HYD_OK;
 }
 

@@ -155,6 +155,16 @@ struct VolumeArgs {
 hipError_t launch_volume(const VolumeArgs &a, int fmt, bool resume, hipStream_t s);
 hipError_t launch_segments_clear(float4 *slots, size_t n, hipStream_t s);
 hipError_t launch_segments_pass(const VolumeArgs &a, int fmt, hipStream_t s);
+// Residency batches rendered by one segment launch (grid z = batch): each
+// batch's pool, per-brick codes and page descriptors replace the arguments'.
+constexpr int kMaxSegBatches = 16;
+struct SegBatchTable {
+  const void *pool[kMaxSegBatches];
+  const int32_t *codes[kMaxSegBatches];
+  const float4 *page[kMaxSegBatches];
+  int32_t n;
+};
+hipError_t launch_segments_batches(const VolumeArgs &a, const SegBatchTable &tb, int fmt, hipStream_t s);
 hipError_t launch_segments_composite(const float4 *slots, const int32_t *nslots, int max_slots, int P, float4 *rgba,
                                      hipStream_t s);
 hipError_t launch_volume_trace(const VolumeArgs &a, int fmt, int x, int y, hyd_trace_rec *out_dev, int max_recs,

@@ -598,9 +598,14 @@ __device__ __forceinline__ bool same_cell(f3 p, f3 ip) {
   return q.x == ip.x && q.y == ip.y && q.z == ip.z;
 }
 
-template <int FMT, bool COUNT, bool BIG>
-__global__ __launch_bounds__(64) void segment_kernel(VolumeArgs a) {
+template <int FMT, bool COUNT, bool BIG, bool MULTI>
+__global__ __launch_bounds__(64) void segment_kernel(VolumeArgs a, SegBatchTable tb) {
   using OFF = std::conditional_t<BIG, uint64_t, uint32_t>;
+  if (MULTI) {   // this workgroup's residency batch
+    a.pool = tb.pool[blockIdx.z];
+    a.codes = tb.codes[blockIdx.z];
+    a.page = tb.page[blockIdx.z];
+  }
   constexpr int U = HYD_SEG_GROUP;
   __shared__ float s_tf[4 * kMaxTf];
   __shared__ float s_lut[FMT == HYD_FMT_U8 ? 256 : 1];
@@ -749,18 +754,25 @@ bool force_offsets64() {
   return on;
 }
 
-template <int FMT>
-hipError_t launch_segments_fmt(const VolumeArgs &a, hipStream_t s) {
-  dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile);
+template <int FMT, bool MULTI>
+void launch_segments_multi(const VolumeArgs &a, const SegBatchTable &tb, hipStream_t s) {
+  dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile, MULTI ? tb.n : 1);
+  // pool_elems is the largest pool's when MULTI
   const uint64_t bytes = a.pool_elems * (uint64_t)sizeof(typename RawVox<FMT>::T);
   const bool big = bytes > (1ull << 32) || force_offsets64();
   if (a.samples) {
-    if (big) hipLaunchKernelGGL((segment_kernel<FMT, true, true>), grid, dim3(64), 0, s, a);
-    else hipLaunchKernelGGL((segment_kernel<FMT, true, false>), grid, dim3(64), 0, s, a);
+    if (big) hipLaunchKernelGGL((segment_kernel<FMT, true, true, MULTI>), grid, dim3(64), 0, s, a, tb);
+    else hipLaunchKernelGGL((segment_kernel<FMT, true, false, MULTI>), grid, dim3(64), 0, s, a, tb);
   } else {
-    if (big) hipLaunchKernelGGL((segment_kernel<FMT, false, true>), grid, dim3(64), 0, s, a);
-    else hipLaunchKernelGGL((segment_kernel<FMT, false, false>), grid, dim3(64), 0, s, a);
+    if (big) hipLaunchKernelGGL((segment_kernel<FMT, false, true, MULTI>), grid, dim3(64), 0, s, a, tb);
+    else hipLaunchKernelGGL((segment_kernel<FMT, false, false, MULTI>), grid, dim3(64), 0, s, a, tb);
   }
+}
+
+template <int FMT>
+hipError_t launch_segments_fmt(const VolumeArgs &a, hipStream_t s) {
+  SegBatchTable none{};
+  launch_segments_multi<FMT, false>(a, none, s);
   return hipGetLastError();
 }
 
@@ -795,6 +807,15 @@ hipError_t launch_segments_pass(const VolumeArgs &a, int fmt, hipStream_t s) {
     case HYD_FMT_F16: return launch_segments_fmt<HYD_FMT_F16>(a, s);
     default: return launch_segments_fmt<HYD_FMT_F32>(a, s);
   }
+}
+
+hipError_t launch_segments_batches(const VolumeArgs &a, const SegBatchTable &tb, int fmt, hipStream_t s) {
+  switch (fmt) {
+    case HYD_FMT_U8: launch_segments_multi<HYD_FMT_U8, true>(a, tb, s); break;
+    case HYD_FMT_F16: launch_segments_multi<HYD_FMT_F16, true>(a, tb, s); break;
+    default: launch_segments_multi<HYD_FMT_F32, true>(a, tb, s); break;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_segments_composite(const float4 *slots, const int32_t *nslots, int max_slots, int P, float4 *rgba,

@@ -276,6 +276,36 @@ class Context:
                C.c_void_p(nslots.data_ptr()), C.c_void_p(rgba.data_ptr()))
         return rgba, cnt
 
+    def segment_set(self, batches) -> "SegmentSet":
+        """A set of residency batches rendered by one segment launch: batches is a list of
+        (fmt, pad, pool_tensor, vaddr, page_k, page_b) with pool_tensor a device tensor
+        (render_sort_last_resident's form); uses the context's current empty-space grid."""
+        return SegmentSet(self, batches)
+
+    def render_sort_last_set(self, params: L.VolumeParams, view: L.View, W: int, H: int, sset: "SegmentSet",
+                             slots=None, nslots=None, rgba=None, count: bool = False):
+        """render_sort_last_resident with the set's batches rendered in one launch."""
+        t = self.torch
+        self.sync_stream()
+        max_slots = int(sum(sset.grid_dims))
+        if slots is None:
+            slots = t.empty((max_slots, H, W, 4), dtype=t.float32, device=self.dev)
+        check_output(slots, max_slots * H * W * 4, "segment slots")
+        if nslots is None:
+            nslots = t.zeros((H, W), dtype=t.int32, device=self.dev)
+        if rgba is None:
+            rgba = t.empty((H, W, 4), dtype=t.float32, device=self.dev)
+        check_output(rgba, H * W * 4, "sort-last output")
+        if nslots.dtype != t.int32 or not nslots.is_contiguous() or nslots.numel() < H * W:
+            raise ValueError("nslots must be a contiguous int32 tensor of H * W entries")
+        cnt = t.zeros(1, dtype=t.int64, device=self.dev) if count else None
+        L.call("hyd_segments_begin", self.h, W, H, max_slots, C.c_void_p(slots.data_ptr()))
+        L.call("hyd_segments_pass_set", self.h, sset.h, C.byref(params), C.byref(view), W, H, max_slots,
+               C.c_void_p(slots.data_ptr()), C.c_void_p(nslots.data_ptr()), C.c_void_p(cnt.data_ptr() if count else 0))
+        L.call("hyd_segments_composite", self.h, W, H, max_slots, C.c_void_p(slots.data_ptr()),
+               C.c_void_p(nslots.data_ptr()), C.c_void_p(rgba.data_ptr()))
+        return rgba, cnt
+
     def trace_volume(self, params: L.VolumeParams, view: L.View, W: int, H: int, x: int, y: int,
                      max_recs: int = 8192) -> np.ndarray:
         recs = (L.TraceRec * max_recs)()
@@ -364,6 +394,41 @@ def check_image(x, per_px: int, what: str, dev, lead: int = 1):
         raise ValueError(f"{what} must hold {lead} x {H} x {W} x {per_px} floats; got {shape}")
     return H, W
 
+
+
+class SegmentSet:
+    """hyd_segment_set: residency batches (device pools and their page tables) that one
+    segment launch renders together.  Keeps the pools referenced while it lives."""
+
+    def __init__(self, ctx, batches):
+        self._keep = []
+        arr = (L.SegmentBatch * len(batches))()
+        dims = None
+        for i, (fmt, pad, pool_t, vaddr, page_k, page_b) in enumerate(batches):
+            vaddr = np.ascontiguousarray(vaddr, np.int32)
+            page_k = np.ascontiguousarray(page_k, np.float32)
+            page_b = np.ascontiguousarray(page_b, np.float32)
+            dims = vaddr.shape[::-1] if dims is None else dims
+            if vaddr.shape[::-1] != dims:
+                raise ValueError("every batch's page table must have the same grid")
+            self._keep += [pool_t, vaddr, page_k, page_b]
+            arr[i] = L.SegmentBatch(int(fmt), int(pad), int(page_k.shape[0]), C.c_void_p(pool_t.data_ptr()),
+                                    L.ptr(vaddr, C.c_int32), L.ptr(page_k, C.c_float), L.ptr(page_b, C.c_float))
+        self.grid_dims = tuple(int(d) for d in dims)
+        h = C.c_void_p()
+        L.call("hyd_segment_set_create", ctx.h, len(batches), arr, C.byref(h))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.call("hyd_segment_set_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 class Batch:
     """A batched-render plan: many views, one persistent launch (optionally a HIP graph)."""

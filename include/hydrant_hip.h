@@ -282,6 +282,35 @@ int32_t hyd_segments_begin(hyd_ctx *ctx, int32_t width, int32_t height, int32_t 
 int32_t hyd_segments_pass(hyd_ctx *ctx, const hyd_volume_params *p, const hyd_view *view, int32_t width,
                           int32_t height, int32_t max_slots, float *slots_dev, int32_t *nslots_dev,
                           unsigned long long *samples_dev);
+/* Residency batches kept in device memory together (no counterpart in the
+ * reference, whose batches are rendered one after another, or by different
+ * ranks): a set renders every batch's segments in one launch.  A batch is a
+ * device pool (hyd_bind_pool_device's layout) and its page table
+ * (hyd_set_page_table's arguments, host memory, over the brick grid of the
+ * context's empty-space grid); every batch has the same format and page edge.
+ * The set copies the page tables and the empty-space grid at creation (a later
+ * hyd_set_empty_grid does not change it) and holds the pools' pointers, which
+ * must stay valid while the set is used. */
+typedef struct {
+  int32_t fmt;            /* HYD_FMT_* of the pool */
+  int32_t pad_edge;       /* page edge in voxels */
+  int32_t n_pages;
+  const void *pool_dev;
+  const int32_t *vaddr;   /* host, empty-grid dims: page id or -1 */
+  const float *page_k;    /* host, n_pages */
+  const float *page_b;    /* host, 3 * n_pages */
+} hyd_segment_batch;
+typedef struct hyd_segment_set hyd_segment_set;
+/* n_batches in [1, 16]. */
+int32_t hyd_segment_set_create(hyd_ctx *ctx, int32_t n_batches, const hyd_segment_batch *batches,
+                               hyd_segment_set **out);
+int32_t hyd_segment_set_destroy(hyd_segment_set *set);
+/* hyd_segments_pass for every batch of the set, in one launch (the batches'
+ * bricks are disjoint, so their slots are too): the same slots as one pass per
+ * batch with that batch bound. */
+int32_t hyd_segments_pass_set(hyd_ctx *ctx, const hyd_segment_set *set, const hyd_volume_params *p,
+                              const hyd_view *view, int32_t width, int32_t height, int32_t max_slots,
+                              float *slots_dev, int32_t *nslots_dev, unsigned long long *samples_dev);
 /* Fold the slots in ray order into RGBA; a ray stops at its first slot that no
  * pass filled (a brick never resident), as the reference stops at a page fault. */
 int32_t hyd_segments_composite(hyd_ctx *ctx, int32_t width, int32_t height, int32_t max_slots,

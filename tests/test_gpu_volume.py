@@ -143,6 +143,17 @@ def test_sort_last_segments_equal_single_march(gpu_ctx):
     assert int(cnt.item()) == int(ref["samples"].item())
     o = oracle_volume(sc).render(oracle_view(sc), 32, film=False)
     assert max_abs(rgba.cpu().numpy(), o["rgba"]) <= TOL
+    # the three batches kept in device memory and rendered by one launch: the same
+    # slots, so the same image bit for bit and the same samples
+    import torch
+
+    dev = [(fmt, pad, torch.from_numpy(pool).cuda(), vaddr, pk, pb) for fmt, pad, pool, vaddr, pk, pb in batches]
+    sset = ctx.segment_set(dev)
+    rgba_set, cnt_set = ctx.render_sort_last_set(p, view, sc.W, sc.H, sset, count=True)
+    ctx.synchronize()
+    assert torch.equal(rgba_set, rgba)
+    assert int(cnt_set.item()) == int(cnt.item())
+    sset.close()
     # only two of the three batches ever resident: rays stop at the first missing brick
     part, _ = ctx.render_sort_last(p, view, sc.W, sc.H, batches[:2], b.grid)
     vaddr2 = np.full(b.vaddr.shape, -1, np.int32)
