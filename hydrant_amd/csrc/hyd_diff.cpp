@@ -160,13 +160,9 @@ int32_t hyd_diff_backward(hyd_ctx *c, const hyd_diff_params *p, const hyd_view *
   a.bound = a.dsmax + n_rays;
   a.gtf_acc = reinterpret_cast<unsigned long long *>(base + gtf_off);
   a.tile_counter = reinterpret_cast<int32_t *>(a.gtf_acc + 4 * 256);
-  a.dsbound = reinterpret_cast<unsigned int *>(a.tile_counter + 1);
   // fixed-point headroom: the int64 accumulators hold 2^33 terms of up to 2^30 each
   const double terms = 2.0 * (double)n_rays * (double)a.K;
   a.headroom = terms > 8589934592.0 ? (int32_t)std::ceil(std::log2(terms / 8589934592.0)) : 0;
-  // a block slot gets at most one term per sample of the launch
-  const double vterms = (double)n_rays * (double)a.K;
-  a.vheadroom = vterms > 8589934592.0 ? (int32_t)std::ceil(std::log2(vterms / 8589934592.0)) : 0;
   a.n_workers = c->n_cus > 0 ? c->n_cus * 5 : 1280;   // 4-wave workgroups at 5 waves per SIMD
   HYD_CHECK_HIP(launch_diff_backward(a, c->stream));
   return HYD_OK;

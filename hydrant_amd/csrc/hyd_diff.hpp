@@ -40,8 +40,6 @@ struct DiffArgs {
   unsigned long long *gtf_acc; // [4][256] fixed-point TF-gradient sums of the launch
   int32_t *tile_counter;       // [1] work queue of the ray-order pass
   int32_t headroom;            // extra right shift of the TF fixed-point scale for very large launches
-  int32_t vheadroom;           // the same for the voxel-gradient blocks
-  unsigned int *dsbound;       // [1] max |dL/ds| of the launch (float bits; non-negative floats order as uints)
   int32_t n_workers;           // workgroups of the ray-order pass
 };
 

@@ -248,11 +248,11 @@ __device__ __forceinline__ int fixed_exponent(float b, int headroom) {
   return min(100, max(-100, 30 - E - headroom));
 }
 
-// A term already scaled (|v| < 2^30) to fixed point by truncation: one
-// conversion; the bias (under one unit, 2^-30 of the launch's bound, per term)
-// is far below the FP32 rounding of the final sums.
-__device__ __forceinline__ unsigned long long fixed_trunc(float v) {
-  return (unsigned long long)(long long)(int)v;
+// A term already scaled (|v| < 2^30) to fixed point, rounded to nearest (an
+// unbiased error of at most half a unit per term), sign-extended for the int64
+// accumulators.
+__device__ __forceinline__ unsigned long long fixed_round(float v) {
+  return (unsigned long long)(long long)(int)rintf(v);
 }
 
 // The TF-gradient scale exponent of the launch, from the prepass's partial
@@ -273,10 +273,7 @@ __global__ __launch_bounds__(256) void diff_bound_kernel(DiffArgs a) {
   __shared__ float s_red[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (blockIdx.x == 0) {
-    if (tid == 0) {
-      *a.tile_counter = 0;
-      *a.dsbound = 0u;
-    }
+    if (tid == 0) *a.tile_counter = 0;
     for (int i = tid; i < 4 * kMaxTf; i += 256) a.gtf_acc[i] = 0ull;
   }
   float mc = 0.f;
@@ -473,8 +470,6 @@ __device__ void adjoint_tile(const DiffArgs &a, const float4 *tf, unsigned long 
     }
   }
   if (in_image) a.dsmax[ray] = dmax;
-  const float wmax = wave_max(dmax);
-  if (lane == 0 && wmax > 0.f) atomicMax(a.dsbound, __float_as_uint(wmax));
 }
 
 // Ray-order adjoint: persistent workgroups of four waves; each wave claims
@@ -533,11 +528,12 @@ __device__ __forceinline__ void slab(float o, float d, float lo, float hi, float
 // 100, ...).  Per view: the pixels in the projection of the box holding the
 // brick's cells (plus a margin); the rays that reach the box go to an LDS
 // list with their sample range inside it (conservative); the list's samples
-// are split into 64 contiguous runs, one per lane; each candidate sample's
-// position and cell are recomputed with the forward's operations and samples
-// of other bricks' cells dropped.  Each corner term is added to the brick's
-// int64 fixed-point block in LDS (one scale for the launch), which is
-// converted and stored once, after the group's views, in global scratch;
+// are cut into segments of 8, one per lane; each candidate sample's position
+// and cell are recomputed with the forward's operations and samples of other
+// bricks' cells dropped.  Each corner term is added to the brick's int64
+// fixed-point block in LDS (one scale per view, from the largest |dL/ds| of
+// the rays reaching the box), folded per view into FP32 sums in registers
+// and stored once, after the group's views, in global scratch;
 // diff_combine_kernel then adds, per voxel, the block entries that
 // map to it (own brick, the high halo of the lower neighbour, and the border
 // slots that clamp onto it) in a fixed order.  The views are split into groups
@@ -548,6 +544,7 @@ constexpr int kList = 64;    // rays per scatter list
 constexpr int kBlk = kBrick + 2;                    // corner coordinates -1 .. kBrick, plus one
 constexpr int kBlk3 = kBlk * kBlk * kBlk;
 static_assert(kBlk3 == kDiffBlock, "block size shared with the host");
+constexpr int kBlkPerLane = (kBlk3 + 63) / 64;
 
 // Rays of a scatter list: setup, candidate samples k0 .. k1, first segment.
 struct ScatterList {
@@ -562,7 +559,7 @@ struct BrickBox {
 };
 
 // A lane's segment has at most 8 samples, so a register sum holds at most 8
-// terms; the launch's scale leaves 3 bits of headroom for that, so a sum stays
+// terms; the view's scale leaves 3 bits of headroom for that, so a sum stays
 // below 2^30.  Measured for 32 views at 256^2 of a 128^3 volume (K = 512):
 // lanes on single samples (neighbouring lanes adding to the same slots)
 // 12.7 ms, runs of 4 samples 5.2 ms, of 8 4.9 ms, of 16 5.8 ms; keeping the
@@ -581,7 +578,7 @@ __device__ __forceinline__ void flush_cell(unsigned long long *p, const float *A
                           kBlk * kBlk + kBlk + 1};
 #pragma unroll
   for (int q = 0; q < 8; ++q)
-    if (A[q] != 0.f) atomicAdd(p + off[q], fixed_trunc(A[q]));
+    if (A[q] != 0.f) atomicAdd(p + off[q], fixed_round(A[q]));
 }
 
 // Scatter the first n rays of the list, whose candidate samples form nseg
@@ -589,9 +586,9 @@ __device__ __forceinline__ void flush_cell(unsigned long long *p, const float *A
 // range), in rounds of 64 segments: lane l takes segment l of the round, so a
 // lane's samples are consecutive samples of one ray (which mostly share cells)
 // and its dL/ds is one 32-byte run (two 16-byte loads when K is a multiple of
-// 8).  Each owned sample's corner terms (dL/ds times the launch's scale, times
-// the weight) are summed in registers per cell and added in fixed point (int32
-// by truncation, sign-extended) to the brick's int64 block in LDS.
+// 8).  Each owned sample's corner terms (dL/ds times the view's scale, times
+// the weight) are summed in registers per cell and added in fixed point (int32,
+// rounded, sign-extended) to the brick's int64 block in LDS.
 __device__ void scatter_list(const DiffArgs &a, const DiffView &v, const ScatterList &L, int n, int nseg,
                              float scale, unsigned long long *s_acc, int lane, const BrickBox &bb) {
   const float fnx = float(a.nx), fny = float(a.ny), fnz = float(a.nz);
@@ -653,32 +650,24 @@ __device__ void scatter_list(const DiffArgs &a, const DiffView &v, const Scatter
   }
 }
 
-#ifndef HYD_SCATTER_WAVES
-#define HYD_SCATTER_WAVES 1
-#endif
-// Waves per brick workgroup, one list each.  The scatter is bound by the LDS's
-// 64-bit atomics: one wave per block measured 4.95 ms for 32 views at 256^2 of
-// a 128^3 volume (K = 512), two 5.2 ms, four 7.1 ms (more waves contend for
-// the same block).
-constexpr int kScatterWaves = HYD_SCATTER_WAVES;
-
-// Orders one wave's LDS list writes and reads (the waves of the workgroup
-// build and scatter their lists independently).
+// One wave per brick workgroup: measured 4.95 ms for 32 views at 256^2 of a
+// 128^3 volume (K = 512) against 5.2 and 7.1 ms with two and four waves
+// sharing the block (more waves contend for the same LDS slots).
+// Orders the wave's LDS list writes and reads.
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One workgroup per (brick, view group): its waves share the brick's int64
-// block and take the group's views in turn, each with its own ray list.  The
-// block's integer sums do not depend on the order of the additions, and are
-// converted to FP32 once, after the group's views.
-__global__ __launch_bounds__(64 * kScatterWaves) void diff_scatter_kernel(DiffArgs a) {
+// One workgroup (one wave) per (brick, view group).  Per view: one scale from
+// the largest |dL/ds| of the rays reaching the brick's box, the view's terms
+// in the int64 block, then the block folded into FP32 sums held in registers
+// (fixed order over the views: deterministic) and cleared.
+__global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
   __shared__ unsigned long long s_acc[kBlk3];
-  __shared__ ScatterList s_list[kScatterWaves];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  ScatterList &L = s_list[wave];
+  __shared__ ScatterList L;
+  const int lane = threadIdx.x;
   const int nbx = (a.nx + kBrick - 1) / kBrick, nby = (a.ny + kBrick - 1) / kBrick,
             nbz = (a.nz + kBrick - 1) / kBrick;
   const int bi = blockIdx.x % (nbx * nby * nbz), grp = blockIdx.x / (nbx * nby * nbz);
@@ -692,11 +681,10 @@ __global__ __launch_bounds__(64 * kScatterWaves) void diff_scatter_kernel(DiffAr
   bb.cx1 = min(bb.X0 + kBrick - 1, a.nx - 1); bb.cy1 = min(bb.Y0 + kBrick - 1, a.ny - 1);
   bb.cz1 = min(bb.Z0 + kBrick - 1, a.nz - 1);
   float *blk = a.blocks + ((size_t)grp * nbx * nby * nbz + bi) * kBlk3;
-  for (int i = tid; i < kBlk3; i += 64 * kScatterWaves) s_acc[i] = 0ull;
-  // one scale for the launch: the largest |dL/ds| times 2^e below 2^30, with
-  // headroom for every sample of the launch adding to one slot
-  const int e = fixed_exponent(__uint_as_float(*a.dsbound), a.vheadroom + kScatterBits);
-  const float scale = ldexpf(1.f, e);
+  for (int i = lane; i < kBlk3; i += 64) s_acc[i] = 0ull;
+  float acc[kBlkPerLane];   // the block's FP32 sums over this group's views; entry lane + 64 i
+#pragma unroll
+  for (int i = 0; i < kBlkPerLane; ++i) acc[i] = 0.f;
   // the box of the owned cells: floor(n p - 0.5) in [c0, c1]  <=>  p in [(c0 + 0.5) / n, (c1 + 1.5) / n)
   const float delta = 1e-4f;
   const float lo[3] = {(float(bb.cx0) + 0.5f) / float(a.nx) - delta, (float(bb.cy0) + 0.5f) / float(a.ny) - delta,
@@ -704,7 +692,7 @@ __global__ __launch_bounds__(64 * kScatterWaves) void diff_scatter_kernel(DiffAr
   const float hi[3] = {(float(bb.cx1) + 1.5f) / float(a.nx) + delta, (float(bb.cy1) + 1.5f) / float(a.ny) + delta,
                        (float(bb.cz1) + 1.5f) / float(a.nz) + delta};
   __syncthreads();
-  for (int view = v_begin + wave; view < v_end; view += kScatterWaves) {
+  for (int view = v_begin; view < v_end; ++view) {
     const DiffView &v = a.views[view];
     // pixel bounding box of the projected box (whole image if a corner is not in front)
     int px0 = 0, px1 = a.W - 1, py0 = 0, py1 = a.H - 1;
@@ -735,6 +723,17 @@ __global__ __launch_bounds__(64 * kScatterWaves) void diff_scatter_kernel(DiffAr
     const int bw = px1 - px0 + 1, bh = py1 - py0 + 1;
     if (bw <= 0 || bh <= 0) continue;
     const int nR = bw * bh;
+    // the view's scale: the largest |dL/ds| of the box's rays times 2^e below
+    // 2^30, less 3 bits for the register sums, less the bits for every listed
+    // sample adding to one slot beyond 2^33 terms
+    float B = 0.f;
+    for (int r = lane; r < nR; r += 64) B = fmaxf(B, a.dsmax[((size_t)view * a.H + py0 + r / bw) * a.W + px0 + r % bw]);
+    B = wave_max(B);
+    if (!(B > 0.f)) continue;
+    const unsigned long long items = (unsigned long long)nR * (unsigned long long)a.K;
+    const int hv = items > (1ull << 33) ? 64 - __clzll(items - 1ull) - 33 : 0;
+    const int e = fixed_exponent(B, hv + kScatterBits);
+    const float scale = ldexpf(1.f, e);
     int n = 0, total = 0;   // rays and segments in the list (wave-uniform)
     for (int base = 0; base < nR; base += 64) {
       const int rr = base + lane;
@@ -794,10 +793,25 @@ __global__ __launch_bounds__(64 * kScatterWaves) void diff_scatter_kernel(DiffAr
       scatter_list(a, v, L, n, total, scale, s_acc, lane, bb);
     }
     wave_sync();
+    const float unscale = ldexpf(1.f, -e);
+#pragma unroll
+    for (int i = 0; i < kBlkPerLane; ++i) {
+      const int idx = lane + 64 * i;
+      if (idx < kBlk3) {
+        const long long val = (long long)s_acc[idx];
+        if (val != 0) {
+          acc[i] += (float)val * unscale;
+          s_acc[idx] = 0ull;
+        }
+      }
+    }
+    wave_sync();
   }
-  __syncthreads();
-  const float unscale = ldexpf(1.f, -e);
-  for (int i = tid; i < kBlk3; i += 64 * kScatterWaves) blk[i] = (float)(long long)s_acc[i] * unscale;
+#pragma unroll
+  for (int i = 0; i < kBlkPerLane; ++i) {
+    const int idx = lane + 64 * i;
+    if (idx < kBlk3) blk[idx] = acc[i];
+  }
 }
 
 // Block entries that map to voxel coordinate v on one axis of extent n: the
@@ -866,7 +880,7 @@ hipError_t launch_diff_backward(const DiffArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(diff_adjoint_kernel, dim3(a.n_workers), dim3(256), 0, s, a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int nb = ((a.nx + kBrick - 1) / kBrick) * ((a.ny + kBrick - 1) / kBrick) * ((a.nz + kBrick - 1) / kBrick);
-  hipLaunchKernelGGL(diff_scatter_kernel, dim3(nb * a.view_groups), dim3(64 * kScatterWaves), 0, s, a);
+  hipLaunchKernelGGL(diff_scatter_kernel, dim3(nb * a.view_groups), dim3(64), 0, s, a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const size_t nvox = (size_t)a.nx * a.ny * a.nz;
   const int cg = (int)std::min<size_t>((nvox + 255) / 256, 4096);

@@ -251,8 +251,10 @@ def differentiable(n: int = 128, n_views: int = 32, W: int = 256, H: int = 256, 
                    n_steps: int = 512, ckpt_every: int = 64, sigma: float = 8.0) -> DiffWorkload:
     """Hidden n^3 FP32 blob volume (8 blobs, sigma 6-20 voxels at n = 128, scaled with n) and a
     hidden 256-entry TF of seeded smooth bumps; n_views seeded orbit views (arm.x ~ U[0, 360),
-    arm.y ~ U[-60, 60], distance 2.2).  The fit starts from a constant volume (0.25) and a grey
-    translucent TF."""
+    arm.y ~ U[-60, 60], distance 2.2).  The fit starts from a grey translucent TF and a volume
+    ramping linearly from 0.15 to 0.35 along x + y + z.  (A constant start is a stationary
+    point of the exact gradient: every sample reads the same TF pair with equal weights, so the
+    two entries stay equal and dL/dvolume stays zero; a deterministic adjoint stays there.)"""
     from .diff import smooth_tf
 
     vol = D.gen_blobs(n, 8, seed, sig=(6.0, 20.0), sigma_scale=n / 128.0, amp=(0.3, 1.0))
@@ -260,7 +262,9 @@ def differentiable(n: int = 128, n_views: int = 32, W: int = 256, H: int = 256, 
     tf = smooth_tf(256, seed + 1)
     rng = np.random.default_rng(seed + 2)
     arms = [(float(rng.uniform(0.0, 360.0)), float(rng.uniform(-60.0, 60.0)), 2.2) for _ in range(n_views)]
-    init_v = np.full((n, n, n), 0.25, np.float32)
+    ramp = np.arange(n, dtype=np.float32) / np.float32(max(n - 1, 1))
+    init_v = (np.float32(0.15) + np.float32(0.2 / 3.0) * (ramp[:, None, None] + ramp[None, :, None]
+                                                         + ramp[None, None, :])).astype(np.float32)
     init_tf = np.tile(np.array([0.5, 0.5, 0.5, 0.2], np.float32), (256, 1))
     return DiffWorkload(n, W, H, arms, hidden, tf, init_v, init_tf, n_steps, ckpt_every, sigma)
 

@@ -73,8 +73,15 @@ def test_adjoint_matches_fp64_gradient(gpu_ctx):
         ref_t += b
     ev = float(np.max(np.abs(gv.cpu().numpy() - ref_v)) / np.max(np.abs(ref_v)))
     et = float(np.max(np.abs(gt.cpu().numpy() - ref_t)) / np.max(np.abs(ref_t)))
-    print(f"adjoint vs FP64 gradient: volume rel {ev:.3g}, tf rel {et:.3g}", flush=True)
+    # each voxel's own relative error, down to gradients 1e-4 of the largest (an
+    # optimiser normalises every parameter's gradient, so small ones must be right too)
+    g = gv.cpu().numpy().astype(np.float64)
+    big = np.abs(ref_v) >= 1e-4 * np.max(np.abs(ref_v))
+    rel = np.abs(g[big] - ref_v[big]) / np.abs(ref_v[big])
+    print(f"adjoint vs FP64 gradient: volume rel {ev:.3g}, tf rel {et:.3g}; per-voxel rel p99 "
+          f"{np.quantile(rel, 0.99):.3g}, max {rel.max():.3g} over {int(big.sum())} voxels", flush=True)
     assert ev <= 1e-3 and et <= 1e-3
+    assert np.quantile(rel, 0.99) <= 1e-3
 
 
 @pytest.mark.parametrize("n_steps", [None, 98])
@@ -187,5 +194,5 @@ def test_autograd_adam_fit_reduces_loss(gpu_ctx):
         opt.step()
         losses.append(float(loss.item()))
     print(f"adam: loss {losses[0]:.4g} -> {losses[-1]:.4g}", flush=True)
-    assert losses[-1] < 0.5 * losses[0]
+    assert losses[-1] < 0.25 * losses[0]
     assert vol.grad is not None and torch.isfinite(vol.grad).all()
