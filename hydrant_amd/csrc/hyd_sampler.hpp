@@ -56,19 +56,23 @@ __device__ __forceinline__ RefTap<OFF> ref_tap(f3 x, float4 pg, int pad, OFF pag
   return t;
 }
 
-// Raw taps as loaded.  FP32 and FP16 rows are read as x pairs: the two voxels
-// at m = min(ix, pad - 2) and m + 1 in one load (8 bytes at 4-byte alignment
-// for FP32, 4 bytes at 2-byte alignment for FP16), which the vector memory
-// unit serves at the per-lane cost of one of the two single loads; the taps
-// are then v(ix) = ix == m ? lo : hi and v(jx) = jx == m + 1 ? hi : lo, which
-// also covers the clamped and integral cases (ix == jx).  u8 taps are single
-// bytes, one 32-bit register each (bytes packed into a register would make
-// each load wait for the previous ones).  Values are converted to float once,
-// when a sample uses them.
+// Raw taps as loaded.  Every row is read as an x pair: the two voxels at
+// m = min(ix, pad - 2) and m + 1 in one load (8 bytes at 4-byte alignment for
+// FP32, 4 bytes at 2-byte alignment for FP16, 2 bytes at any alignment for
+// u8), which the vector memory unit serves at the per-lane cost of one of the
+// two single loads; the taps are then v(ix) = ix == m ? lo : hi and
+// v(jx) = jx == m + 1 ? hi : lo, which also covers the clamped and integral
+// cases (ix == jx).  Each load has a register of its own (pairs of rows packed
+// into one register would make each load wait for the previous one).  A
+// grouped u8 march issues 8 samples x 4 row loads per group; with one load per
+// tap it issued 64, and the vector memory unit's per-instruction cost (one
+// cache-line lookup per distinct line a wave's lanes touch) set the time of a
+// group, not the memory latency.  Values are converted to float once, when a
+// sample uses them.
 template <int FMT> struct RawVox;
 template <> struct RawVox<HYD_FMT_F32> { using T = float; static constexpr int N = 8; };
 template <> struct RawVox<HYD_FMT_F16> { using T = uint16_t; static constexpr int N = 4; };
-template <> struct RawVox<HYD_FMT_U8> { using T = uint8_t; static constexpr int N = 8; };
+template <> struct RawVox<HYD_FMT_U8> { using T = uint8_t; static constexpr int N = 4; };
 
 template <int FMT> struct RawTaps { uint32_t v[RawVox<FMT>::N]; };
 
@@ -80,19 +84,17 @@ __device__ __forceinline__ void ref_fetch(const void *pool, const RefTap<OFF> &t
   const char *b = static_cast<const char *>(pool);
   constexpr OFF sz = (OFF)sizeof(T);
   const OFF rows[4] = {t.r00 * sz, t.r10 * sz, t.r01 * sz, t.r11 * sz};
-  if constexpr (FMT == HYD_FMT_U8) {
-    const OFF xi = (OFF)t.ix, xj = (OFF)t.jx;
+  const OFF xm = (OFF)min(t.ix, pad - 2) * sz;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      r.v[2 * k] = *reinterpret_cast<const uint8_t *>(b + (rows[k] + xi));
-      r.v[2 * k + 1] = *reinterpret_cast<const uint8_t *>(b + (rows[k] + xj));
-    }
-  } else {
-    const OFF xm = (OFF)min(t.ix, pad - 2) * sz;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if constexpr (FMT == HYD_FMT_F32) __builtin_memcpy(&r.v[2 * k], b + (rows[k] + xm), 8);
-      else __builtin_memcpy(&r.v[k], b + (rows[k] + xm), 4);
+  for (int k = 0; k < 4; ++k) {
+    if constexpr (FMT == HYD_FMT_F32) {
+      __builtin_memcpy(&r.v[2 * k], b + (rows[k] + xm), 8);
+    } else if constexpr (FMT == HYD_FMT_F16) {
+      __builtin_memcpy(&r.v[k], b + (rows[k] + xm), 4);
+    } else {
+      uint16_t pair;
+      __builtin_memcpy(&pair, b + (rows[k] + xm), 2);
+      r.v[k] = pair;
     }
   }
 }
@@ -101,14 +103,16 @@ __device__ __forceinline__ void ref_fetch(const void *pool, const RefTap<OFF> &t
 template <int FMT, class OFF>
 __device__ __forceinline__ void ref_values(const RawTaps<FMT> &r, const RefTap<OFF> &t, int pad, const float *lut,
                                            float (&v)[8]) {
-  if constexpr (FMT == HYD_FMT_U8) {
+  const int m = min(t.ix, pad - 2);
+  const bool i_lo = t.ix == m, j_hi = t.jx == m + 1;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = lut[r.v[k]];
-  } else {
-    const int m = min(t.ix, pad - 2);
-    const bool i_lo = t.ix == m, j_hi = t.jx == m + 1;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 4; ++k) {
+    if constexpr (FMT == HYD_FMT_U8) {
+      // select the bytes first: two table reads per row, as with single loads
+      const uint32_t lo = r.v[k] & 0xffu, hi = r.v[k] >> 8;
+      v[2 * k] = lut[i_lo ? lo : hi];
+      v[2 * k + 1] = lut[j_hi ? hi : lo];
+    } else {
       float lo, hi;
       if constexpr (FMT == HYD_FMT_F32) {
         lo = __uint_as_float(r.v[2 * k]);

@@ -140,9 +140,10 @@ struct MarchState {
 // (volume_shader.cu:52-70, plus the density scale on the TF alpha).  theta and
 // phi are carried only for a film (the fetch pass exports them); without one
 // they are never observed and THETA = false leaves them out.
-template <bool WEIGHTED = false, bool THETA = true>
-__device__ __forceinline__ void shade_composite_impl(const VolumeArgs &a, const float *tf, float s, int pgid,
-                                                     MarchState &st, float w) {
+// shade_sample gives the premultiplied colour of a sample (it does not depend
+// on the ray's state), composite_ub adds it behind what the ray holds.
+template <bool WEIGHTED = false>
+__device__ __forceinline__ float4 shade_sample(const VolumeArgs &a, const float *tf, float s, int pgid, float w) {
   float4 ub = tf_sample(tf, a.tf_n, s);
   ub.w = ub.w * a.density;
   if (WEIGHTED) ub.w = ub.w * w;
@@ -155,6 +156,11 @@ __device__ __forceinline__ void shade_composite_impl(const VolumeArgs &a, const 
   }
   const float aw = ub.w;
   ub.x = ub.x * aw; ub.y = ub.y * aw; ub.z = ub.z * aw;
+  return ub;
+}
+
+template <bool THETA = true>
+__device__ __forceinline__ void composite_ub(float4 ub, MarchState &st) {
   if (THETA) {
     st.theta.x = st.theta.x + ub.x * st.phi;
     st.theta.y = st.theta.y + ub.y * st.phi;
@@ -166,6 +172,12 @@ __device__ __forceinline__ void shade_composite_impl(const VolumeArgs &a, const 
   st.v.y = st.v.y + ub.y * t;
   st.v.z = st.v.z + ub.z * t;
   st.v.w = st.v.w + ub.w * t;
+}
+
+template <bool WEIGHTED = false, bool THETA = true>
+__device__ __forceinline__ void shade_composite_impl(const VolumeArgs &a, const float *tf, float s, int pgid,
+                                                     MarchState &st, float w) {
+  composite_ub<THETA>(shade_sample<WEIGHTED>(a, tf, s, pgid, w), st);
 }
 
 __device__ __forceinline__ void shade_composite(const VolumeArgs &a, const float *tf, float s, int pgid,
@@ -401,18 +413,19 @@ __device__ void march_reference_grouped(const VolumeArgs &a, const float *tf, co
   st.step_size = step_size;
 }
 
-// The reference march of one ray by kCoopLanes lanes that hold identical state
-// (lane h of the ray's lane group): per group of kCoopLanes consecutive samples
+// The reference march of one ray by G lanes that hold identical state
+// (lane h of the ray's lane group): per group of G consecutive samples
 // of one brick, lane h resolves sample h (taps, loads, interpolation) and
 // every lane composites the group's values in order, taken from their lanes,
 // stopping where the step escalates.  Positions come from the same additions
 // as the one-lane march and each value from the same taps, so the ray's
 // samples, values and composite equal march_reference_grouped's bit for bit;
-// a long ray's chain of dependent loads is cut kCoopLanes-fold.
-template <int FMT, bool COUNT, bool THETA, class OFF>
+// a long ray's chain of dependent loads is cut G-fold.  Lane h also shades
+// its sample (the colour does not depend on the ray's state), so the in-order
+// part is the compositing arithmetic alone.
+template <int FMT, bool COUNT, bool THETA, class OFF, int G>
 __device__ void march_reference_coop(const VolumeArgs &a, const float *tf, const float *lut, MarchState &st, int h,
                                      unsigned &count) {
-  constexpr int G = kCoopLanes;
   const float cdu = 1.f / comp_max(abs3(st.d));
   const f3 d = st.d;
   const f3 invr = recip3(d);
@@ -466,14 +479,14 @@ __device__ void march_reference_coop(const VolumeArgs &a, const float *tf, const
         okh = ok[j];
       }
     }
-    float sh = 0.f;
+    float4 ubh = make_float4(0.f, 0.f, 0.f, 0.f);
     if (okh) {
       const RefTap<OFF> t = ref_tap<OFF>(sub(ph, ip), pgd, a.pad, page_base);
       RawTaps<FMT> raw;
       ref_fetch<FMT, OFF>(a.pool, t, a.pad, raw);
       float cur[8];
       ref_values<FMT, OFF>(raw, t, a.pad, lut, cur);
-      sh = ref_interp<OFF>(cur, t);
+      ubh = shade_sample<false>(a, tf, ref_interp<OFF>(cur, t), pgid, 1.f);
     }
     // composite in order; stop at the sample that escalates
     int last = 0, last_ss = step_size;
@@ -481,9 +494,10 @@ __device__ void march_reference_coop(const VolumeArgs &a, const float *tf, const
     bool live = true;
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-      const float s = __shfl(sh, j, G);
+      const float4 ub = make_float4(__shfl(ubh.x, j, G), __shfl(ubh.y, j, G), __shfl(ubh.z, j, G),
+                                    __shfl(ubh.w, j, G));
       if (ok[j] && live) {
-        shade_composite_impl<false, THETA>(a, tf, s, pgid, st, 1.f);
+        composite_ub<THETA>(ub, st);
         if (COUNT && h == 0) ++count;
         last = j;
         plast = p[j];
@@ -681,7 +695,7 @@ __global__ __launch_bounds__(64) void volume_kernel(VolumeArgs a) {
     const int r = lane / kCoopLanes, h = lane % kCoopLanes;
     if (r < __popcll(hm)) {
       MarchState c = s_st[r];
-      march_reference_coop<FMT, COUNT, THETA, OFF>(a, s_tf, s_lut, c, h, count);
+      march_reference_coop<FMT, COUNT, THETA, OFF, kCoopLanes>(a, s_tf, s_lut, c, h, count);
       if (h == 0) store_pixel(a, s_idx[r], c);
     }
   }

@@ -34,6 +34,11 @@ def _compare(ctx, sc, trace_pixels=(), check_film=True):
     exact = sc.schedule == L.SCHED_REFERENCE
     assert err <= (0.0 if exact else TOL)
     assert g_samples == r["samples"]
+    # the image-only kernel (no film, no theta/phi) renders the film frame's image
+    ro = ctx.render_volume(p, view, sc.W, sc.H, film=False, count=True)
+    ctx.synchronize()
+    assert np.array_equal(ro["rgba"].cpu().numpy(), g_rgba)
+    assert int(ro["samples"].item()) == g_samples
     if check_film:
         for k in ("v", "theta", "phi", "o"):
             assert max_abs(g[k], o[k]) <= (0.0 if exact else TOL), k
