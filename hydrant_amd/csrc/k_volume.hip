@@ -414,18 +414,19 @@ __device__ void march_reference_grouped(const VolumeArgs &a, const float *tf, co
 }
 
 // The reference march of one ray by G lanes that hold identical state
-// (lane h of the ray's lane group): per group of G consecutive samples
-// of one brick, lane h resolves sample h (taps, loads, interpolation) and
-// every lane composites the group's values in order, taken from their lanes,
-// stopping where the step escalates.  Positions come from the same additions
-// as the one-lane march and each value from the same taps, so the ray's
-// samples, values and composite equal march_reference_grouped's bit for bit;
-// a long ray's chain of dependent loads is cut G-fold.  Lane h also shades
-// its sample (the colour does not depend on the ray's state), so the in-order
-// part is the compositing arithmetic alone.
-template <int FMT, bool COUNT, bool THETA, class OFF, int G>
+// (lane h of the ray's lane group): per group of G * UL consecutive samples
+// of one brick, lane h resolves samples h, h + G, ... (taps, loads issued
+// together, interpolation, shading: the colour does not depend on the ray's
+// state) and every lane composites the group's colours in order, taken from
+// their lanes, stopping where the step escalates.  Positions come from the
+// same additions as the one-lane march and each value from the same taps, so
+// the ray's samples, values and composite equal march_reference_grouped's bit
+// for bit.  A lane's instruction stream per group holds UL samples' sampling
+// and shading instead of G * UL, which is what a long ray's time is made of.
+template <int FMT, bool COUNT, bool THETA, class OFF, int G, int UL = 1>
 __device__ void march_reference_coop(const VolumeArgs &a, const float *tf, const float *lut, MarchState &st, int h,
                                      unsigned &count) {
+  constexpr int N = G * UL;
   const float cdu = 1.f / comp_max(abs3(st.d));
   const f3 d = st.d;
   const f3 invr = recip3(d);
@@ -459,43 +460,53 @@ __device__ void march_reference_coop(const VolumeArgs &a, const float *tf, const
     }
     const OFF page_base = (OFF)pgid * pad3;
     const f3 inc = muls(muls(d, a.step), float(step_size));
-    f3 p[G];
-    bool ok[G];
+    f3 p[N];
+    bool ok[N];
     p[0] = o;
     ok[0] = true;
 #pragma unroll
-    for (int j = 1; j < G; ++j) {
+    for (int j = 1; j < N; ++j) {
       p[j] = add(p[j - 1], inc);
       const f3 q = floor3(p[j]);
       ok[j] = ok[j - 1] && nsteps - j * step_size > 0 && q.x == ip.x && q.y == ip.y && q.z == ip.z;
     }
-    // this lane's sample
-    f3 ph = p[0];
-    bool okh = ok[0];
+    // this lane's samples h + G * k: taps and loads first, then values and colours
+    RefTap<OFF> t[UL];
+    RawTaps<FMT> raw[UL];
+    bool okh[UL];
 #pragma unroll
-    for (int j = 1; j < G; ++j) {
-      if (h == j) {
-        ph = p[j];
-        okh = ok[j];
+    for (int k = 0; k < UL; ++k) {
+      f3 ph = p[G * k];
+      okh[k] = ok[G * k];
+#pragma unroll
+      for (int q = 1; q < G; ++q) {
+        if (h == q) {
+          ph = p[G * k + q];
+          okh[k] = ok[G * k + q];
+        }
       }
+      t[k] = ref_tap<OFF>(sub(ph, ip), pgd, a.pad, page_base);
+      if (okh[k]) ref_fetch<FMT, OFF>(a.pool, t[k], a.pad, raw[k]);
     }
-    float4 ubh = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (okh) {
-      const RefTap<OFF> t = ref_tap<OFF>(sub(ph, ip), pgd, a.pad, page_base);
-      RawTaps<FMT> raw;
-      ref_fetch<FMT, OFF>(a.pool, t, a.pad, raw);
-      float cur[8];
-      ref_values<FMT, OFF>(raw, t, a.pad, lut, cur);
-      ubh = shade_sample<false>(a, tf, ref_interp<OFF>(cur, t), pgid, 1.f);
+    float4 ubh[UL];
+#pragma unroll
+    for (int k = 0; k < UL; ++k) {
+      ubh[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (okh[k]) {
+        float cur[8];
+        ref_values<FMT, OFF>(raw[k], t[k], a.pad, lut, cur);
+        ubh[k] = shade_sample<false>(a, tf, ref_interp<OFF>(cur, t[k]), pgid, 1.f);
+      }
     }
     // composite in order; stop at the sample that escalates
     int last = 0, last_ss = step_size;
     f3 plast = p[0];
     bool live = true;
 #pragma unroll
-    for (int j = 0; j < G; ++j) {
-      const float4 ub = make_float4(__shfl(ubh.x, j, G), __shfl(ubh.y, j, G), __shfl(ubh.z, j, G),
-                                    __shfl(ubh.w, j, G));
+    for (int j = 0; j < N; ++j) {
+      const float4 &uk = ubh[j / G];
+      const float4 ub = make_float4(__shfl(uk.x, j % G, G), __shfl(uk.y, j % G, G), __shfl(uk.z, j % G, G),
+                                    __shfl(uk.w, j % G, G));
       if (ok[j] && live) {
         composite_ub<THETA>(ub, st);
         if (COUNT && h == 0) ++count;
@@ -620,8 +631,8 @@ __device__ __forceinline__ void load_pixel(const VolumeArgs &a, size_t idx, Marc
 // Samples whose tap loads a reference-schedule group issues together, per
 // brick format.  Measured on the 512^2 frames: FP32 (256^3 single brick) takes
 // 0.54 ms with 2 (105 registers, 4 waves per SIMD hold the whole frame) and
-// 0.64 ms with 4 (3 waves per SIMD); u8 (1024^3 paged, whose few long rays set
-// the frame) 0.58 ms with 8 and 0.68 ms with 2.
+// 0.64 ms with 4 (3 waves per SIMD); u8 (1024^3 paged, 4 lanes per ray) 0.357,
+// 0.354 and 0.350 ms with 4, 8 and 12.
 #ifndef HYD_REF_GROUP_U8
 #define HYD_REF_GROUP_U8 8
 #endif
@@ -635,14 +646,37 @@ template <int FMT> constexpr int ref_group() {
   return FMT == HYD_FMT_U8 ? HYD_REF_GROUP_U8 : FMT == HYD_FMT_F16 ? HYD_REF_GROUP_F16 : HYD_REF_GROUP_F32;
 }
 
+// Lanes per ray of the reference-schedule frame, per brick format (a wave
+// then covers an 8 x 8/LANES tile; LANES > 1 marches with
+// march_reference_coop, ref_group / LANES samples per lane and group).  A
+// frame whose time is the chain of its longest rays gains: the paged u8 frame
+// takes 0.52 ms with 1 lane (groups of 8 on one lane, the last rays handed to
+// 4 lanes each), 0.405 with 2, 0.354 with 4 and 0.42 with 8.  A frame bound by
+// instruction issue loses, since every lane of a ray repeats the positions
+// and the compositing: the plain FP32 frame takes 0.55 ms with 1 lane, 0.59
+// with 2 and 0.69 with 4.
+#ifndef HYD_REF_LANES_U8
+#define HYD_REF_LANES_U8 4
+#endif
+#ifndef HYD_REF_LANES_F16
+#define HYD_REF_LANES_F16 1
+#endif
+#ifndef HYD_REF_LANES_F32
+#define HYD_REF_LANES_F32 1
+#endif
+template <int FMT> constexpr int ref_lanes() {
+  return FMT == HYD_FMT_U8 ? HYD_REF_LANES_U8 : FMT == HYD_FMT_F16 ? HYD_REF_LANES_F16 : HYD_REF_LANES_F32;
+}
+
 // Render modes of volume_kernel: emit into an RGBA image only, emit into a film
 // (and optionally RGBA), or resume a film.
 enum { kEmitRgba = 0, kEmitFilm = 1, kResume = 2 };
 
-template <int FMT, bool COUNT, int MODE, bool BIG>
+template <int FMT, bool COUNT, int MODE, bool BIG, int LANES>
 __global__ __launch_bounds__(64) void volume_kernel(VolumeArgs a) {
   constexpr bool RESUME = MODE == kResume;
   constexpr bool THETA = MODE != kEmitRgba;
+  static_assert(LANES == 1 || (kTile % LANES == 0 && ref_group<FMT>() % LANES == 0), "lanes per ray");
   __shared__ float s_tf[4 * kMaxTf];
   __shared__ float s_lut[FMT == HYD_FMT_U8 ? 256 : 1];
   __shared__ unsigned long long s_count;
@@ -653,8 +687,9 @@ __global__ __launch_bounds__(64) void volume_kernel(VolumeArgs a) {
   __syncthreads();
 
   using OFF = std::conditional_t<BIG, uint64_t, uint32_t>;
-  const int x = blockIdx.x * kTile + (threadIdx.x % kTile);
-  const int y = blockIdx.y * kTile + (threadIdx.x / kTile);
+  const int ray = (int)threadIdx.x / LANES, h = (int)threadIdx.x % LANES;
+  const int x = blockIdx.x * kTile + (ray % kTile);
+  const int y = blockIdx.y * (kTile / LANES) + (ray / kTile);
   unsigned count = 0;
   bool handed = false;   // this lane's ray went to the cooperative march
   MarchState st;
@@ -669,12 +704,14 @@ __global__ __launch_bounds__(64) void volume_kernel(VolumeArgs a) {
       live = emit(a, x, y, st);
     }
     if (live) {
-      if (a.schedule == HYD_SCHED_LATTICE)
+      if (LANES > 1)   // reference schedule only (the launch sees to it)
+        march_reference_coop<FMT, COUNT, THETA, OFF, LANES, ref_group<FMT>() / LANES>(a, s_tf, s_lut, st, h, count);
+      else if (a.schedule == HYD_SCHED_LATTICE)
         march_lattice<FMT, COUNT, THETA>(a, s_tf, s_lut, st, count);
       else
         march_reference_grouped<FMT, ref_group<FMT>(), COUNT, THETA, OFF>(a, s_tf, s_lut, st, count, handed);
     }
-    if (!handed) store_pixel(a, idx, st);
+    if (!handed && h == 0) store_pixel(a, idx, st);
   }
   // the rays handed over: kCoopLanes lanes each, with the state copied through LDS
   const unsigned long long hm = __ballot(handed);
@@ -889,10 +926,17 @@ hipError_t launch_segments_fmt(const VolumeArgs &a, hipStream_t s);
 
 template <int FMT, int MODE, bool BIG>
 void launch_mode(const VolumeArgs &a, hipStream_t s) {
-  dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile);
   dim3 block(kTile * kTile);
-  if (a.samples) hipLaunchKernelGGL((volume_kernel<FMT, true, MODE, BIG>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((volume_kernel<FMT, false, MODE, BIG>), grid, block, 0, s, a);
+  constexpr int L = ref_lanes<FMT>();
+  if (L > 1 && a.schedule == HYD_SCHED_REFERENCE) {
+    dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile / L - 1) / (kTile / L));
+    if (a.samples) hipLaunchKernelGGL((volume_kernel<FMT, true, MODE, BIG, L>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((volume_kernel<FMT, false, MODE, BIG, L>), grid, block, 0, s, a);
+  } else {
+    dim3 grid((a.W + kTile - 1) / kTile, (a.H + kTile - 1) / kTile);
+    if (a.samples) hipLaunchKernelGGL((volume_kernel<FMT, true, MODE, BIG, 1>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((volume_kernel<FMT, false, MODE, BIG, 1>), grid, block, 0, s, a);
+  }
 }
 
 template <int FMT, bool BIG>
