@@ -151,6 +151,7 @@ def bench_paged(torch, args):
     return {"ms_per_frame": round(ms, 4), "kernel_ms": round(k_ms, 4), "samples_per_frame": n_samples,
             "msamples_per_s": round(n_samples / (ms * 1e-3) / 1e6, 3),
             "achieved_gbs": round(alg / (k_ms * 1e-3) / 1e9, 1),
+            "roofline_frac": round(alg / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "alg_bytes_rule": "samples x 8 taps x 1 B + pixels x 16 B",
             "resident_pages": int((sc.bricked.vaddr >= 0).sum()), "bricks": int(sc.bricked.vaddr.size),
             "volume": 1024, "res": sc.W, "gen_s": round(gen_s, 1), "scene": sc, "rgba": rgba}
@@ -201,9 +202,13 @@ def bench_paged_sort_last(torch, args):
     wall_seq, _ = time_kernel(torch, step_seq, args.steps, args.warmup)
     wall, ev = time_kernel(torch, step, args.steps, args.warmup)
     ms = wall * 1e3 / args.steps
+    alg = n_samples * 8 * 1 + sc.W * sc.H * 16
     res = {"ms_per_frame": round(ms, 4), "samples_per_frame": n_samples,
            "sequential_batches_ms_per_frame": round(wall_seq * 1e3 / args.steps, 4),
            "msamples_per_s": round(n_samples / (ms * 1e-3) / 1e6, 3),
+           "achieved_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+           "roofline_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "alg_bytes_rule": "samples x 8 taps x 1 B + pixels x 16 B, over the whole frame (segments, clear, fold)",
            "batches": len(dev_batches), "pages_per_batch": per, "bricks": int(b.vaddr.size),
            "nonempty_bricks": int(b.n_nonempty), "max_slots": max_slots, "volume": 1024, "res": sc.W,
            "gen_s": round(gen_s, 1), "scene": sc, "rgba": rgba}
@@ -252,6 +257,7 @@ def bench_iso(torch, args):
     return {"ms_per_frame": round(ms, 4), "kernel_ms": round(k_ms, 4), "samples_per_frame": n_samples,
             "msamples_per_s": round(n_samples / (ms * 1e-3) / 1e6, 3),
             "achieved_gbs": round(alg / (k_ms * 1e-3) / 1e9, 1),
+            "roofline_frac": round(alg / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "alg_bytes_rule": "samples x 8 taps x 2 B + pixels x 24 B",
             "hits": hits, "volume": 512, "res": sc.W, "scene": sc, "params": p, "px": px}
 
@@ -619,6 +625,7 @@ def main():
     plain_info = {"msamples_per_s": round(plain["msamples_per_s"], 3), "ms_per_frame": round(plain["ms_per_step"], 4),
                   "samples_per_frame": plain["n_samples"], "kernel": "volume_kernel<f32> (reference schedule)",
                   "kernel_ms": round(plain["kernel_ms"], 4), "achieved_gbs": round(plain["achieved_gbs"], 1),
+                  "roofline_frac": round(plain["achieved_gbs"] / HBM_PEAK_GBS, 4),
                   "alg_bytes_rule": "samples x 8 taps x 4 B + pixels x 16 B"}
     if head is batched:
         workload = ("batched_fp16: 96 views (seeded resolutions in {256,384,512,768,1024}^2, seeded orbit "

@@ -111,6 +111,22 @@ struct Ctx {
 };
 
 // Flattened kernel arguments for the volume march (passed by value).
+// Tile order of a frame launch (one 8x8 tile per workgroup, grid x = tile
+// column): row r of the tile grid is rotated by HYD_TILE_ROTATE * r tiles, a
+// bijection on each row.  Workgroups are dealt round-robin over the 8 XCDs and
+// then over each XCD's 32 CUs, so one CU receives workgroups 256 apart; in a
+// 64-tile-wide grid (512 pixels) those are the tiles of one image column,
+// every fourth tile row, and the CUs that hold the centre columns, whose rays
+// are the longest, set the frame.  The rotation spreads a CU's tiles over 16
+// columns.  512^2 frames: plain FP32 0.551 -> 0.528 ms, paged u8 0.351 ->
+// 0.317 ms (rotations 3, 5, 9, 11, 23: 0.531-0.557 and 0.316-0.319 ms).
+#ifndef HYD_TILE_ROTATE
+#define HYD_TILE_ROTATE 7
+#endif
+__device__ __forceinline__ int rotated_tile_x() {
+  return (int)((blockIdx.x + (unsigned)HYD_TILE_ROTATE * blockIdx.y) % gridDim.x);
+}
+
 struct VolumeArgs {
   // shader
   f3 bmin, bmax;

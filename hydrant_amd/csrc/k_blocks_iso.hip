@@ -273,7 +273,7 @@ __global__ __launch_bounds__(64) void iso_march_grouped_kernel(IsoArgs a) {
   if (FMT == HYD_FMT_U8)
     for (int i = threadIdx.x; i < 256; i += blockDim.x) s_lut[i] = a.u8_lut[i];
   __syncthreads();
-  const int x = blockIdx.x * kTile + (threadIdx.x % kTile);
+  const int x = rotated_tile_x() * kTile + (threadIdx.x % kTile);
   const int y = blockIdx.y * kTile + (threadIdx.x / kTile);
   if (x >= a.W || y >= a.H) return;
   unsigned count = 0;

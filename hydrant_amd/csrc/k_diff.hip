@@ -170,7 +170,7 @@ __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs a) {
   if (COUNT && threadIdx.x == 0) s_count = 0;
   __syncthreads();
   const int view = blockIdx.z;
-  const int x = blockIdx.x * kTile + (threadIdx.x % kTile);
+  const int x = rotated_tile_x() * kTile + (threadIdx.x % kTile);
   const int y = blockIdx.y * kTile + (threadIdx.x / kTile);
   unsigned count = 0;
   if (x < a.W && y < a.H) {

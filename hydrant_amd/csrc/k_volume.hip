@@ -688,7 +688,8 @@ __global__ __launch_bounds__(64) void volume_kernel(VolumeArgs a) {
 
   using OFF = std::conditional_t<BIG, uint64_t, uint32_t>;
   const int ray = (int)threadIdx.x / LANES, h = (int)threadIdx.x % LANES;
-  const int x = blockIdx.x * kTile + (ray % kTile);
+  const int tx = rotated_tile_x();
+  const int x = tx * kTile + (ray % kTile);
   const int y = blockIdx.y * (kTile / LANES) + (ray / kTile);
   unsigned count = 0;
   bool handed = false;   // this lane's ray went to the cooperative march
@@ -810,7 +811,7 @@ __global__ __launch_bounds__(64) void segment_kernel(VolumeArgs a, SegBatchTable
     for (int i = threadIdx.x; i < 256; i += blockDim.x) s_lut[i] = a.u8_lut[i];
   if (COUNT && threadIdx.x == 0) s_count = 0;
   __syncthreads();
-  const int x = blockIdx.x * kTile + (threadIdx.x % kTile);
+  const int x = rotated_tile_x() * kTile + (threadIdx.x % kTile);
   const int y = blockIdx.y * kTile + (threadIdx.x / kTile);
   unsigned count = 0;
   if (x < a.W && y < a.H) {
