@@ -664,14 +664,27 @@ __device__ __forceinline__ void wave_sync() {
 // the largest |dL/ds| of the rays reaching the brick's box, the view's terms
 // in the int64 block, then the block folded into FP32 sums held in registers
 // (fixed order over the views: deterministic) and cleared.
+#ifndef HYD_SCATTER_SHIFT
+#define HYD_SCATTER_SHIFT 1
+#endif
 __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
   __shared__ unsigned long long s_acc[kBlk3];
   __shared__ ScatterList L;
   const int lane = threadIdx.x;
   const int nbx = (a.nx + kBrick - 1) / kBrick, nby = (a.ny + kBrick - 1) / kBrick,
             nbz = (a.nz + kBrick - 1) / kBrick;
-  const int bi = blockIdx.x % (nbx * nby * nbz), grp = blockIdx.x / (nbx * nby * nbz);
-  const int bx = bi % nbx, by = (bi / nbx) % nby, bz = bi / (nbx * nby);
+  // brick order: each z slice of the brick grid is shifted by (5, 3) * z bricks
+  // in x and y, so the workgroups a CU receives (256 apart in dispatch order,
+  // otherwise one z column of bricks) spread over the volume's central and
+  // outer bricks
+  const int bl = blockIdx.x % (nbx * nby * nbz), grp = blockIdx.x / (nbx * nby * nbz);
+  const int bz = bl / (nbx * nby);
+#if HYD_SCATTER_SHIFT
+  const int bx = (bl % nbx + 5 * bz) % nbx, by = ((bl / nbx) % nby + 3 * bz) % nby;
+#else
+  const int bx = bl % nbx, by = (bl / nbx) % nby;
+#endif
+  const int bi = (bz * nby + by) * nbx + bx;
   // views of this workgroup: a contiguous share of the launch's views
   const int v_begin = (int)((long long)a.n_views * grp / a.view_groups);
   const int v_end = (int)((long long)a.n_views * (grp + 1) / a.view_groups);
