@@ -72,6 +72,15 @@ def test_plain_volume_density_one(gpu_ctx):
     _compare(gpu_ctx, sc)
 
 
+@pytest.mark.parametrize("fmt", [L.FMT_F32, L.FMT_U8])
+def test_ragged_frame_and_escalation(gpu_ctx, fmt):
+    """A frame whose width and height are not multiples of the tile (partial tiles
+    in the last column and row of the rotated tile grid; u8 frames run several
+    lanes per ray), at density 1, so the step escalates inside groups."""
+    sc = W.plain_volume(n=40, W=67, H=53, fmt=fmt, density=1.0)
+    _compare(gpu_ctx, sc, trace_pixels=[(33, 26), (66, 52)])
+
+
 @pytest.mark.slow
 def test_plain_volume_full(gpu_ctx):
     sc = W.plain_volume()   # 256^3 FP32 brick, 512x512
