@@ -664,9 +664,6 @@ __device__ __forceinline__ void wave_sync() {
 // the largest |dL/ds| of the rays reaching the brick's box, the view's terms
 // in the int64 block, then the block folded into FP32 sums held in registers
 // (fixed order over the views: deterministic) and cleared.
-#ifndef HYD_SCATTER_SHIFT
-#define HYD_SCATTER_SHIFT 1
-#endif
 __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
   __shared__ unsigned long long s_acc[kBlk3];
   __shared__ ScatterList L;
@@ -679,11 +676,7 @@ __global__ __launch_bounds__(64) void diff_scatter_kernel(DiffArgs a) {
   // outer bricks
   const int bl = blockIdx.x % (nbx * nby * nbz), grp = blockIdx.x / (nbx * nby * nbz);
   const int bz = bl / (nbx * nby);
-#if HYD_SCATTER_SHIFT
   const int bx = (bl % nbx + 5 * bz) % nbx, by = ((bl / nbx) % nby + 3 * bz) % nby;
-#else
-  const int bx = bl % nbx, by = (bl / nbx) % nby;
-#endif
   const int bi = (bz * nby + by) * nbx + bx;
   // views of this workgroup: a contiguous share of the launch's views
   const int v_begin = (int)((long long)a.n_views * grp / a.view_groups);
