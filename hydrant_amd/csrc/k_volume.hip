@@ -14,6 +14,7 @@
 
 #include <cstdlib>
 #include <type_traits>
+#include <utility>
 
 #include "hyd_internal.hpp"
 #include "hyd_sampler.hpp"
@@ -413,6 +414,26 @@ __device__ void march_reference_grouped(const VolumeArgs &a, const float *tf, co
   st.step_size = step_size;
 }
 
+// Calls f(std::integral_constant<int, j>) for j = 0 .. N-1, so that the body
+// sees j as a constant (DPP controls must be).
+template <int... I, class F>
+__device__ __forceinline__ void static_for_impl(std::integer_sequence<int, I...>, F &&f) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F> __device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(std::make_integer_sequence<int, N>{}, f);
+}
+
+// Lane J of each group of G lanes, to every lane of the group.  Groups of 4
+// are the hardware quads: one DPP quad_perm move (a VALU operation) instead
+// of an LDS permute.
+template <int G, int J> __device__ __forceinline__ float group_bcast(float v) {
+  if constexpr (G == 4)
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), J * 0x55, 0xF, 0xF, false));
+  else
+    return __shfl(v, J, G);
+}
+
 // The reference march of one ray by G lanes that hold identical state
 // (lane h of the ray's lane group): per group of G * UL consecutive samples
 // of one brick, lane h resolves samples h, h + G, ... (taps, loads issued
@@ -502,11 +523,11 @@ __device__ void march_reference_coop(const VolumeArgs &a, const float *tf, const
     int last = 0, last_ss = step_size;
     f3 plast = p[0];
     bool live = true;
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
+    static_for<N>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
       const float4 &uk = ubh[j / G];
-      const float4 ub = make_float4(__shfl(uk.x, j % G, G), __shfl(uk.y, j % G, G), __shfl(uk.z, j % G, G),
-                                    __shfl(uk.w, j % G, G));
+      const float4 ub = make_float4(group_bcast<G, j % G>(uk.x), group_bcast<G, j % G>(uk.y),
+                                    group_bcast<G, j % G>(uk.z), group_bcast<G, j % G>(uk.w));
       if (ok[j] && live) {
         composite_ub<THETA>(ub, st);
         if (COUNT && h == 0) ++count;
@@ -519,7 +540,7 @@ __device__ void march_reference_coop(const VolumeArgs &a, const float *tf, const
         }
         if (last_ss != step_size) live = false;
       }
-    }
+    });
     o = add(plast, muls(muls(d, a.step), float(last_ss)));
     nsteps -= last * step_size + last_ss;
     step_size = last_ss;
