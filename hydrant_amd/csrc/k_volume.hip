@@ -290,8 +290,11 @@ __device__ void march_reference(const VolumeArgs &a, const float *tf, const floa
 // follow from the current step size alone (o += (d * step) * step_size, the
 // same binary32 operations one at a time), so a group of U samples resolves
 // its positions, tap offsets and weights first, issues the loads of every
-// sample whose cell differs from its predecessor's, and then composites the
-// samples in order.  Escalation changes the step size; a group stops at the
+// sample, and then composites the samples in order.  (Skipping the loads of a
+// sample whose cell repeats its predecessor's saved memory work, but its
+// branches and register copies cost more instructions than the loads: the
+// 512^2 FP32 frame took 0.534 ms with that reuse and groups of 2, 0.509 /
+// 0.499 / 0.607 ms without it and groups of 3 / 4 / 5.)  Escalation changes the step size; a group stops at the
 // sample that escalates (the ones after it were placed with the old step and
 // are dropped), so positions, taps, values and step budget are those of the
 // one-sample-at-a-time march, bit for bit.  Samples are also cut at the
@@ -322,9 +325,6 @@ __device__ void march_reference_grouped(const VolumeArgs &a, const float *tf, co
   int nsteps = st.nsteps;
   int step_size = st.step_size;
   const OFF pad3 = (OFF)a.pad * (OFF)a.pad * (OFF)a.pad;
-  // the last composited sample's taps: first and last tap offset, values
-  OFF c0 = ~(OFF)0, c7 = ~(OFF)0;
-  float cur[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int cpg = -1;
   float4 pgd = make_float4(0.f, 0.f, 0.f, 0.f);
   BrickLookup bl;
@@ -367,19 +367,13 @@ __device__ void march_reference_grouped(const VolumeArgs &a, const float *tf, co
       const f3 q = floor3(p[j]);
       ok[j] = ok[j - 1] && nsteps - j * step_size > 0 && q.x == ip.x && q.y == ip.y && q.z == ip.z;
     }
-    // taps, and the loads of every sample whose cell is not its predecessor's
+    // taps and loads of every sample of the group
     RefTap<OFF> t[U];
     RawTaps<FMT> raw[U];
-    bool fresh[U];
-    OFF p0 = c0, p7 = c7;
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       t[j] = ref_tap<OFF>(sub(p[j], ip), pgd, a.pad, page_base);
-      const OFF i0 = t[j].r00 + (OFF)t[j].ix, i7 = t[j].r11 + (OFF)t[j].jx;
-      fresh[j] = i0 != p0 || i7 != p7;
-      p0 = i0;
-      p7 = i7;
-      if (ok[j] && fresh[j]) ref_fetch<FMT, OFF>(a.pool, t[j], a.pad, raw[j]);
+      if (ok[j]) ref_fetch<FMT, OFF>(a.pool, t[j], a.pad, raw[j]);
     }
     // composite in order; stop at the sample that escalates
     int last = 0, last_ss = step_size;
@@ -388,12 +382,11 @@ __device__ void march_reference_grouped(const VolumeArgs &a, const float *tf, co
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       if (ok[j] && live) {
-        if (fresh[j]) ref_values<FMT, OFF>(raw[j], t[j], a.pad, lut, cur);
+        float cur[8];
+        ref_values<FMT, OFF>(raw[j], t[j], a.pad, lut, cur);
         const float s = ref_interp<OFF>(cur, t[j]);
         shade_composite_impl<false, THETA>(a, tf, s, pgid, st, 1.f);
         if (COUNT) ++count;
-        c0 = t[j].r00 + (OFF)t[j].ix;
-        c7 = t[j].r11 + (OFF)t[j].jx;
         last = j;
         plast = p[j];
         if (st.v.w >= 0.93f) {   // v.w > 0.93 (double), see march_reference
@@ -651,17 +644,17 @@ __device__ __forceinline__ void load_pixel(const VolumeArgs &a, size_t idx, Marc
 
 // Samples whose tap loads a reference-schedule group issues together, per
 // brick format.  Measured on the 512^2 frames: FP32 (256^3 single brick) takes
-// 0.54 ms with 2 (105 registers, 4 waves per SIMD hold the whole frame) and
-// 0.64 ms with 4 (3 waves per SIMD); u8 (1024^3 paged, 4 lanes per ray) 0.357,
-// 0.354 and 0.350 ms with 4, 8 and 12.
+// 0.499 ms with 4 (128 registers, 4 waves per SIMD hold the whole frame) and
+// 0.509 / 0.607 / 0.603 ms with 3 / 5 / 6; u8 (1024^3 paged, 4 lanes per ray)
+// 0.357, 0.354 and 0.350 ms with 4, 8 and 12.
 #ifndef HYD_REF_GROUP_U8
 #define HYD_REF_GROUP_U8 8
 #endif
 #ifndef HYD_REF_GROUP_F16
-#define HYD_REF_GROUP_F16 2
+#define HYD_REF_GROUP_F16 4
 #endif
 #ifndef HYD_REF_GROUP_F32
-#define HYD_REF_GROUP_F32 2
+#define HYD_REF_GROUP_F32 4
 #endif
 template <int FMT> constexpr int ref_group() {
   return FMT == HYD_FMT_U8 ? HYD_REF_GROUP_U8 : FMT == HYD_FMT_F16 ? HYD_REF_GROUP_F16 : HYD_REF_GROUP_F32;
@@ -798,7 +791,9 @@ __global__ __launch_bounds__(64) void volume_trace_kernel(VolumeArgs a, int x, i
 // sample positions, evaluated exactly as the walk evaluates them, so every
 // pass sees the same cells and samples as the single march.  A cell not
 // resident in this pass is passed over in one step; a resident one composites
-// its run with the tap loads of up to U samples in flight together.
+// its run with the tap loads of up to U samples in flight together (every
+// sample loads its taps: skipping a repeated cell's loads cost more in
+// branches and register copies, 4.60 against 4.51 ms for the 3-batch set).
 #ifndef HYD_SEG_GROUP
 #define HYD_SEG_GROUP 4
 #endif
@@ -846,8 +841,6 @@ __global__ __launch_bounds__(64) void segment_kernel(VolumeArgs a, SegBatchTable
       const float cdu = 1.f / comp_max(abs3(d));
       const f3 invr = recip3(d);
       const OFF pad3 = (OFF)a.pad * (OFF)a.pad * (OFF)a.pad;
-      OFF c0 = ~(OFF)0, c7 = ~(OFF)0;   // taps of the last composited sample
-      float cur[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       int k = 0;
       while (k < kend && ord < a.max_slots) {
         float w;
@@ -882,28 +875,22 @@ __global__ __launch_bounds__(64) void segment_kernel(VolumeArgs a, SegBatchTable
           for (int kk = k; kk < kc; kk += U) {
             RefTap<OFF> tp[U];
             RawTaps<FMT> raw[U];
-            bool ok[U], fresh[U];
+            bool ok[U];
             float wj[U];
-            OFF p0 = c0, p7 = c7;
 #pragma unroll
             for (int j = 0; j < U; ++j) {
               ok[j] = kk + j < kc;
               const f3 pj = lattice_pos(o0, d, lf, a.step, ok[j] ? kk + j : kk, wj[j]);
               tp[j] = ref_tap<OFF>(sub(pj, ip), pgd, a.pad, page_base);
-              const OFF i0 = tp[j].r00 + (OFF)tp[j].ix, i7 = tp[j].r11 + (OFF)tp[j].jx;
-              fresh[j] = i0 != p0 || i7 != p7;
-              p0 = i0;
-              p7 = i7;
-              if (ok[j] && fresh[j]) ref_fetch<FMT, OFF>(a.pool, tp[j], a.pad, raw[j]);
+              if (ok[j]) ref_fetch<FMT, OFF>(a.pool, tp[j], a.pad, raw[j]);
             }
 #pragma unroll
             for (int j = 0; j < U; ++j) {
               if (ok[j]) {
-                if (fresh[j]) ref_values<FMT, OFF>(raw[j], tp[j], a.pad, s_lut, cur);
+                float cur[8];
+                ref_values<FMT, OFF>(raw[j], tp[j], a.pad, s_lut, cur);
                 const float s = ref_interp<OFF>(cur, tp[j]);
                 shade_composite_impl<true, false>(a, s_tf, s, pgid, seg, wj[j]);
-                c0 = tp[j].r00 + (OFF)tp[j].ix;
-                c7 = tp[j].r11 + (OFF)tp[j].jx;
                 if (COUNT) ++count;
               }
             }
