@@ -304,11 +304,13 @@ __device__ void march_reference(const VolumeArgs &a, const float *tf, const floa
 // pool is below 4 GiB (OFF = uint32_t, so the base stays in scalar registers),
 // else 64-bit.
 
-// Rays per wave below which a wave hands its remaining rays to the
-// cooperative march (kCoopLanes lanes per ray).  Paged 1024^3 u8 frame at
-// 512^2: 0.587 ms without the handover; 0.543 / 0.519 / 0.550 / 0.565 ms with
-// 2 / 4 / 8 / 16 lanes per ray; the plain FP32 frame is unchanged (0.553 ms):
-// its last waves still hold more rays than the handover takes.
+// Rays per wave below which a wave of the one-lane-per-ray march (FP32 and
+// FP16 frames) hands its remaining rays to the cooperative march (kCoopLanes
+// lanes per ray).  Measured when the paged 1024^3 u8 frame at 512^2 still ran
+// one lane per ray: 0.587 ms without the handover; 0.543 / 0.519 / 0.550 /
+// 0.565 ms with 2 / 4 / 8 / 16 lanes per ray.  The plain FP32 frame is
+// unchanged by it: its last waves still hold more rays than the handover
+// takes.
 #ifndef HYD_COOP_LANES
 #define HYD_COOP_LANES 4
 #endif
