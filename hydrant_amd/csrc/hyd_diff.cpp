@@ -163,7 +163,10 @@ int32_t hyd_diff_backward(hyd_ctx *c, const hyd_diff_params *p, const hyd_view *
   // fixed-point headroom: the int64 accumulators hold 2^33 terms of up to 2^30 each
   const double terms = 2.0 * (double)n_rays * (double)a.K;
   a.headroom = terms > 8589934592.0 ? (int32_t)std::ceil(std::log2(terms / 8589934592.0)) : 0;
-  a.n_workers = c->n_cus > 0 ? c->n_cus * 5 : 1280;   // 4-wave workgroups at 5 waves per SIMD
+  // 4-wave workgroups, one per SIMD's wave slot: the ray-order pass holds 124
+  // registers, so 4 waves per SIMD are resident (a fifth, at 96 registers with
+  // spills, measured 2% faster and is not used)
+  a.n_workers = c->n_cus > 0 ? c->n_cus * 4 : 1024;
   HYD_CHECK_HIP(launch_diff_backward(a, c->stream));
   return HYD_OK;
 }
