@@ -402,15 +402,16 @@ int32_t hyd_batch_autotune_schedule(hyd_ctx *c, hyd_batch *b, float density, flo
     if (1.f - ert <= 0.5f * max_error) erts.push_back(ert);
   erts.push_back(0.f);
   // coordinate search: launch shapes at the earliest admissible termination,
-  // then every admissible threshold at that shape
+  // then every admissible threshold (that one again too) at that shape, timed
+  // back to back so that the thresholds are compared under the same conditions
   hyd_batch_config cfg;
-  float t_best = 0.f;
-  if ((st = hyd_batch_autotune(c, b, density, erts[0], rgba_dev, reps, &cfg, &t_best))) return st;
-  float e_best = erts[0];
+  float t_shape = 0.f;
+  if ((st = hyd_batch_autotune(c, b, density, erts[0], rgba_dev, reps, &cfg, &t_shape))) return st;
+  float t_best = 1e30f, e_best = erts[0];
   hipEvent_t e0, e1;
   HYD_CHECK_HIP(hipEventCreate(&e0));
   HYD_CHECK_HIP(hipEventCreate(&e1));
-  for (size_t i = 1; i < erts.size() && st == HYD_OK; ++i) {
+  for (size_t i = 0; i < erts.size() && st == HYD_OK; ++i) {
     BatchArgs a;
     int grid = 0;
     if ((st = fill_args(c, b, &cfg, density, erts[i], rgba_dev, nullptr, a, grid))) break;

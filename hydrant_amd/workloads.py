@@ -192,7 +192,7 @@ class BatchedWorkload:
 
 
 def batched(sizes=(256, 512, 768), n_views: int = 96, res_choices=(256, 384, 512, 768, 1024), fmt: int = L.FMT_F16,
-            bs: int = 64, apron: int = 1, density: float = 1e-2, ert: float = 0.9995, sample_rate: float = 0.5,
+            bs: int = 64, apron: int = 1, density: float = 1e-2, ert: float = 0.9999, sample_rate: float = 0.5,
             seed: int = SEED_BATCH, normalize: bool = False) -> BatchedWorkload:
     """Batched precision form: seeded Gaussian-blob volumes (sigma scaled by size/256) stored
     as fmt bricks of bs^3 + apron, n_views views with resolutions drawn from res_choices and
