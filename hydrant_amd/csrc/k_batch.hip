@@ -112,15 +112,15 @@ __device__ __forceinline__ Tap local_tap(float fx, float fy, float fz, float fpa
   return t;
 }
 
-// Fetched taps in the layout's order, as loaded: FP16 as four words each
-// holding a pair of neighbours along the fastest axis, u8 as the raw bytes,
-// FP32 as floats.  Values are unpacked / converted only in the interpolation,
+// Fetched taps in the layout's order, as loaded: FP16 and u8 as four words
+// each holding a pair of neighbours along the fastest axis (u8 in the low 16
+// bits), FP32 as floats.  Values are unpacked / converted only in the interpolation,
 // so a sample that reuses its predecessor's taps selects whole registers.
 template <int FMT> struct TapVals { float v[8]; };
-template <> struct TapVals<HYD_FMT_U8> { uint32_t v[8]; };
+template <> struct TapVals<HYD_FMT_U8> { uint32_t v[4]; };
 template <> struct TapVals<HYD_FMT_F16> { uint32_t v[4]; };
 
-template <int FMT> constexpr int tap_regs() { return FMT == HYD_FMT_F16 ? 4 : 8; }
+template <int FMT> constexpr int tap_regs() { return FMT == HYD_FMT_F32 ? 8 : 4; }
 
 // The 8 taps: two planes along the slowest axis, each addressed once; PAD > 0
 // is the padded brick edge as a compile-time constant (0: runtime pad).
@@ -138,9 +138,14 @@ __device__ __forceinline__ TapVals<FMT> fetch(const char *pool, uint32_t off, ui
     __builtin_memcpy(&r.v[2], pool + b1, 4);
     __builtin_memcpy(&r.v[3], pool + b1 + 2 * pad, 4);
   } else if constexpr (FMT == HYD_FMT_U8) {
-    const uint8_t *q0 = reinterpret_cast<const uint8_t *>(pool + b0), *q1 = reinterpret_cast<const uint8_t *>(pool + b1);
-    r.v[0] = q0[0]; r.v[1] = q0[1]; r.v[2] = q0[pad]; r.v[3] = q0[pad + 1];
-    r.v[4] = q1[0]; r.v[5] = q1[1]; r.v[6] = q1[pad]; r.v[7] = q1[pad + 1];
+    // pairs (a0, a0 + 1) at any alignment: one 2-byte load each, a register each
+    uint16_t w[4];
+    __builtin_memcpy(&w[0], pool + b0, 2);
+    __builtin_memcpy(&w[1], pool + b0 + pad, 2);
+    __builtin_memcpy(&w[2], pool + b1, 2);
+    __builtin_memcpy(&w[3], pool + b1 + pad, 2);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r.v[i] = w[i];
   } else {
     const float *q0 = reinterpret_cast<const float *>(pool + b0), *q1 = reinterpret_cast<const float *>(pool + b1);
     r.v[0] = q0[0]; r.v[1] = q0[1]; r.v[2] = q0[pad]; r.v[3] = q0[pad + 1];
@@ -173,8 +178,12 @@ __device__ __forceinline__ TapVals<HYD_FMT_F16> fetch_pairs(const char *pool, ui
 // Tap with x-y-z bits c of layout PERM, as a float (u8 / FP32).
 template <int PERM, int FMT>
 __device__ __forceinline__ float tap_f(const TapVals<FMT> &t, int c) {
-  if constexpr (FMT == HYD_FMT_U8) return float(t.v[tap_slot<PERM>(c)]);
-  else return t.v[tap_slot<PERM>(c)];
+  if constexpr (FMT == HYD_FMT_U8) {
+    const int sl = tap_slot<PERM>(c);   // byte sl & 1 of pair sl / 2
+    return float((t.v[sl >> 1] >> (8 * (sl & 1))) & 0xffu);
+  } else {
+    return t.v[tap_slot<PERM>(c)];
+  }
 }
 
 // FP16 tap with x-y-z bits c of layout PERM (slot s: word s / 2, high half if s is odd).
