@@ -118,8 +118,8 @@ bool env_is_1(const char *name) {
 
 // The pool's oriented layouts, made once per pool upload / bind (the stream is
 // synchronised so that graphs captured on another stream see them).  FP16
-// pools get the three row-pair layouts (6x the pool's bytes) while their byte
-// offsets fit 32 bits; otherwise, or when the device cannot hold them, the
+// and u8 pools get the three row-pair layouts (6x the pool's bytes) while
+// their byte offsets fit 32 bits; otherwise, or when the device cannot hold them, the
 // y- and z-fastest layouts (2x).  When the device cannot hold those either, or
 // when the environment sets HYD_BATCH_SINGLE_LAYOUT=1, the renderer runs on
 // the x-fastest layout alone.  HYD_BATCH_NO_PAIRS=1 skips the row-pair
@@ -138,9 +138,9 @@ int32_t ensure_oriented(Ctx *c) {
     c->oriented_gen = c->pool_gen;
     return HYD_OK;
   }
-  const bool pairs_fit = c->pool.fmt == HYD_FMT_F16 && 2 * bytes <= (size_t)UINT32_MAX;
+  const bool pairs_fit = (c->pool.fmt == HYD_FMT_F16 || c->pool.fmt == HYD_FMT_U8) && 2 * bytes <= (size_t)UINT32_MAX;
   if (pairs_fit && !env_is_1("HYD_BATCH_NO_PAIRS") && reserve_oriented(c, 6 * bytes)) {
-    HYD_CHECK_HIP(launch_orient_pairs(c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages, 0,
+    HYD_CHECK_HIP(launch_orient_pairs(c->pool.fmt, c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages, 0,
                                       c->pool.n_pages, c->stream));
     pairs = true;
   } else if (reserve_oriented(c, 2 * bytes)) {
@@ -164,8 +164,8 @@ int32_t ensure_oriented(Ctx *c) {
 int32_t hyd::refresh_oriented_pages(Ctx *c, int32_t first, int32_t n) {
   if (c->oriented_gen != c->pool_gen || !c->own_oriented || n <= 0) return HYD_OK;
   if (c->oriented_pairs) {
-    HYD_CHECK_HIP(launch_orient_pairs(c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages, first, n,
-                                      c->stream));
+    HYD_CHECK_HIP(launch_orient_pairs(c->pool.fmt, c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages,
+                                      first, n, c->stream));
   } else {
     HYD_CHECK_HIP(launch_orient_pool(c->pool.fmt, c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages,
                                      first, n, c->stream));

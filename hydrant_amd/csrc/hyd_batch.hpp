@@ -49,11 +49,11 @@ struct BatchArgs {
 
 hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int group, int grid, hipStream_t s);
 
-// Writes pages [first_page, first_page + n_pages) of the three FP16 row-pair
-// layouts of an x-fastest FP16 pool of total_pages pages (layout p at dst + p *
-// 4 pad^3 total_pages bytes).
-hipError_t launch_orient_pairs(const void *src, void *dst, int pad, int total_pages, int first_page, int n_pages,
-                               hipStream_t s);
+// Writes pages [first_page, first_page + n_pages) of the three row-pair
+// layouts of an x-fastest FP16 or u8 pool of total_pages pages (layout p at
+// dst + p * 2 e pad^3 total_pages bytes, e the voxel's size).
+hipError_t launch_orient_pairs(int fmt, const void *src, void *dst, int pad, int total_pages, int first_page,
+                               int n_pages, hipStream_t s);
 
 // Writes pages [first_page, first_page + n_pages) of the y-fastest (at dst) and
 // z-fastest (at dst + pad^3 * total_pages elements) layouts of an x-fastest

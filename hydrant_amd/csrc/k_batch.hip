@@ -166,19 +166,31 @@ __host__ __device__ constexpr int pair_slot(int c) {
                    : 2 * ((c & 1) + 2 * (c >> 2)) + ((c >> 1) & 1);
 }
 
-template <int PAD>
-__device__ __forceinline__ TapVals<HYD_FMT_F16> fetch_pairs(const char *pool, uint32_t off, uint32_t pad_rt) {
+// u8 row-pair layouts: 2-byte elements (v(a0, a1, a2), v(a0, a1 + 1, a2)), so
+// a plane's four taps are the 4 bytes of elements a0 and a0 + 1: one load per
+// plane at 2-byte alignment.  Taps: byte 2 b0 + b1 of word b2.
+template <int FMT, int PAD>
+__device__ __forceinline__ TapVals<FMT> fetch_pairs(const char *pool, uint32_t off, uint32_t pad_rt) {
   const uint32_t pad = PAD > 0 ? (uint32_t)PAD : pad_rt;
-  TapVals<HYD_FMT_F16> r;
-  __builtin_memcpy(&r.v[0], pool + off * 4u, 8);
-  __builtin_memcpy(&r.v[2], pool + (off + pad * pad) * 4u, 8);
+  TapVals<FMT> r;
+  if constexpr (FMT == HYD_FMT_F16) {
+    __builtin_memcpy(&r.v[0], pool + off * 4u, 8);
+    __builtin_memcpy(&r.v[2], pool + (off + pad * pad) * 4u, 8);
+  } else {
+    __builtin_memcpy(&r.v[0], pool + off * 2u, 4);
+    __builtin_memcpy(&r.v[1], pool + (off + pad * pad) * 2u, 4);
+    r.v[2] = r.v[3] = 0u;
+  }
   return r;
 }
 
 // Tap with x-y-z bits c of layout PERM, as a float (u8 / FP32).
-template <int PERM, int FMT>
+template <int PERM, int FMT, bool PAIRS>
 __device__ __forceinline__ float tap_f(const TapVals<FMT> &t, int c) {
-  if constexpr (FMT == HYD_FMT_U8) {
+  if constexpr (FMT == HYD_FMT_U8 && PAIRS) {
+    const int sl = pair_slot<PERM>(c);   // byte sl & 3 of word sl / 4
+    return float((t.v[sl >> 2] >> (8 * (sl & 3))) & 0xffu);
+  } else if constexpr (FMT == HYD_FMT_U8) {
     const int sl = tap_slot<PERM>(c);   // byte sl & 1 of pair sl / 2
     return float((t.v[sl >> 1] >> (8 * (sl & 1))) & 0xffu);
   } else {
@@ -195,12 +207,13 @@ __device__ __forceinline__ __half tap_h(const TapVals<HYD_FMT_F16> &t, int c) {
 }
 
 // Trilinear x -> y -> z.
-template <int PERM, int FMT>
+template <int PERM, int FMT, bool PAIRS>
 __device__ __forceinline__ float lerp8(const TapVals<FMT> &v, const Tap &t) {
-  const float x00 = fmaf(t.ax, tap_f<PERM>(v, 1) - tap_f<PERM>(v, 0), tap_f<PERM>(v, 0));
-  const float x10 = fmaf(t.ax, tap_f<PERM>(v, 3) - tap_f<PERM>(v, 2), tap_f<PERM>(v, 2));
-  const float x01 = fmaf(t.ax, tap_f<PERM>(v, 5) - tap_f<PERM>(v, 4), tap_f<PERM>(v, 4));
-  const float x11 = fmaf(t.ax, tap_f<PERM>(v, 7) - tap_f<PERM>(v, 6), tap_f<PERM>(v, 6));
+  const auto tap = [&](int c) { return tap_f<PERM, FMT, PAIRS>(v, c); };
+  const float x00 = fmaf(t.ax, tap(1) - tap(0), tap(0));
+  const float x10 = fmaf(t.ax, tap(3) - tap(2), tap(2));
+  const float x01 = fmaf(t.ax, tap(5) - tap(4), tap(4));
+  const float x11 = fmaf(t.ax, tap(7) - tap(6), tap(6));
   const float y0 = fmaf(t.ay, x10 - x00, x00);
   const float y1 = fmaf(t.ay, x11 - x01, x01);
   return fmaf(t.az, y1 - y0, y0);
@@ -225,7 +238,7 @@ __device__ __forceinline__ float lerp8_h(const TapVals<HYD_FMT_F16> &v, const Ta
 template <int PERM, int FMT, bool PAIRS>
 __device__ __forceinline__ float interpolate(const TapVals<FMT> &v, const Tap &t) {
   if constexpr (FMT == HYD_FMT_F16) return lerp8_h<PERM, PAIRS>(v, t);
-  else return lerp8<PERM, FMT>(v, t);
+  else return lerp8<PERM, FMT, PAIRS>(v, t);
 }
 
 // TF lookup on the edge-extended table in slope form: entry i holds
@@ -335,7 +348,7 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope 
         tp[u] = local_tap<PERM>(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), fpad, lim, base);
         fresh[u] = tp[u].off != (u == 0 ? poff : tp[u - 1].off);
         if (fresh[u]) {
-          if constexpr (PAIRS) tv[u] = fetch_pairs<PAD>(pool, tp[u].off, pad);
+          if constexpr (PAIRS) tv[u] = fetch_pairs<FMT, PAD>(pool, tp[u].off, pad);
           else tv[u] = fetch<FMT, PAD>(pool, tp[u].off, pad);
         }
       }
@@ -473,12 +486,14 @@ __global__ __launch_bounds__(64 * GROUP) void batch_kernel(BatchArgs a) {
 //   layout 1 (y, z, x), plane a1 = z: source plane z (and z + 1 for pairs);
 //   layout 2 (z, x, y), plane a2 = y: the source rows of every z at that y.
 // Layout 0 (the x-fastest copy of the row-pair form) reads plane z directly.
-// PAIRS: 4-byte elements (v(a0, a1, a2), v(a0, min(a1 + 1, pad - 1), a2)) of a
-// 2-byte pool; otherwise single voxels of type T.
+// PAIRS: elements (v(a0, a1, a2), v(a0, min(a1 + 1, pad - 1), a2)) of twice the
+// voxel's size (4 bytes for a 2-byte pool, 2 for a u8 pool); otherwise single
+// voxels of type T.
 template <typename T, bool PAIRS>
 __global__ __launch_bounds__(256) void orient_kernel(const T *src, void *dst_v, int pad, int total_pages,
                                                      int first_page, int layout0, bool staged) {
-  using Out = typename std::conditional<PAIRS, uint32_t, T>::type;
+  using Pair = typename std::conditional<sizeof(T) == 1, uint16_t, uint32_t>::type;
+  using Out = typename std::conditional<PAIRS, Pair, T>::type;
   extern __shared__ unsigned char s_raw[];
   T *s_a = reinterpret_cast<T *>(s_raw), *s_b = s_a + pad * pad;
   const size_t page = (size_t)first_page + blockIdx.x;
@@ -488,7 +503,7 @@ __global__ __launch_bounds__(256) void orient_kernel(const T *src, void *dst_v, 
   Out *out = static_cast<Out *>(dst_v) + (size_t)(p - (PAIRS ? 0 : 1)) * p3 * (size_t)total_pages + page * p3;
   const int n2 = pad * pad;
   const auto pack = [](T v0, T v1) -> Out {
-    if constexpr (PAIRS) return (uint32_t)v0 | ((uint32_t)v1 << 16);
+    if constexpr (PAIRS) return (Out)((uint32_t)v0 | ((uint32_t)v1 << (8 * sizeof(T))));
     else return v0;
   };
   if (p == 0) {   // (x, y, z): plane z; the pair neighbour is y + 1
@@ -535,7 +550,7 @@ __global__ void reset_counters(unsigned *c, int n) {
 }
 
 // 64^3 bricks with a one-voxel apron (pad 66) get the compile-time brick edge;
-// FP16 pools read through row-pair layouts when the host built them.
+// FP16 and u8 pools read through row-pair layouts when the host built them.
 template <int FMT, int TW, int UNROLL, int GROUP, bool PAIRS>
 hipError_t launch5(const BatchArgs &a, int grid, hipStream_t s) {
   const size_t shmem = (size_t)(a.tf_n + 1) * sizeof(TfSlope);
@@ -556,7 +571,7 @@ hipError_t launch5(const BatchArgs &a, int grid, hipStream_t s) {
 
 template <int FMT, int TW, int UNROLL, int GROUP>
 hipError_t launch4(const BatchArgs &a, int grid, hipStream_t s) {
-  if constexpr (FMT == HYD_FMT_F16) {
+  if constexpr (FMT == HYD_FMT_F16 || FMT == HYD_FMT_U8) {
     if (a.pairs) return launch5<FMT, TW, UNROLL, GROUP, true>(a, grid, s);
   }
   return launch5<FMT, TW, UNROLL, GROUP, false>(a, grid, s);
@@ -610,8 +625,9 @@ hipError_t launch_orient_pool(int fmt, const void *src, void *dst, int pad, int 
   }
 }
 
-hipError_t launch_orient_pairs(const void *src, void *dst, int pad, int total_pages, int first_page, int n_pages,
-                               hipStream_t s) {
+hipError_t launch_orient_pairs(int fmt, const void *src, void *dst, int pad, int total_pages, int first_page,
+                               int n_pages, hipStream_t s) {
+  if (fmt == HYD_FMT_U8) return launch_orient<uint8_t, true>(src, dst, pad, total_pages, first_page, n_pages, 0, 3, s);
   return launch_orient<uint16_t, true>(src, dst, pad, total_pages, first_page, n_pages, 0, 3, s);
 }
 

@@ -235,16 +235,17 @@ def test_pool_change_refreshes_oriented_layouts():
     ctx.close()
 
 
+@pytest.mark.parametrize("fmt", [L.FMT_F16, L.FMT_U8])
 @pytest.mark.parametrize("switch", ["HYD_BATCH_NO_PAIRS", "HYD_BATCH_SINGLE_LAYOUT"])
-def test_layout_fallbacks_render_the_same(monkeypatch, switch):
-    """The FP16 row-pair layouts (the default for FP16 pools), the y-/z-fastest layouts
+def test_layout_fallbacks_render_the_same(monkeypatch, switch, fmt):
+    """The row-pair layouts (the default for FP16 and u8 pools), the y-/z-fastest layouts
     (HYD_BATCH_NO_PAIRS=1, the path taken when the device cannot hold the pairs) and the
     x-fastest layout alone (HYD_BATCH_SINGLE_LAYOUT=1, when it cannot hold either) hold the
     same voxels: same image, bitwise."""
     import torch
     from hydrant_amd.hip import Context
 
-    wl = _small(L.FMT_F16, n_views=4)
+    wl = _small(fmt, n_views=4)
     cfg = batch_config(8, 4, 8)
     ctx = Context(0)
     b = _setup(ctx, wl)
@@ -309,8 +310,9 @@ def test_batch_64_bricks_vs_fp64_oracle(monkeypatch, fmt):
     ctx.close()
 
 
+@pytest.mark.parametrize("fmt", [L.FMT_F16, L.FMT_U8])
 @pytest.mark.parametrize("switch", [None, "HYD_BATCH_NO_PAIRS", "HYD_BATCH_SINGLE_LAYOUT"])
-def test_pool_page_writes_refresh_oriented_layouts(monkeypatch, switch):
+def test_pool_page_writes_refresh_oriented_layouts(monkeypatch, switch, fmt):
     """Pages overwritten in place (hyd_write_pool_pages) after the oriented copies were
     made render as a fresh context with the modified pool does: eagerly, and from a graph
     captured before the write (the copies are updated in place)."""
@@ -321,7 +323,7 @@ def test_pool_page_writes_refresh_oriented_layouts(monkeypatch, switch):
 
     if switch is not None:
         monkeypatch.setenv(switch, "1")
-    wl = _small(L.FMT_F16, n_views=4)
+    wl = _small(fmt, n_views=4)
     pad = wl.bs + 2 * wl.apron
     cfg = batch_config(8, 4, 8)
     ctx = Context(0)
