@@ -118,19 +118,21 @@ bool env_is_1(const char *name) {
 
 // The pool's oriented layouts, made once per pool upload / bind (the stream is
 // synchronised so that graphs captured on another stream see them).  FP16
-// and u8 pools get the three row-pair layouts (6x the pool's bytes) while
-// their byte offsets fit 32 bits; otherwise, or when the device cannot hold them, the
-// y- and z-fastest layouts (2x).  When the device cannot hold those either, or
-// when the environment sets HYD_BATCH_SINGLE_LAYOUT=1, the renderer runs on
-// the x-fastest layout alone.  HYD_BATCH_NO_PAIRS=1 skips the row-pair
-// layouts.  Every choice renders the same images.
+// and u8 pools get the three quad layouts (12x the pool's bytes), or, when
+// the device cannot hold them, the three row-pair layouts (6x, while their
+// byte offsets fit 32 bits); otherwise, or when the device cannot hold those,
+// the y- and z-fastest layouts (2x).  When the device cannot hold those
+// either, or when the environment sets HYD_BATCH_SINGLE_LAYOUT=1, the
+// renderer runs on the x-fastest layout alone.  HYD_BATCH_NO_QUADS=1 skips the
+// quad layouts, HYD_BATCH_NO_PAIRS=1 the quads and the row pairs.  Every
+// choice renders the same images.
 int32_t ensure_oriented(Ctx *c) {
   if (c->oriented_gen == c->pool_gen) return HYD_OK;
   const size_t bytes = layout_bytes(c);
   // until the copies below are complete the context holds none (a failed
   // launch leaves the generation stale, so the next launch tries again)
-  c->oriented_pairs = false;
-  bool pairs = false;
+  c->oriented_pairs = 0;
+  int pairs = 0;
   if (env_is_1("HYD_BATCH_SINGLE_LAYOUT")) {
     if (c->own_oriented) (void)hipFree(c->own_oriented);
     c->own_oriented = nullptr;
@@ -139,10 +141,16 @@ int32_t ensure_oriented(Ctx *c) {
     return HYD_OK;
   }
   const bool pairs_fit = (c->pool.fmt == HYD_FMT_F16 || c->pool.fmt == HYD_FMT_U8) && 2 * bytes <= (size_t)UINT32_MAX;
-  if (pairs_fit && !env_is_1("HYD_BATCH_NO_PAIRS") && reserve_oriented(c, 6 * bytes)) {
+  // the quads' offsets are 64-bit in the kernel
+  const bool quads = (c->pool.fmt == HYD_FMT_F16 || c->pool.fmt == HYD_FMT_U8) && !env_is_1("HYD_BATCH_NO_QUADS");
+  if (quads && !env_is_1("HYD_BATCH_NO_PAIRS") && reserve_oriented(c, 12 * bytes)) {
+    HYD_CHECK_HIP(launch_orient_quads(c->pool.fmt, c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages, 0,
+                                      c->pool.n_pages, c->stream));
+    pairs = 2;
+  } else if (pairs_fit && !env_is_1("HYD_BATCH_NO_PAIRS") && reserve_oriented(c, 6 * bytes)) {
     HYD_CHECK_HIP(launch_orient_pairs(c->pool.fmt, c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages, 0,
                                       c->pool.n_pages, c->stream));
-    pairs = true;
+    pairs = 1;
   } else if (reserve_oriented(c, 2 * bytes)) {
     HYD_CHECK_HIP(launch_orient_pool(c->pool.fmt, c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages, 0,
                                      c->pool.n_pages, c->stream));
@@ -163,7 +171,10 @@ int32_t ensure_oriented(Ctx *c) {
 // copies are stale anyway, or when the renderer reads the pool alone).
 int32_t hyd::refresh_oriented_pages(Ctx *c, int32_t first, int32_t n) {
   if (c->oriented_gen != c->pool_gen || !c->own_oriented || n <= 0) return HYD_OK;
-  if (c->oriented_pairs) {
+  if (c->oriented_pairs == 2) {
+    HYD_CHECK_HIP(launch_orient_quads(c->pool.fmt, c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages,
+                                      first, n, c->stream));
+  } else if (c->oriented_pairs) {
     HYD_CHECK_HIP(launch_orient_pairs(c->pool.fmt, c->pool.data, c->own_oriented, c->pool.pad, c->pool.n_pages,
                                       first, n, c->stream));
   } else {
@@ -210,7 +221,7 @@ int32_t fill_args(Ctx *c, hyd_batch *b, const hyd_batch_config *cfg, float densi
   a.pairs = c->oriented_pairs;
   if (a.pairs) {
     const char *o = static_cast<const char *>(c->own_oriented);
-    for (int p = 0; p < 3; ++p) a.pools[p] = o + p * 2 * layout_bytes(c);
+    for (int p = 0; p < 3; ++p) a.pools[p] = o + p * 2 * a.pairs * layout_bytes(c);
   } else {
     a.pools[0] = c->pool.data;
     a.pools[1] = a.oriented ? c->own_oriented : c->pool.data;

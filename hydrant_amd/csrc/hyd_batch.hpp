@@ -35,7 +35,7 @@ struct BatchArgs {
   const int32_t *cells;        // concatenated cell codes of every volume
   const void *pools[3];         // the pool in its three layouts (x, y, z fastest)
   int32_t oriented;             // 0: only the x-fastest layout is held (every tile reads layout 0)
-  int32_t pairs;                // 1: pools[] are the FP16 row-pair layouts (4 B per element)
+  int32_t pairs;                // 1: pools[] are the row-pair layouts (2 voxels per element); 2: the quad layouts (4)
   int32_t pad;
   uint32_t page_elems;         // pad^3
   const float *tf;
@@ -53,6 +53,12 @@ hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int
 // layouts of an x-fastest FP16 or u8 pool of total_pages pages (layout p at
 // dst + p * 2 e pad^3 total_pages bytes, e the voxel's size).
 hipError_t launch_orient_pairs(int fmt, const void *src, void *dst, int pad, int total_pages, int first_page,
+                               int n_pages, hipStream_t s);
+
+// Writes pages [first_page, first_page + n_pages) of the three quad layouts
+// of an x-fastest FP16 or u8 pool of total_pages pages (layout p at dst + p *
+// 4 e pad^3 total_pages bytes, e the voxel's size).
+hipError_t launch_orient_quads(int fmt, const void *src, void *dst, int pad, int total_pages, int first_page,
                                int n_pages, hipStream_t s);
 
 // Writes pages [first_page, first_page + n_pages) of the y-fastest (at dst) and

@@ -79,7 +79,7 @@ struct Ctx {
   void *own_pool = nullptr;
   void *own_oriented = nullptr;   // the pool's y- and z-fastest layouts (batched renderer)
   size_t oriented_cap = 0;
-  bool oriented_pairs = false;     // own_oriented holds the three FP16 row-pair layouts
+  int oriented_pairs = 0;          // own_oriented holds the three row-pair (1) or FP16 quad (2) layouts
   uint64_t oriented_gen = ~0ull;  // pool_gen the oriented layouts were made from
   int32_t *own_vaddr = nullptr;
   float4 *own_page = nullptr;

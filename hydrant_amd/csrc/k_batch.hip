@@ -34,11 +34,13 @@
 // fastest axis is best aligned with that direction at the tile: the quad's
 // taps then lie along one voxel row.  The eight taps are put back in x-y-z
 // order before the interpolation and a ray's work does not depend on its lane,
-// so every choice gives bit-identical images.  FP16 pools are held as row
-// pairs (each 4-byte element also holds the voxel's neighbour along the
-// layout's middle axis): a sample's taps are then two 4-byte-aligned 8-byte
-// loads, which the data path serves at the cost of two dword gathers, instead
-// of four 2-byte-aligned dwords, which cost two accesses per lane each.
+// so every choice gives bit-identical images.  FP16 and u8 pools are held as
+// quads (each element also holds the voxel's neighbours along the layout's
+// middle and slow axes): a sample's eight taps are then one 16-byte (FP16) or
+// 8-byte (u8) load.  Row pairs (the neighbour along the middle axis only; two
+// loads per sample) serve when the device cannot hold the quads; single-voxel
+// FP16 layouts need four 2-byte-aligned dwords, which cost two accesses per
+// lane each.
 #include <hip/hip_fp16.h>
 
 #include <type_traits>
@@ -184,11 +186,36 @@ __device__ __forceinline__ TapVals<FMT> fetch_pairs(const char *pool, uint32_t o
   return r;
 }
 
+// Quad layouts (FP16 and u8): element (a0, a1, a2) holds the four voxels
+// v(a0, a1, a2), v(a0, a1 + 1, a2), v(a0, a1, a2 + 1), v(a0, a1 + 1, a2 + 1)
+// (neighbours clamped at the page edge), so a sample's eight taps are elements
+// a0 and a0 + 1: one 16-byte (FP16) or 8-byte (u8) load.  Tap 4 b0 + 2 b2 + b1
+// of the layout bits (b0, b1, b2), counted in voxels.  A layout can pass
+// 4 GiB: 64-bit byte offsets.
+template <int PERM>
+__host__ __device__ constexpr int quad_slot(int c) {
+  return PERM == 1 ? 4 * ((c >> 1) & 1) + 2 * (c & 1) + (c >> 2)
+       : PERM == 2 ? 4 * (c >> 2) + 2 * ((c >> 1) & 1) + (c & 1)
+                   : 4 * (c & 1) + 2 * (c >> 2) + ((c >> 1) & 1);
+}
+
+template <int FMT>
+__device__ __forceinline__ TapVals<FMT> fetch_quads(const char *pool, uint32_t off) {
+  TapVals<FMT> r;
+  if constexpr (FMT == HYD_FMT_F16) {
+    __builtin_memcpy(&r.v[0], pool + (uint64_t)off * 8u, 16);
+  } else {
+    __builtin_memcpy(&r.v[0], pool + (uint64_t)off * 4u, 8);
+    r.v[2] = r.v[3] = 0u;
+  }
+  return r;
+}
+
 // Tap with x-y-z bits c of layout PERM, as a float (u8 / FP32).
-template <int PERM, int FMT, bool PAIRS>
+template <int PERM, int FMT, int PAIRS>
 __device__ __forceinline__ float tap_f(const TapVals<FMT> &t, int c) {
   if constexpr (FMT == HYD_FMT_U8 && PAIRS) {
-    const int sl = pair_slot<PERM>(c);   // byte sl & 3 of word sl / 4
+    const int sl = PAIRS == 2 ? quad_slot<PERM>(c) : pair_slot<PERM>(c);   // byte sl & 3 of word sl / 4
     return float((t.v[sl >> 2] >> (8 * (sl & 3))) & 0xffu);
   } else if constexpr (FMT == HYD_FMT_U8) {
     const int sl = tap_slot<PERM>(c);   // byte sl & 1 of pair sl / 2
@@ -199,15 +226,15 @@ __device__ __forceinline__ float tap_f(const TapVals<FMT> &t, int c) {
 }
 
 // FP16 tap with x-y-z bits c of layout PERM (slot s: word s / 2, high half if s is odd).
-template <int PERM, bool PAIRS>
+template <int PERM, int PAIRS>
 __device__ __forceinline__ __half tap_h(const TapVals<HYD_FMT_F16> &t, int c) {
-  const int s = PAIRS ? pair_slot<PERM>(c) : tap_slot<PERM>(c);
+  const int s = PAIRS == 2 ? quad_slot<PERM>(c) : PAIRS ? pair_slot<PERM>(c) : tap_slot<PERM>(c);
   const uint32_t w = t.v[s >> 1];
   return __ushort_as_half((unsigned short)((s & 1) ? (w >> 16) : (w & 0xffffu)));
 }
 
 // Trilinear x -> y -> z.
-template <int PERM, int FMT, bool PAIRS>
+template <int PERM, int FMT, int PAIRS>
 __device__ __forceinline__ float lerp8(const TapVals<FMT> &v, const Tap &t) {
   const auto tap = [&](int c) { return tap_f<PERM, FMT, PAIRS>(v, c); };
   const float x00 = fmaf(t.ax, tap(1) - tap(0), tap(0));
@@ -221,7 +248,7 @@ __device__ __forceinline__ float lerp8(const TapVals<FMT> &v, const Tap &t) {
 
 // FP16: x-lerps as (v0 - a v0) + a v1, both FMAs taking the half taps directly
 // (v_fma_mix_f32), no separate conversions.
-template <int PERM, bool PAIRS>
+template <int PERM, int PAIRS>
 __device__ __forceinline__ float lerp8_h(const TapVals<HYD_FMT_F16> &v, const Tap &t) {
   float x[4];
 #pragma unroll
@@ -235,7 +262,7 @@ __device__ __forceinline__ float lerp8_h(const TapVals<HYD_FMT_F16> &v, const Ta
   return fmaf(t.az, y1 - y0, y0);
 }
 
-template <int PERM, int FMT, bool PAIRS>
+template <int PERM, int FMT, int PAIRS>
 __device__ __forceinline__ float interpolate(const TapVals<FMT> &v, const Tap &t) {
   if constexpr (FMT == HYD_FMT_F16) return lerp8_h<PERM, PAIRS>(v, t);
   else return lerp8<PERM, FMT, PAIRS>(v, t);
@@ -289,7 +316,7 @@ __device__ __forceinline__ float box_exit(const RayState &r, float ipx, float ip
   return fminf(fminf(tx, ty), tz);
 }
 
-template <int FMT, int UNROLL, bool COUNT, int PAD, int PERM, bool PAIRS>
+template <int FMT, int UNROLL, bool COUNT, int PAD, int PERM, int PAIRS>
 __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope *ext, RayState &r,
                       unsigned &count) {
   const float step = vol.step, inv_step = 1.f / vol.step;
@@ -348,7 +375,8 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope 
         tp[u] = local_tap<PERM>(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), fpad, lim, base);
         fresh[u] = tp[u].off != (u == 0 ? poff : tp[u - 1].off);
         if (fresh[u]) {
-          if constexpr (PAIRS) tv[u] = fetch_pairs<FMT, PAD>(pool, tp[u].off, pad);
+          if constexpr (PAIRS == 2) tv[u] = fetch_quads<FMT>(pool, tp[u].off);
+          else if constexpr (PAIRS) tv[u] = fetch_pairs<FMT, PAD>(pool, tp[u].off, pad);
           else tv[u] = fetch<FMT, PAD>(pool, tp[u].off, pad);
         }
       }
@@ -386,7 +414,7 @@ __device__ __forceinline__ unsigned claim(unsigned *counter) {
 // GROUP 1: single-wave workgroups, each claiming TW x 8 tiles.  GROUP 4: four
 // waves per workgroup claim a 16 x 16 block and take one 8 x 8 tile each, so
 // the four rays packets that touch the same bricks share one CU's L1.
-template <int FMT, int TW, int UNROLL, bool COUNT, int PAD, int GROUP, bool PAIRS>
+template <int FMT, int TW, int UNROLL, bool COUNT, int PAD, int GROUP, int PAIRS>
 __global__ __launch_bounds__(64 * GROUP) void batch_kernel(BatchArgs a) {
   extern __shared__ TfSlope s_ext[];   // tf_n + 1 entries (dynamic LDS)
   __shared__ unsigned long long s_count;
@@ -550,8 +578,8 @@ __global__ void reset_counters(unsigned *c, int n) {
 }
 
 // 64^3 bricks with a one-voxel apron (pad 66) get the compile-time brick edge;
-// FP16 and u8 pools read through row-pair layouts when the host built them.
-template <int FMT, int TW, int UNROLL, int GROUP, bool PAIRS>
+// FP16 and u8 pools read through quad or row-pair layouts when the host built them.
+template <int FMT, int TW, int UNROLL, int GROUP, int PAIRS>
 hipError_t launch5(const BatchArgs &a, int grid, hipStream_t s) {
   const size_t shmem = (size_t)(a.tf_n + 1) * sizeof(TfSlope);
   const dim3 blk(64 * GROUP);
@@ -572,9 +600,10 @@ hipError_t launch5(const BatchArgs &a, int grid, hipStream_t s) {
 template <int FMT, int TW, int UNROLL, int GROUP>
 hipError_t launch4(const BatchArgs &a, int grid, hipStream_t s) {
   if constexpr (FMT == HYD_FMT_F16 || FMT == HYD_FMT_U8) {
-    if (a.pairs) return launch5<FMT, TW, UNROLL, GROUP, true>(a, grid, s);
+    if (a.pairs == 2) return launch5<FMT, TW, UNROLL, GROUP, 2>(a, grid, s);
+    if (a.pairs == 1) return launch5<FMT, TW, UNROLL, GROUP, 1>(a, grid, s);
   }
-  return launch5<FMT, TW, UNROLL, GROUP, false>(a, grid, s);
+  return launch5<FMT, TW, UNROLL, GROUP, 0>(a, grid, s);
 }
 
 template <int FMT, int TW, int UNROLL>
@@ -629,6 +658,74 @@ hipError_t launch_orient_pairs(int fmt, const void *src, void *dst, int pad, int
                                int n_pages, hipStream_t s) {
   if (fmt == HYD_FMT_U8) return launch_orient<uint8_t, true>(src, dst, pad, total_pages, first_page, n_pages, 0, 3, s);
   return launch_orient<uint16_t, true>(src, dst, pad, total_pages, first_page, n_pages, 0, 3, s);
+}
+
+// The three quad layouts of an x-fastest pool of voxels T (FP16 bits or u8),
+// for pages first_page + blockIdx.x: block (plane s = blockIdx.y, layout p =
+// blockIdx.z) writes the pad^2 elements whose quad starts at s along one axis,
+// one thread per element, consecutive threads writing consecutive elements.
+// Their voxels lie in two source planes (z = s, s + 1 for layouts 0 and 1;
+// rows y = s, s + 1 for layout 2), staged in LDS when `staged` (layout p at
+// dst + p * 4 sizeof(T) pad^3 total_pages bytes).
+template <typename T>
+__global__ __launch_bounds__(256) void orient_quads_kernel(const T *src, void *dst_v, int pad, int total_pages,
+                                                           int first_page, bool staged) {
+  using Out = typename std::conditional<sizeof(T) == 1, uint32_t, uint64_t>::type;
+  constexpr int bits = 8 * sizeof(T);
+  extern __shared__ unsigned char s_raw[];
+  T *s_a = reinterpret_cast<T *>(s_raw), *s_b = s_a + pad * pad;
+  const size_t page = (size_t)first_page + blockIdx.x;
+  const size_t p3 = (size_t)pad * pad * pad;
+  const int s = blockIdx.y, s1 = s + 1 < pad ? s + 1 : pad - 1, p = blockIdx.z, n2 = pad * pad;
+  const T *pg = src + page * p3;
+  Out *out = static_cast<Out *>(dst_v) + (size_t)p * p3 * (size_t)total_pages + page * p3;
+  // voxel (i, j) of source plane q (0: s, 1: s1): planes z = s / s1 indexed
+  // [y][x] for layouts 0 and 1, rows y = s / s1 indexed [z][x] for layout 2
+  const auto src_at = [&](int q, int i, int j) -> Out {
+    const int t = q ? s1 : s;
+    return (Out)(p < 2 ? pg[((size_t)t * pad + i) * pad + j] : pg[((size_t)i * pad + t) * pad + j]);
+  };
+  const auto at = [&](int q, int i, int j) -> Out {
+    return staged ? (Out)(q ? s_b : s_a)[i * pad + j] : src_at(q, i, j);
+  };
+  if (staged) {
+    for (int r = threadIdx.x; r < n2; r += 256) {
+      s_a[r] = (T)src_at(0, r / pad, r % pad);
+      s_b[r] = (T)src_at(1, r / pad, r % pad);
+    }
+    __syncthreads();
+  }
+  const auto quad = [](Out v00, Out v10, Out v01, Out v11) -> Out {
+    return v00 | (v10 << bits) | (v01 << (2 * bits)) | (v11 << (3 * bits));
+  };
+  for (int r = threadIdx.x; r < n2; r += 256) {
+    const int u = r % pad, w = r / pad, w1 = w + 1 < pad ? w + 1 : pad - 1;
+    if (p == 0) {   // (x, y, z), z = s: element (x = u, y = w); quad over y, z
+      out[(size_t)s * n2 + r] = quad(at(0, w, u), at(0, w1, u), at(1, w, u), at(1, w1, u));
+    } else if (p == 1) {   // (y, z, x), z = s: element (y = u, x = w); quad over z, x
+      out[((size_t)w * pad + s) * pad + u] = quad(at(0, u, w), at(1, u, w), at(0, u, w1), at(1, u, w1));
+    } else {   // (z, x, y), y = s: element (z = u, x = w); quad over x, y
+      out[(size_t)s * n2 + r] = quad(at(0, u, w), at(0, u, w1), at(1, u, w), at(1, u, w1));
+    }
+  }
+}
+
+hipError_t launch_orient_quads(int fmt, const void *src, void *dst, int pad, int total_pages, int first_page,
+                               int n_pages, hipStream_t s) {
+  if (n_pages <= 0) return hipSuccess;
+  const dim3 grid(n_pages, pad, 3);
+  const size_t e = fmt == HYD_FMT_U8 ? 1 : 2, lds = 2 * (size_t)pad * pad * e;
+  const bool staged = lds <= 65536;
+  if (fmt == HYD_FMT_U8) {
+    hipLaunchKernelGGL(orient_quads_kernel<uint8_t>, grid, dim3(256), staged ? lds : 0, s,
+                       static_cast<const uint8_t *>(src), dst, pad, total_pages, first_page, staged);
+  } else if (fmt == HYD_FMT_F16) {
+    hipLaunchKernelGGL(orient_quads_kernel<uint16_t>, grid, dim3(256), staged ? lds : 0, s,
+                       static_cast<const uint16_t *>(src), dst, pad, total_pages, first_page, staged);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_batch(const BatchArgs &a, int fmt, int tile_w, int unroll, int group, int grid, hipStream_t s) {

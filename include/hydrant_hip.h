@@ -193,7 +193,8 @@ int32_t hyd_upload_pool(hyd_ctx *ctx, int32_t fmt, int32_t pad_edge, int32_t n_p
  * The batched renderer reads the pool through oriented copies that it derives
  * at its first launch after an upload or bind: after changing a bound pool's
  * contents in place, call this again before the next batched launch.  The
- * copies take 6x the pool's bytes for FP16 pools (three row-pair layouts),
+ * copies take 12x the pool's bytes for FP16 and u8 pools (three quad
+ * layouts), or 6x (three row-pair layouts) when the device cannot hold those,
  * else 2x (y- and z-fastest layouts); with less free memory the renderer
  * reads the pool alone.  Every choice renders the same image. */
 int32_t hyd_bind_pool_device(hyd_ctx *ctx, int32_t fmt, int32_t pad_edge, int32_t n_pages, const void *pool_dev);

@@ -236,10 +236,11 @@ def test_pool_change_refreshes_oriented_layouts():
 
 
 @pytest.mark.parametrize("fmt", [L.FMT_F16, L.FMT_U8])
-@pytest.mark.parametrize("switch", ["HYD_BATCH_NO_PAIRS", "HYD_BATCH_SINGLE_LAYOUT"])
+@pytest.mark.parametrize("switch", ["HYD_BATCH_NO_QUADS", "HYD_BATCH_NO_PAIRS", "HYD_BATCH_SINGLE_LAYOUT"])
 def test_layout_fallbacks_render_the_same(monkeypatch, switch, fmt):
-    """The row-pair layouts (the default for FP16 and u8 pools), the y-/z-fastest layouts
-    (HYD_BATCH_NO_PAIRS=1, the path taken when the device cannot hold the pairs) and the
+    """The quad layouts (the FP16 default), the row-pair layouts (the u8 default, and FP16's
+    with HYD_BATCH_NO_QUADS=1 or when the device cannot hold the quads), the y-/z-fastest
+    layouts (HYD_BATCH_NO_PAIRS=1, or when the device cannot hold the pairs) and the
     x-fastest layout alone (HYD_BATCH_SINGLE_LAYOUT=1, when it cannot hold either) hold the
     same voxels: same image, bitwise."""
     import torch
@@ -283,8 +284,8 @@ def test_batch_rejects_pools_beyond_32_bit_offsets():
 @pytest.mark.parametrize("fmt", [L.FMT_F16, L.FMT_U8, L.FMT_F32])
 def test_batch_64_bricks_vs_fp64_oracle(monkeypatch, fmt):
     """64^3 bricks with a one-voxel apron take the kernels compiled for a padded edge of
-    66 (the 96-view benchmark's bricks), through each pool layout: row pairs (the FP16
-    default), the single-voxel oriented layouts and the x-fastest pool alone.  Error
+    66 (the 96-view benchmark's bricks), through each pool layout: quads (the FP16
+    default), row pairs, the single-voxel oriented layouts and the x-fastest pool alone.  Error
     against the FP64 oracle within 1e-3, and the layouts give the same image."""
     import torch
     from hydrant_amd.hip import Context
@@ -293,7 +294,7 @@ def test_batch_64_bricks_vs_fp64_oracle(monkeypatch, fmt):
     ctx = Context(0)
     b = _setup(ctx, wl)
     ref = None
-    for switch in (None, "HYD_BATCH_NO_PAIRS", "HYD_BATCH_SINGLE_LAYOUT"):
+    for switch in (None, "HYD_BATCH_NO_QUADS", "HYD_BATCH_NO_PAIRS", "HYD_BATCH_SINGLE_LAYOUT"):
         if switch is not None:
             monkeypatch.setenv(switch, "1")
             ctx.upload_pool(wl.fmt, wl.bs + 2 * wl.apron, wl.pool)   # the switch is read when the pool changes
@@ -311,7 +312,7 @@ def test_batch_64_bricks_vs_fp64_oracle(monkeypatch, fmt):
 
 
 @pytest.mark.parametrize("fmt", [L.FMT_F16, L.FMT_U8])
-@pytest.mark.parametrize("switch", [None, "HYD_BATCH_NO_PAIRS", "HYD_BATCH_SINGLE_LAYOUT"])
+@pytest.mark.parametrize("switch", [None, "HYD_BATCH_NO_QUADS", "HYD_BATCH_NO_PAIRS", "HYD_BATCH_SINGLE_LAYOUT"])
 def test_pool_page_writes_refresh_oriented_layouts(monkeypatch, switch, fmt):
     """Pages overwritten in place (hyd_write_pool_pages) after the oriented copies were
     made render as a fresh context with the modified pool does: eagerly, and from a graph
