@@ -327,6 +327,12 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope 
   const uint32_t lim = pad * pad * pad - pad * pad - pad - 2u;   // largest base index of a tap cell
   const char *pool = static_cast<const char *>(a.pools[PERM]);
   const float fbs = float(vol.bs), off = float(vol.apron) - 0.5f;
+  // Tap reuse: a sample in its predecessor's cell takes that sample's taps
+  // instead of loading them.  It pays for FP16 quads (16-byte loads keep the
+  // data path busy); for u8 quads (8-byte loads) the registers and selects it
+  // costs outweigh the loads it saves (55.3 ms with, 52.7 ms without, per
+  // batch of the 96-view workload).
+  constexpr bool kReuse = !(FMT == HYD_FMT_U8 && PAIRS == 2);
   uint32_t poff = 0xffffffffu;   // tap offset of the last resolved sample (none yet)
   TapVals<FMT> pv{};             // its taps
   // fetched taps (layout order) of the group's samples; kept across groups so
@@ -358,7 +364,9 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope 
     bool done = false;
     // groups of UNROLL samples with all tap loads in flight together; slots
     // past kb repeat the brick's last sample with weight 0 (they composite
-    // nothing, and reuse that sample's taps, so they issue no loads)
+    // nothing; with reuse they take that sample's taps and issue no loads,
+    // without it they load them again: every slot loading keeps no taps live
+    // across groups, 86 VGPRs against 99 for loading only valid slots)
     const float klast = float(kb - 1);
     while (k < kb) {
       const float kf_base = float(k);   // sample indices stay exact in float (< 2^24)
@@ -373,7 +381,7 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope 
         wv[u] = u < nvalid ? w : 0.f;
         const float kf = fmaf(0.5f, w, kf0);
         tp[u] = local_tap<PERM>(fmaf(qdx, kf, qox), fmaf(qdy, kf, qoy), fmaf(qdz, kf, qoz), fpad, lim, base);
-        fresh[u] = tp[u].off != (u == 0 ? poff : tp[u - 1].off);
+        fresh[u] = !kReuse || tp[u].off != (u == 0 ? poff : tp[u - 1].off);
         if (fresh[u]) {
           if constexpr (PAIRS == 2) tv[u] = fetch_quads<FMT>(pool, tp[u].off);
           else if constexpr (PAIRS) tv[u] = fetch_pairs<FMT, PAD>(pool, tp[u].off, pad);
@@ -387,11 +395,13 @@ __device__ void march(const BatchArgs &a, const BatchVolDev &vol, const TfSlope 
         // that the compiler cannot fold this select into the fetch branch (as
         // "copy the previous taps, then load": the copy would wait for the
         // previous sample's loads)
-        uint32_t o = tp[u].off;
-        asm("" : "+v"(o));
-        const bool keep = o == (u == 0 ? poff : tp[u - 1].off);
+        if constexpr (kReuse) {
+          uint32_t o = tp[u].off;
+          asm("" : "+v"(o));
+          const bool keep = o == (u == 0 ? poff : tp[u - 1].off);
 #pragma unroll
-        for (int c = 0; c < tap_regs<FMT>(); ++c) tv[u].v[c] = keep ? (u == 0 ? pv.v[c] : tv[u - 1].v[c]) : tv[u].v[c];
+          for (int c = 0; c < tap_regs<FMT>(); ++c) tv[u].v[c] = keep ? (u == 0 ? pv.v[c] : tv[u - 1].v[c]) : tv[u].v[c];
+        }
         const bool live = r.v.w <= ert;
         composite(ext, fsc, smax, interpolate<PERM, FMT, PAIRS>(tv[u], tp[u]), wv[u], live, r.v);
         if (COUNT) count += (live && u < nvalid) ? 1u : 0u;
