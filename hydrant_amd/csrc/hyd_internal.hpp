@@ -77,9 +77,10 @@ struct Ctx {
   uint64_t batch_state_gen = 0;
   // owned device allocations (freed at destroy)
   void *own_pool = nullptr;
-  void *own_oriented = nullptr;   // the pool's y- and z-fastest layouts (batched renderer)
+  void *own_oriented = nullptr;   // the pool's oriented layouts (batched renderer)
   size_t oriented_cap = 0;
-  int oriented_pairs = 0;          // own_oriented holds the three row-pair (1) or FP16 quad (2) layouts
+  int oriented_pairs = 0;          // own_oriented holds the y- and z-fastest single-voxel layouts (0),
+                                   // the three row-pair layouts (1) or the three quad layouts (2)
   uint64_t oriented_gen = ~0ull;  // pool_gen the oriented layouts were made from
   int32_t *own_vaddr = nullptr;
   float4 *own_page = nullptr;
