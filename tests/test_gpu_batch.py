@@ -312,6 +312,41 @@ def test_batch_64_bricks_vs_fp64_oracle(monkeypatch, fmt):
 
 
 @pytest.mark.parametrize("fmt", [L.FMT_F16, L.FMT_U8])
+def test_large_bricks_unstaged_orientation(monkeypatch, fmt):
+    """128^3 bricks with a one-voxel apron (padded edge 130): for FP16 the two source
+    planes a workgroup of the orientation passes needs exceed 64 KB of LDS, so the quad,
+    row-pair and single-voxel layouts are written without LDS staging (u8 planes still
+    fit).  Every layout renders the x-fastest pool's image bit for bit, within 1e-3 of
+    the FP64 oracle."""
+    import torch
+    from hydrant_amd.hip import Context
+
+    wl = W.batched(sizes=(256,), n_views=3, res_choices=(64, 96), fmt=fmt, bs=128, apron=1, seed=93)
+    if fmt == L.FMT_F16:
+        assert 2 * (wl.bs + 2 * wl.apron) ** 2 * 2 > 65536   # two FP16 planes exceed the staging limit
+    ctx = Context(0)
+    b = _setup(ctx, wl)
+    ref = None
+    for switch in ("HYD_BATCH_SINGLE_LAYOUT", "HYD_BATCH_NO_PAIRS", "HYD_BATCH_NO_QUADS", None):
+        for s in ("HYD_BATCH_SINGLE_LAYOUT", "HYD_BATCH_NO_PAIRS", "HYD_BATCH_NO_QUADS"):
+            monkeypatch.delenv(s, raising=False)
+        if switch is not None:
+            monkeypatch.setenv(switch, "1")
+        ctx.upload_pool(wl.fmt, wl.bs + 2 * wl.apron, wl.pool)   # the switches are read when the pool changes
+        out = b.render(batch_config(8, 4, 8), wl.density, wl.ert)[0]
+        torch.cuda.synchronize()
+        if ref is None:
+            ref = out.clone()
+            errs = _view_errors(wl, out, b, 64, exact=True)
+            print(f"pad 130 fmt {fmt}: per-view max-abs vs FP64 {['%.2e' % e for e in errs]}", flush=True)
+            assert max(errs) <= 1e-3
+        else:
+            assert torch.equal(out, ref), f"layout switch {switch} changed the image"
+    b.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("fmt", [L.FMT_F16, L.FMT_U8])
 @pytest.mark.parametrize("switch", [None, "HYD_BATCH_NO_QUADS", "HYD_BATCH_NO_PAIRS", "HYD_BATCH_SINGLE_LAYOUT"])
 def test_pool_page_writes_refresh_oriented_layouts(monkeypatch, switch, fmt):
     """Pages overwritten in place (hyd_write_pool_pages) after the oriented copies were
